@@ -1,0 +1,109 @@
+"""ctypes binding of include/gdx_col.h. Loads the in-tree library of one model directory and fails
+loudly when it is missing: there is no fallback path."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_DIR = PKG / "lib"
+
+# status codes (include/gdx_col.h)
+STATUS = {
+    0: "OK", 1: "CHAIN_RATE", 2: "CHAIN_TOTAL", 3: "CHAIN_WAIT", 4: "CHAIN_CLOCK", 5: "CHAIN_PARTNER",
+    6: "CHAIN_MASS", 7: "CONTINUATION", 8: "RATE_MOMENTS", 9: "AUDIT", 10: "SCHEDULE", 11: "MORTON_OVERFLOW",
+    12: "MORTON_ARGS", 13: "ALLOC", 14: "NONFINITE", 15: "ARG", 16: "STATE", 17: "HIP",
+}
+DEFER_WRITEBACK = 1
+
+
+class GdxError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"gdx status {code} ({STATUS.get(code, '?')}): {message}")
+        self.code = code
+
+
+class ModelInfo(C.Structure):
+    _fields_ = [("model", C.c_char*32)] + [(n, C.c_int) for n in (
+        "n_k", "dim", "coag_kernel", "n_x", "n_y", "n_z", "local_groups", "col_bin_s", "morton_leaf_target",
+        "morton_max_level")] + [(n, C.c_double) for n in (
+        "x_min", "x_max", "y_min", "y_max", "z_min", "z_max", "h_search", "v_frag", "col_bath_max", "col_bath_eps",
+        "col_bath_alpha", "init_smin", "rho_0")]
+
+
+class Config(C.Structure):
+    _fields_ = [("n", C.c_int), ("total_dust_mass", C.c_double), ("seed", C.c_ulonglong), ("flags", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("waves", C.c_int), ("baths", C.c_int), ("owner_updates", C.c_longlong),
+                ("active_owners", C.c_longlong), ("events", C.c_longlong), ("chain_owners", C.c_longlong),
+                ("pair_evals", C.c_longlong), ("dt_col", C.c_double), ("clock_dyn", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+_SIGS = {
+    "gdx_col_model_info": [C.POINTER(ModelInfo)],
+    "gdx_col_create": [C.POINTER(Config), C.POINTER(P)],
+    "gdx_col_destroy": [P],
+    "gdx_col_bind_soa": [P, P, P, P, P, P],
+    "gdx_col_bind_swarm": [P, P],
+    "gdx_col_set_stream": [P, P],
+    "gdx_col_geometry": [P],
+    "gdx_col_check_state": [P],
+    "gdx_col_load_state": [P],
+    "gdx_col_store_state": [P],
+    "gdx_col_operator": [P, D, C.POINTER(Stats)],
+    "gdx_col_reset_controller": [P],
+    "gdx_col_synchronize": [P],
+    "gdx_col_active_count": [P, C.POINTER(I)],
+    "gdx_col_get_neighbors": [P, P, P],
+    "gdx_col_get_sites": [P, P, P, P],
+    "gdx_col_get_morton": [P, P, P],
+    "gdx_col_get_cached": [P, P],
+    "gdx_col_get_rng": [P, P],
+    "gdx_col_set_rng": [P, P],
+    "gdx_col_get_graph": [P, P],
+    "gdx_col_rate_pass": [P],
+    "gdx_morton_build": [P, P, I, C.c_float, C.c_float, C.c_float, C.c_float, I, I, I, C.POINTER(P)],
+    "gdx_morton_destroy": [P],
+    "gdx_morton_node_count": [P, C.POINTER(I)],
+    "gdx_morton_get": [P, P, P, P, P, P, P, P, P],
+    "gdx_morton_knn": [P, P, I, I, P, P],
+}
+EXPORTS = tuple(_SIGS) + ("gdx_col_last_error",)
+
+_cache: dict[str, C.CDLL] = {}
+
+
+def lib_path(model: str) -> Path:
+    return LIB_DIR / f"libgdx_col_{model}.so"
+
+
+def load(model: str) -> C.CDLL:
+    if model in _cache:
+        return _cache[model]
+    path = lib_path(model)
+    if not path.exists():
+        raise RuntimeError(f"collision library for model '{model}' is not built: {path} "
+                           f"(run python -m gamedev_amd.build)")
+    lib = C.CDLL(str(path))
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.gdx_col_last_error.argtypes = [P]
+    lib.gdx_col_last_error.restype = C.c_char_p
+    _cache[model] = lib
+    return lib
+
+
+def check(lib: C.CDLL, handle, code: int) -> None:
+    if code != 0:
+        msg = lib.gdx_col_last_error(handle)
+        raise GdxError(code, msg.decode() if msg else "")

@@ -1,0 +1,633 @@
+// C ABI of the collision library and the host driver of one operator call.
+//
+// gdx_col_operator follows evolve_local_collisions (inc/swarm/_col_chain.cuh:108-539): publish and
+// rate every group at tick zero, request per-group bath durations, schedule them on the power-of-two
+// lattice, then advance the earliest due groups wave by wave with a rating pass (tick > 0), the fused
+// screen + chain + audit kernel, and the host audit. Two host synchronisations per wave.
+#include "../../include/gdx_col.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "controller.h"
+#include "gdx_internal.h"
+
+struct gdx_col_ctx
+{
+    gdx::context c;
+};
+
+struct gdx_morton_ctx
+{
+    gdx::morton_index ix;
+    gdx::sort_scratch scratch;
+    gdx::dbuf<unsigned int> overflow;
+};
+
+namespace gdx {
+
+int state_flag(context &c);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guard(context *c, F &&f)
+{
+    try
+    {
+        f();
+        return GDX_COL_OK;
+    }
+    catch (const status_error &e)
+    {
+        (c ? c->last_error : g_last_error) = e.what();
+        return e.code;
+    }
+    catch (const std::bad_alloc &)
+    {
+        (c ? c->last_error : g_last_error) = "host allocation failure";
+        return GDX_COL_ERR_ALLOC;
+    }
+    catch (const std::exception &e)
+    {
+        (c ? c->last_error : g_last_error) = e.what();
+        return GDX_COL_ERR_HIP;
+    }
+}
+
+void require(bool ok, int code, const char *what)
+{
+    if (!ok) throw status_error(code, what);
+}
+
+gdx_due make_due(const context &c, const std::vector<int> &groups, const std::vector<double> &steps)
+{
+    gdx_due due{};
+    due.n = 0;
+    due.total = 0;
+    for (int g : groups)
+    {
+        if (c.group_cnt[g] <= 0) continue;
+        due.gid[due.n] = g;
+        due.beg[due.n] = c.group_beg[g];
+        due.cnt[due.n] = c.group_cnt[g];
+        due.dt[due.n] = steps.empty() ? 0.0 : steps[g];
+        due.total += c.group_cnt[g];
+        due.n++;
+    }
+    return due;
+}
+
+void copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "device to host copy");
+}
+
+}  // namespace
+
+// one operator call (inc/swarm/_col_chain.cuh:108-539)
+void run_operator(context &c, double duration, gdx_col_stats *out)
+{
+    require(c.state_loaded, GDX_COL_ERR_STATE, "collision state has not been loaded");
+    if (!(duration > 0) || !std::isfinite(duration)) throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
+    hipStream_t st = c.stream;
+    const double lambda0 = c.n / static_cast<double>(N_K) / c.total_mass;
+    gdx_col_stats stats{};
+    stats.active_owners = c.n_active;
+    hip_check(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long)*3, st), "zero counters");
+    launch_reset_scratch(c);
+
+    std::vector<gdx_raw_rate> raw_rate(RAW_BINS);
+    std::vector<gdx_raw_audit> raw_audit(RAW_BINS);
+    std::vector<int> counts(RAW_BINS), binmap(RAW_BINS);
+    std::vector<rate_bin> ratebin(RAW_BINS);
+    std::vector<audit_bin> audit(RAW_BINS);
+    int merged = 0;
+
+    // publish, rate and bin the due owners; returns the merged-bin count (_col_chain.cuh:190-243)
+    auto rate_wave = [&](const gdx_due &due) {
+        hip_check(hipMemsetAsync(c.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, st), "zero rate bins");
+        launch_publish(c, due);
+        launch_bath_rate(c, due, lambda0);
+        copy_d2h(raw_rate.data(), c.raw_rate.p, sizeof(gdx_raw_rate)*RAW_BINS, st);
+        hip_check(hipStreamSynchronize(st), "rating pass");
+        stats.pair_evals += (long long)due.total*N_K;
+        for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count;
+        int m = build_binmap(counts, binmap);
+        std::fill(ratebin.begin(), ratebin.begin() + m, rate_bin{});
+        for (int r = 0; r < RAW_BINS; ++r)
+        {
+            rate_bin &b = ratebin[binmap[r]];
+            b.mass += raw_rate[r].mass;
+            b.weighted_rate += raw_rate[r].weighted_rate;
+            b.weighted_change += raw_rate[r].weighted_change;
+            b.weighted_second += raw_rate[r].weighted_second;
+            b.owner_count += raw_rate[r].owner_count;
+            b.invalid_count += raw_rate[r].invalid_count;
+        }
+        return m;
+    };
+    auto bin_end = [&](int g) { return g + 1 < LOCAL_GROUPS ? binmap[(g + 1)*COL_BIN_S] : merged; };
+    auto requested_step = [&](int g) {
+        int first = binmap[g*COL_BIN_S];
+        return choose_bath(ratebin.data() + first, bin_end(g) - first, duration, c.state[g].limit_scale);
+    };
+
+    std::vector<int> all(LOCAL_GROUPS);
+    for (int g = 0; g < LOCAL_GROUPS; ++g) all[g] = g;
+    merged = rate_wave(make_due(c, all, {}));
+    std::vector<double> requested(LOCAL_GROUPS);
+    for (int g = 0; g < LOCAL_GROUPS; ++g) requested[g] = requested_step(g);
+    schedule sched(duration, requested, c.edges);
+    std::vector<double> steps(LOCAL_GROUPS);
+    for (int g = 0; g < LOCAL_GROUPS; ++g) steps[g] = sched.seconds(sched.step[g]);
+    std::vector<bool> passed(LOCAL_GROUPS, true);
+    std::vector<double> current = requested;
+    double dt_col = duration;
+
+    while (sched.time() < sched.end)
+    {
+        const uint64_t tick = sched.time();
+        std::vector<int> groups = sched.due(tick);
+        long long listed = 0;
+        for (int g : groups) listed += c.group_all[g];
+        if (listed == 0)
+        {
+            sched.advance(groups);
+            continue;
+        }
+        if (tick != 0)
+        {
+            merged = rate_wave(make_due(c, groups, {}));
+            for (int g : groups) current[g] = requested_step(g);
+            sched.adapt(tick, groups, current, passed);
+            for (int g : groups) steps[g] = sched.seconds(sched.step[g]);
+        }
+        gdx_due due = make_due(c, groups, steps);
+        hip_check(hipMemsetAsync(c.raw_audit.p, 0, sizeof(gdx_raw_audit)*RAW_BINS, st), "zero audit bins");
+        hip_check(hipMemsetAsync(c.err.p, 0, sizeof(int), st), "zero error");
+        launch_chain(c, due, lambda0);
+        int err = 0;
+        copy_d2h(&err, c.err.p, sizeof(int), st);
+        copy_d2h(raw_audit.data(), c.raw_audit.p, sizeof(gdx_raw_audit)*RAW_BINS, st);
+        hip_check(hipStreamSynchronize(st), "chain pass");
+        if (err) throw status_error(err, "local collision chain error " + std::to_string(err));
+
+        std::fill(audit.begin(), audit.begin() + merged, audit_bin{});
+        for (int r = 0; r < RAW_BINS; ++r)
+        {
+            const gdx_raw_audit &v = raw_audit[r];
+            audit_bin &b = audit[binmap[r]];
+            b.mass += v.mass;
+            b.predicted_f += v.predicted_f;
+            b.predicted_e += v.predicted_e;
+            b.predicted_var_f += v.predicted_var_f;
+            b.predicted_var_e += v.predicted_var_e;
+            b.predicted_g += v.predicted_g;
+            b.predicted_var_g += v.predicted_var_g;
+            double mw, mg;
+            std::memcpy(&mw, &v.maximum_weight_bits, sizeof(double));
+            std::memcpy(&mg, &v.maximum_g_jump_bits, sizeof(double));
+            b.maximum_weight = std::max(b.maximum_weight, mw);
+            b.maximum_g_jump = std::max(b.maximum_g_jump, mg);
+            b.touched += v.touched;
+            b.event_weight += v.event_weight;
+            b.growth += v.growth;
+            b.end_mass += v.end_mass;
+            b.owner_count += v.owner_count;
+            b.invalid_count += v.invalid_count;
+        }
+        for (int g : groups)
+        {
+            if (c.group_all[g] == 0) continue;
+            int first = binmap[g*COL_BIN_S], last = bin_end(g);
+            double mass = 0.0;
+            for (int b = first; b < last; ++b)
+            {
+                if (audit[b].invalid_count) throw status_error(GDX_COL_ERR_AUDIT, "invalid local audit state");
+                mass += audit[b].mass;
+            }
+            if (!(mass > 0)) continue;
+            bath_result r = finish_bath(audit.data() + first, last - first, c.state[g]);
+            passed[g] = !(r.activity_overshoot || r.distribution_overshoot || r.persistent_overshoot);
+            stats.baths++;
+        }
+        stats.waves++;
+        stats.owner_updates += due.total;
+        dt_col = duration;
+        for (int g : groups) dt_col = std::min(dt_col, steps[g]);
+        sched.advance(groups);
+    }
+    unsigned long long counters[3] = {0, 0, 0};
+    copy_d2h(counters, c.counters.p, sizeof(counters), st);
+    if (!(c.flags & GDX_COL_DEFER_WRITEBACK)) store_state(c);
+    hip_check(hipStreamSynchronize(st), "operator");
+    stats.events = (long long)counters[0];
+    stats.chain_owners = (long long)counters[1];
+    stats.pair_evals += (long long)counters[2];
+    stats.dt_col = dt_col;
+    stats.clock_dyn = duration;
+    if (out) *out = stats;
+}
+
+}  // namespace gdx
+
+using namespace gdx;
+
+extern "C" {
+
+int gdx_col_model_info(gdx_col_model *o)
+{
+    if (!o) return GDX_COL_ERR_ARG;
+    std::memset(o, 0, sizeof(*o));
+    std::strncpy(o->model, GDX_MODEL_NAME, sizeof(o->model) - 1);
+    o->n_k = N_K; o->dim = DIM; o->coag_kernel = COAG_KERNEL; o->n_x = N_X; o->n_y = N_Y; o->n_z = N_Z;
+    o->local_groups = LOCAL_GROUPS; o->col_bin_s = COL_BIN_S;
+    o->morton_leaf_target = MORTON_LEAF_TARGET; o->morton_max_level = MORTON_MAX_LEVEL;
+    o->x_min = X_MIN; o->x_max = X_MAX; o->y_min = Y_MIN; o->y_max = Y_MAX; o->z_min = Z_MIN; o->z_max = Z_MAX;
+    o->h_search = H_SEARCH; o->v_frag = V_FRAG; o->col_bath_max = COL_BATH_MAX; o->col_bath_eps = COL_BATH_EPS;
+    o->col_bath_alpha = COL_BATH_ALPHA; o->init_smin = INIT_SMIN; o->rho_0 = RHO_0;
+    return GDX_COL_OK;
+}
+
+const char *gdx_col_last_error(gdx_col_handle h)
+{
+    return h ? h->c.last_error.c_str() : g_last_error.c_str();
+}
+
+int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
+{
+    if (!cfg || !out) return GDX_COL_ERR_ARG;
+    *out = nullptr;
+    if (cfg->n <= 0 || cfg->n > INT_MAX / 3)
+    {
+        g_last_error = "representative count must be in [1, INT_MAX/3]";
+        return GDX_COL_ERR_ARG;
+    }
+    if (!(cfg->total_dust_mass > 0) || !std::isfinite(cfg->total_dust_mass))
+    {
+        g_last_error = "total dust mass must be positive and finite";
+        return GDX_COL_ERR_ARG;
+    }
+    gdx_col_ctx *h = nullptr;
+    int rc = guard(nullptr, [&] {
+        h = new gdx_col_ctx();
+        context &c = h->c;
+        c.n = cfg->n;
+        c.total_mass = cfg->total_dust_mass;
+        c.seed = cfg->seed;
+        c.flags = cfg->flags;
+        c.state.assign(LOCAL_GROUPS, context::bath_state{});
+        c.rng_p.alloc(size_t(c.n));
+        hip_check(hipMemset(c.rng_p.p, 0, sizeof(uint64_t)*c.n), "zero counters");
+        c.scr_min.alloc(LOCAL_GROUPS);
+        c.scr_max.alloc(LOCAL_GROUPS);
+        c.lower.alloc(LOCAL_GROUPS);
+        c.upper.alloc(LOCAL_GROUPS);
+        hip_check(hipMemset(c.lower.p, 0, sizeof(double)*LOCAL_GROUPS), "zero bounds");
+        hip_check(hipMemset(c.upper.p, 0, sizeof(double)*LOCAL_GROUPS), "zero bounds");
+        c.raw_rate.alloc(RAW_BINS);
+        c.raw_audit.alloc(RAW_BINS);
+        c.err.alloc(1);
+        c.counters.alloc(3);
+    });
+    if (rc != GDX_COL_OK)
+    {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return GDX_COL_OK;
+}
+
+int gdx_col_destroy(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    (void)hipStreamSynchronize(h->c.stream);
+    delete h;
+    return GDX_COL_OK;
+}
+
+int gdx_col_bind_soa(gdx_col_handle h, const double *x, const double *y, const double *z, double *size, double *numr)
+{
+    if (!h || !x || !y || !z || !size || !numr) return GDX_COL_ERR_ARG;
+    bound_state &u = h->c.user;
+    u = bound_state{};
+    u.x = {reinterpret_cast<const char *>(x), 8};
+    u.y = {reinterpret_cast<const char *>(y), 8};
+    u.z = {reinterpret_cast<const char *>(z), 8};
+    u.size = {reinterpret_cast<const char *>(size), 8};
+    u.numr = {reinterpret_cast<const char *>(numr), 8};
+    u.bound = true;
+    h->c.geometry_valid = h->c.state_loaded = false;
+    return GDX_COL_OK;
+}
+
+int gdx_col_bind_swarm(gdx_col_handle h, void *records)
+{
+    if (!h || !records) return GDX_COL_ERR_ARG;
+    const char *b = static_cast<const char *>(records);
+    bound_state &u = h->c.user;
+    u = bound_state{};
+    // swarm layout (inc/swarm/const_defs.cuh:194-203): position xyz, velocity xyz, par_size, par_numr
+    u.x = {b + 0, 64}; u.y = {b + 8, 64}; u.z = {b + 16, 64};
+    u.vx = {b + 24, 64}; u.vy = {b + 32, 64}; u.vz = {b + 40, 64};
+    u.size = {b + 48, 64}; u.numr = {b + 56, 64};
+    u.bound = true;
+    h->c.geometry_valid = h->c.state_loaded = false;
+    return GDX_COL_OK;
+}
+
+int gdx_col_set_stream(gdx_col_handle h, void *stream)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    h->c.stream = static_cast<hipStream_t>(stream);
+    return GDX_COL_OK;
+}
+
+int gdx_col_geometry(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(h->c.user.bound, GDX_COL_ERR_STATE, "no particle state is bound");
+        geometry_epoch(h->c);
+    });
+}
+
+int gdx_col_check_state(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(h->c.user.bound, GDX_COL_ERR_STATE, "no particle state is bound");
+        int bad = state_flag(h->c);
+        if (bad) throw status_error(GDX_COL_ERR_NONFINITE,
+            "non-finite particle state before collision search at particle " + std::to_string(bad - 1));
+    });
+}
+
+int gdx_col_load_state(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] { load_state(h->c); });
+}
+
+int gdx_col_store_state(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        store_state(h->c);
+        hip_check(hipStreamSynchronize(h->c.stream), "store state");
+    });
+}
+
+int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] { run_operator(h->c, duration, out); });
+}
+
+int gdx_col_reset_controller(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    h->c.state.assign(LOCAL_GROUPS, context::bath_state{});
+    return GDX_COL_OK;
+}
+
+int gdx_col_synchronize(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] { hip_check(hipStreamSynchronize(h->c.stream), "synchronize"); });
+}
+
+int gdx_col_active_count(gdx_col_handle h, int *n_active)
+{
+    if (!h || !n_active) return GDX_COL_ERR_ARG;
+    if (!h->c.geometry_valid) { h->c.last_error = "collision geometry has not been built"; return GDX_COL_ERR_STATE; }
+    *n_active = h->c.n_active;
+    return GDX_COL_OK;
+}
+
+int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(c.geometry_valid, GDX_COL_ERR_STATE, "collision geometry has not been built");
+        const size_t na = size_t(c.n_active);
+        std::vector<int> nbr(na*N_K), slot_pidx(na), pidx_slot(c.n);
+        std::vector<double> meas(na);
+        hipStream_t st = c.stream;
+        if (na)
+        {
+            copy_d2h(nbr.data(), c.nbr.p, sizeof(int)*na*N_K, st);
+            copy_d2h(meas.data(), c.measure.p, sizeof(double)*na, st);
+            copy_d2h(slot_pidx.data(), c.slot_pidx.p, sizeof(int)*na, st);
+        }
+        copy_d2h(pidx_slot.data(), c.pidx_slot.p, sizeof(int)*c.n, st);
+        hip_check(hipStreamSynchronize(st), "read neighbours");
+        for (int p = 0; p < c.n; ++p)
+        {
+            int s = pidx_slot[p];
+            for (int k = 0; k < N_K; ++k)
+            {
+                int v = -1;
+                if (s >= 0)
+                {
+                    int j = nbr[size_t(s)*N_K + k];
+                    v = j < 0 ? -1 : 3*slot_pidx[j];
+                }
+                if (codes) codes[size_t(p)*N_K + k] = v;
+            }
+            if (measure) measure[p] = s >= 0 ? meas[s] : 0.0;
+        }
+    });
+}
+
+int gdx_col_get_sites(gdx_col_handle h, float *pts4, unsigned char *active, unsigned char *group)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(c.geometry_valid, GDX_COL_ERR_STATE, "collision geometry has not been built");
+        if (pts4) copy_d2h(pts4, c.pts.p, sizeof(float4)*c.n, c.stream);
+        if (active) copy_d2h(active, c.active.p, c.n, c.stream);
+        if (group) copy_d2h(group, c.group.p, c.n, c.stream);
+        hip_check(hipStreamSynchronize(c.stream), "read sites");
+    });
+}
+
+int gdx_col_get_morton(gdx_col_handle h, unsigned long long *keys, int *idx)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(c.geometry_valid, GDX_COL_ERR_STATE, "collision geometry has not been built");
+        if (keys) copy_d2h(keys, c.index.keys.p, sizeof(uint64_t)*c.n, c.stream);
+        if (idx) copy_d2h(idx, c.index.idx.p, sizeof(int)*c.n, c.stream);
+        hip_check(hipStreamSynchronize(c.stream), "read morton");
+    });
+}
+
+int gdx_col_get_cached(gdx_col_handle h, double *cached4)
+{
+    if (!h || !cached4) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(c.state_loaded, GDX_COL_ERR_STATE, "collision state has not been loaded");
+        dbuf<double> tmp;
+        tmp.alloc(size_t(4)*c.n);
+        hip_check(hipMemsetAsync(tmp.p, 0, sizeof(double)*4*c.n, c.stream), "zero cached");
+        launch_gather_cached(c, tmp.p);
+        copy_d2h(cached4, tmp.p, sizeof(double)*4*c.n, c.stream);
+        hip_check(hipStreamSynchronize(c.stream), "read cached");
+    });
+}
+
+int gdx_col_get_rng(gdx_col_handle h, unsigned long long *counters)
+{
+    if (!h || !counters) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        copy_d2h(counters, c.rng_p.p, sizeof(uint64_t)*c.n, c.stream);
+        if (c.state_loaded && c.n_active)
+        {
+            std::vector<uint64_t> slot(c.n_active);
+            std::vector<int> sp(c.n_active);
+            copy_d2h(slot.data(), c.rng.p, sizeof(uint64_t)*c.n_active, c.stream);
+            copy_d2h(sp.data(), c.slot_pidx.p, sizeof(int)*c.n_active, c.stream);
+            hip_check(hipStreamSynchronize(c.stream), "read rng");
+            for (int s = 0; s < c.n_active; ++s) counters[sp[s]] = slot[s];
+        }
+        hip_check(hipStreamSynchronize(c.stream), "read rng");
+    });
+}
+
+int gdx_col_set_rng(gdx_col_handle h, const unsigned long long *counters)
+{
+    if (!h || !counters) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        hip_check(hipMemcpyAsync(c.rng_p.p, counters, sizeof(uint64_t)*c.n, hipMemcpyHostToDevice, c.stream), "write rng");
+        if (c.state_loaded && c.n_active)
+        {
+            std::vector<int> sp(c.n_active);
+            copy_d2h(sp.data(), c.slot_pidx.p, sizeof(int)*c.n_active, c.stream);
+            hip_check(hipStreamSynchronize(c.stream), "read slots");
+            std::vector<uint64_t> slot(c.n_active);
+            for (int s = 0; s < c.n_active; ++s) slot[s] = counters[sp[s]];
+            hip_check(hipMemcpyAsync(c.rng.p, slot.data(), sizeof(uint64_t)*c.n_active, hipMemcpyHostToDevice, c.stream), "write rng");
+        }
+        hip_check(hipStreamSynchronize(c.stream), "write rng");
+    });
+}
+
+int gdx_col_get_graph(gdx_col_handle h, unsigned long long *rows)
+{
+    if (!h || !rows) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(c.geometry_valid, GDX_COL_ERR_STATE, "collision geometry has not been built");
+        copy_d2h(rows, c.graph.p, sizeof(unsigned long long)*LOCAL_GROUPS, c.stream);
+        hip_check(hipStreamSynchronize(c.stream), "read graph");
+    });
+}
+
+int gdx_col_rate_pass(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(c.state_loaded, GDX_COL_ERR_STATE, "collision state has not been loaded");
+        std::vector<int> all(LOCAL_GROUPS);
+        for (int g = 0; g < LOCAL_GROUPS; ++g) all[g] = g;
+        gdx_due due = make_due(c, all, {});
+        launch_reset_scratch(c);
+        hip_check(hipMemsetAsync(c.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, c.stream), "zero rate bins");
+        launch_publish(c, due);
+        launch_bath_rate(c, due, c.n / static_cast<double>(N_K) / c.total_mass);
+        hip_check(hipStreamSynchronize(c.stream), "rating pass");
+    });
+}
+
+// ---- stand-alone Morton index ----
+int gdx_morton_build(const void *points4, const unsigned char *active, int n, float ox, float oy, float oz,
+                     float width, int dim, int leaf_target, int max_level, gdx_morton_handle *out)
+{
+    if (!out || !points4) return GDX_COL_ERR_ARG;
+    *out = nullptr;
+    gdx_morton_ctx *m = nullptr;
+    int rc = guard(nullptr, [&] {
+        m = new gdx_morton_ctx();
+        morton_build(m->ix, static_cast<const float4 *>(points4), active, n, ox, oy, oz, width, dim, leaf_target,
+                     max_level, m->scratch, nullptr);
+        m->overflow.alloc(1);
+    });
+    if (rc != GDX_COL_OK) { delete m; return rc; }
+    *out = m;
+    return GDX_COL_OK;
+}
+
+int gdx_morton_destroy(gdx_morton_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    delete h;
+    return GDX_COL_OK;
+}
+
+int gdx_morton_node_count(gdx_morton_handle h, int *nodes)
+{
+    if (!h || !nodes) return GDX_COL_ERR_ARG;
+    *nodes = h->ix.node_count;
+    return GDX_COL_OK;
+}
+
+int gdx_morton_get(gdx_morton_handle h, unsigned long long *keys, int *idx, float *lower3, float *width, int *begin,
+                   int *count, int *child8, int *child_count)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(nullptr, [&] {
+        const morton_index &ix = h->ix;
+        if (keys) copy_d2h(keys, ix.keys.p, sizeof(uint64_t)*ix.n, nullptr);
+        if (idx) copy_d2h(idx, ix.idx.p, sizeof(int)*ix.n, nullptr);
+        std::vector<gdx_node> nodes(ix.node_count);
+        copy_d2h(nodes.data(), ix.nodes.p, sizeof(gdx_node)*ix.node_count, nullptr);
+        hip_check(hipStreamSynchronize(nullptr), "read index");
+        for (int i = 0; i < ix.node_count; ++i)
+        {
+            const gdx_node &nd = nodes[i];
+            if (lower3) { lower3[3*i] = nd.lx; lower3[3*i + 1] = nd.ly; lower3[3*i + 2] = nd.lz; }
+            if (width) width[i] = nd.width;
+            if (begin) begin[i] = nd.begin;
+            if (count) count[i] = nd.count;
+            if (child_count) child_count[i] = nd.child_count & 15;
+            if (child8)
+            {
+                // children were allocated in code order; bits 8..15 hold their codes
+                int mask = (nd.child_count >> 8) & 0xff, ch = 0;
+                for (int k = 0; k < 8; ++k) child8[8*i + k] = (mask >> k & 1) ? nd.first_child + ch++ : -1;
+            }
+        }
+    });
+}
+
+int gdx_morton_knn(gdx_morton_handle h, const void *queries4, int nq, int k, int *codes, float *d2)
+{
+    if (!h || !queries4 || !codes || !d2) return GDX_COL_ERR_ARG;
+    return guard(nullptr, [&] {
+        hip_check(hipMemset(h->overflow.p, 0, sizeof(unsigned int)), "zero overflow");
+        knn_query(h->ix, static_cast<const float4 *>(queries4), nq, k, codes, d2, h->overflow.p, nullptr);
+        unsigned int ovf = 0;
+        copy_d2h(&ovf, h->overflow.p, sizeof(ovf), nullptr);
+        hip_check(hipStreamSynchronize(nullptr), "knn");
+        if (ovf) throw status_error(GDX_COL_ERR_MORTON_OVERFLOW, "Morton traversal stack overflow");
+    });
+}
+
+}  // extern "C"

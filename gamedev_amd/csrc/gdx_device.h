@@ -1,0 +1,395 @@
+// Device-side physics, neighbour codes and random numbers of the collision operator (gfx950).
+//
+// Floating-point contraction is disabled for everything in this header: each expression is
+// evaluated in the reference's source order, one rounding per operation, so FP32 distances are
+// bit-exact and FP64 pair terms differ from a host evaluation only through library transcendentals.
+#ifndef GDX_DEVICE_H
+#define GDX_DEVICE_H
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cstdint>
+
+#include "gdx_model.h"
+
+#pragma clang fp contract(off)
+
+namespace gdx {
+
+// ---------------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). Counter words {draw lo, draw hi, seed hi,
+// tag}, key {particle index, seed lo}: the stream of a particle does not depend on storage order.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r)
+    {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo0 = 0xD2511F53u*c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t lo1 = 0xCD9E8D57u*c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// uniform in (0,1) from 52 random bits: (m + 1/2) 2^-52
+__device__ __forceinline__ double uniform01(uint32_t pidx, uint64_t seed, uint64_t counter)
+{
+    uint32_t c[4] = {uint32_t(counter), uint32_t(counter >> 32), uint32_t(seed >> 32), 0x47445843u};
+    philox10(c, pidx, uint32_t(seed));
+    uint64_t m = (uint64_t(c[1]) << 32) | c[0];
+    return (double(m >> 12) + 0.5)*0x1p-52;
+}
+
+// _get_col_uniform (inc/swarm/_col_rates.cuh:145-149)
+__device__ __forceinline__ double col_uniform(uint32_t pidx, uint64_t seed, uint64_t &counter)
+{
+    double u = uniform01(pidx, seed, counter);
+    counter++;
+    return fmin(u, 0x1.fffffffffffffp-1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// disk and grain profiles (inc/swarm/param_phys.cuh:16-210, param_grid.cuh:23-27)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double grain_mass(double size) { return PI*RHO_0*size*size*size / 6.0; }
+__device__ __forceinline__ double omegaK(double R) { return sqrt(G*M_S / R / R / R); }
+__device__ __forceinline__ double hg(double R) { return ASPR_0*pow(R / R_0, 0.5*(IDX_Q + 1.0)); }
+__device__ __forceinline__ double eta(double R, double Z, double h)
+{ return -0.5*((IDX_P + 0.5*IDX_Q - 1.5)*h*h + IDX_Q*(1.0 - R / sqrt(R*R + Z*Z))); }
+__device__ __forceinline__ double gas_strat(double R, double Z, double h)
+{ return exp((R / sqrt(R*R + Z*Z) - 1.0) / (h*h)); }
+__device__ __forceinline__ double sigma_g(double R) { return SIGMA_0*pow(R / R_0, IDX_P); }
+__device__ __forceinline__ double sound_speed(double R, double h) { return h*omegaK(R)*R; }
+__device__ __forceinline__ double alpha_of(double R, double h)
+{
+#ifdef CONST_NU
+    return NU / (h*h*R*R*omegaK(R));
+#else
+    (void)R; (void)h;
+    return ALPHA;
+#endif
+}
+__device__ __forceinline__ double cyl_R(double y, double z) { return (N_Z > 1) ? y*sin(z) : y; }
+__device__ __forceinline__ double cyl_Z(double y, double z) { return (N_Z > 1) ? y*cos(z) : 0.0; }
+
+// _get_re_inv_sqrt (inc/swarm/_collision.cuh:136-150)
+__device__ __forceinline__ double re_inv_sqrt(double alpha, double sig)
+{
+#ifdef CODE_UNIT
+    double alpha_0 = alpha_of(R_0, ASPR_0);
+    double reynolds = REYNOLDS_0*(alpha / alpha_0)*(sig / SIGMA_0);
+#else
+    double reynolds = 0.5*alpha*sig*X_SEC / M_MOL;
+#endif
+    return 1.0 / sqrt(reynolds);
+}
+
+// owner environment: every owner-only factor of _get_vrel_pair (_collision.cuh:285-317), in order
+__device__ __forceinline__ gdx_env make_env(double y, double z)
+{
+    double R = cyl_R(y, z);
+    double Z = cyl_Z(y, z);
+    double h = hg(R);
+    double omega = omegaK(R);
+    double strat = gas_strat(R, Z, h);
+    double c_s = sound_speed(R, h);
+    double alpha = alpha_of(R, h);
+    gdx_env e;
+    e.Z = Z;
+    e.omega = omega;
+    e.vn = -eta(R, Z, h)*R*omega;
+#ifndef CONST_ST
+    e.radial = pow(R / R_0, IDX_P);
+#else
+    e.radial = 1.0;
+#endif
+    e.strat = strat;
+    e.cs = c_s;
+    e.re_inv_sqrt = re_inv_sqrt(alpha, sigma_g(R)*strat);
+    e.vg_sq = 1.5*alpha*c_s*c_s;
+    return e;
+}
+
+// _get_stokes for analytic gas (param_phys.cuh:170-206): divide by the radial and vertical scalings
+__device__ __forceinline__ double stokes(const gdx_env &e, double size)
+{
+    double st = STOKES_0*(size / S_0);
+#ifndef CONST_ST
+    st /= e.radial;
+    st /= e.strat;
+#endif
+    return st;
+}
+
+// Ormel & Cuzzi (2007) turbulent relative speed, regimes of _get_vrel_t (_collision.cuh:152-273)
+__device__ __forceinline__ double vrel_t(const gdx_env &e, double stokes_i, double stokes_j)
+{
+    double ris = e.re_inv_sqrt;
+    double vg_sq = e.vg_sq;
+    double sl, ss;
+    if (stokes_i >= stokes_j) { sl = stokes_i; ss = stokes_j; }
+    else { sl = stokes_j; ss = stokes_i; }
+    double eps = ss / sl;
+    const double y_a = 1.6;
+    double y_s = 1.6015125;
+    y_s += -0.63119577*sl;
+    y_s +=  0.32938936*sl*sl;
+    y_s += -0.29847604*sl*sl*sl;
+    double v;
+    if (sl < 0.2*ris)
+    {
+        v = vg_sq*(sl - ss)*(sl - ss) / ris;
+    }
+    else if (sl < ris / y_a)
+    {
+        v = vg_sq*(sl - ss) / (sl + ss);
+        v *= (sl / (1.0 + ris / sl) - ss / (1.0 + ris / ss));
+    }
+    else if (sl < 5.0*ris)
+    {
+        double coeff = (sl - ss) / (sl + ss);
+        coeff *= (sl / (1.0 + y_a) - ss*ss / (ss + y_a*sl));
+        coeff += 2.0*(y_a*sl - ris) + sl / (1.0 + y_a);
+        coeff -= sl*sl / (sl + ris);
+        coeff += ss*ss / (y_a*sl + ss);
+        coeff -= ss*ss / (ss + ris);
+        v = vg_sq*coeff;
+    }
+    else if (sl < 0.2)
+    {
+        v = vg_sq*sl;
+        v *= (2.0*y_a - (1.0 + eps) + 2.0 / (1.0 + eps)*(1.0 / (1.0 + y_a) + eps*eps*eps / (y_a + eps)));
+    }
+    else if (sl < 1.0)
+    {
+        v = vg_sq*sl;
+        v *= (2.0*y_s - (1.0 + eps) + 2.0 / (1.0 + eps)*(1.0 / (1.0 + y_s) + eps*eps*eps / (y_s + eps)));
+    }
+    else
+    {
+        v = vg_sq*(1.0 / (1.0 + sl) + 1.0 / (1.0 + ss));
+    }
+    // a negative square is an invalid state: the NaN propagates to the chain's validity checks
+    return v < 0.0 ? __builtin_nan("") : sqrt(v);
+}
+
+// query-local drift + settling + turbulence (+ Brownian) speed (_collision.cuh:275-325, _col_rates.cuh:127-141)
+__device__ __forceinline__ double vrel_pair(const gdx_env &e, double size_i, double size_j)
+{
+    double si = stokes(e, size_i);
+    double sj = stokes(e, size_j);
+    double fi = 1.0 / (1.0 + si*si);
+    double fj = 1.0 / (1.0 + sj*sj);
+    double dvr = 2.0*e.vn*(si*fi - sj*fj);
+    double dvphi = e.vn*(fi - fj);
+    double dvz = e.Z*e.omega*(fmin(si, 0.5) - fmin(sj, 0.5));
+    double vt = vrel_t(e, si, sj);
+    double vb = 0.0;
+#ifndef CODE_UNIT
+    double mi = grain_mass(size_i);
+    double mj = grain_mass(size_j);
+    vb = fmin(sqrt(8.0*e.cs*e.cs*M_MOL*(mi + mj) / (PI*mi*mj)), e.cs);
+#endif
+    return sqrt(dvr*dvr + dvphi*dvphi + dvz*dvz + vt*vt + vb*vb);
+}
+
+// KNN ball measure (_collision.cuh:66-115)
+__device__ __forceinline__ double ball_measure(double y, double z, double radius)
+{
+#ifdef GDX_UNIT_MEASURE
+    (void)y; (void)z;
+    return radius > 0.0 ? 1.0 : 0.0;
+#else
+    if (radius <= 0.0) return 0.0;
+    if (N_X == 1 && N_Z == 1)
+    {
+        double R_lo = fmax(Y_MIN, y - radius);
+        double R_hi = fmin(Y_MAX, y + radius);
+        return (R_hi > R_lo) ? PI*(R_hi*R_hi - R_lo*R_lo) : 0.0;
+    }
+    constexpr int dim = 1 + int(N_X > 1) + int(N_Z > 1);
+    double measure = (dim == 1) ? 2.0*radius : (dim == 2) ? PI*radius*radius
+        : 4.0*PI*radius*radius*radius / 3.0;
+    double distances[4] = {y - Y_MIN, Y_MAX - y, 1.0e+100, 1.0e+100};
+    if (N_Z > 1)
+    {
+        distances[2] = y*(z - Z_MIN);
+        distances[3] = y*(Z_MAX - z);
+    }
+    constexpr int bounds = (N_Z > 1) ? 4 : 2;
+#pragma unroll
+    for (int b = 0; b < bounds; b++)
+    {
+        double d = fmax(0.0, distances[b]);
+        if (d >= radius) continue;
+        if (dim == 1)
+        {
+            measure *= 0.5*(1.0 + d / radius);
+        }
+        else if (dim == 2)
+        {
+            double cap = radius*radius*acos(d / radius) - d*sqrt(radius*radius - d*d);
+            measure *= 1.0 - cap / (PI*radius*radius);
+        }
+        else
+        {
+            double cap = PI*(radius - d)*(radius - d)*(2.0*radius + d) / 3.0;
+            measure *= 1.0 - cap / (4.0*PI*radius*radius*radius / 3.0);
+        }
+    }
+    if (N_X == 1) measure *= 2.0*PI*cyl_R(y, z);
+    return measure;
+#endif
+}
+
+// ---------------------------------------------------------------------------------------------
+// outcomes (inc/swarm/_col_event.cuh:15-109)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double sticking_packet(double q)
+{ return q > 0.0 && q <= 1.0e-06 ? fmax(1.0, floor(1.0e-04 / q)) : 1.0; }
+__device__ __forceinline__ double mass_ratio(double size_i, double size_j)
+{ double r = size_j / size_i; return r*r*r; }
+
+__device__ __forceinline__ double outcome_moments(double si, double sj, bool high_speed,
+                                                  double &mean, double &second, double &maximum)
+{
+    double q = mass_ratio(si, sj);
+    double Gp = sticking_packet(q);
+    if (!high_speed)
+    {
+        mean = log1p(Gp*q) / 3.0;
+        second = mean*mean;
+        maximum = mean;
+        return 1.0 / Gp;
+    }
+    if (q <= 0.1)
+    {
+        double remnant = (1.0 - q) / Gp;
+        double debris = q;
+        double factor = remnant + debris;
+        double jr = -log1p(-Gp*q) / 3.0;
+        double jd = -log(q) / 3.0;
+        mean = (remnant*jr + debris*jd) / factor;
+        second = (remnant*jr*jr + debris*jd*jd) / factor;
+        maximum = fmax(jr, jd);
+        return factor;
+    }
+    double L = 0.5*log(si / INIT_SMIN);
+    if (L < 1.0e-03)
+    {
+        mean = L - L*L / 6.0 + L*L*L*L / 360.0;
+        second = L*L*(4.0/3.0 - L / 3.0 + L*L / 90.0 + L*L*L / 180.0);
+    }
+    else
+    {
+        double tail = exp(-L) / (-expm1(-L));
+        mean = 2.0*(1.0 - L*tail);
+        second = 8.0 - (4.0*L*L + 8.0*L)*tail;
+    }
+    maximum = 2.0*L;
+    return 1.0;
+}
+
+__device__ __forceinline__ double sample_outcome(double si, double sj, bool high_speed, double u)
+{
+    double q = mass_ratio(si, sj);
+    double Gp = sticking_packet(q);
+    if (!high_speed) return cbrt(si*si*si + Gp*sj*sj*sj);
+    if (q <= 0.1)
+    {
+        double factor = (1.0 - q) / Gp + q;
+        if (u < q / factor) return sj;
+        return si*cbrt(1.0 - Gp*q);
+    }
+    double lower = sqrt(INIT_SMIN);
+    double root = lower + u*(sqrt(si) - lower);
+    return fmin(si, fmax(INIT_SMIN, root*root));
+}
+
+// one pair term of _get_col_chain_rate (_col_rates.cuh:167-223) before division by the measure
+__device__ __forceinline__ double pair_numer(const gdx_env &e, double hg_i, double hg_j, double size_i,
+                                             double size_j, double numr_j, double lambda_0, double &vrel)
+{
+    vrel = 0.0;
+    if constexpr (COAG_KERNEL == 0)
+    {
+        (void)e; (void)hg_i; (void)hg_j; (void)size_i; (void)size_j;
+        return lambda_0*numr_j;
+    }
+    else if constexpr (COAG_KERNEL == 1)
+    {
+        (void)e; (void)hg_i; (void)hg_j;
+        return lambda_0*numr_j*(grain_mass(size_i) + grain_mass(size_j));
+    }
+    else if constexpr (COAG_KERNEL == 2)
+    {
+        (void)e; (void)hg_i; (void)hg_j;
+        return lambda_0*numr_j*grain_mass(size_i)*grain_mass(size_j);
+    }
+    else
+    {
+        vrel = vrel_pair(e, size_i, size_j);
+        double rate = numr_j*vrel*PI*(size_i + size_j)*(size_i + size_j) / 4.0;
+        if constexpr (N_Z == 1) rate /= sqrt(2.0*PI*(hg_i*hg_i + hg_j*hg_j));
+        return rate;
+    }
+}
+
+// moving logarithmic size bin (inc/swarm/_col_sizes.cuh:12-21)
+__device__ __forceinline__ int size_bin(double size, double lower, double upper)
+{
+    if (!isfinite(size) || !(size > 0)) return 0;
+    double fraction = log(size / lower) / log(upper / lower);
+    return max(0, min(COL_BIN_S - 1, static_cast<int>(floor(fraction*COL_BIN_S))));
+}
+
+// ---------------------------------------------------------------------------------------------
+// wavefront segment helpers: a segment is SEG consecutive lanes holding one owner's K slots
+// ---------------------------------------------------------------------------------------------
+template <int W>
+__device__ __forceinline__ double seg_sum(double v)
+{
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, W);
+    return v;
+}
+template <int W>
+__device__ __forceinline__ double seg_max(double v)
+{
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, W));
+    return v;
+}
+template <int W>
+__device__ __forceinline__ int seg_last(int v)
+{
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1)
+    {
+        int o = __shfl_xor(v, m, W);
+        v = (v == -2 || o == -2) ? -2 : (v > o ? v : o);
+    }
+    return v;
+}
+template <int W>
+__device__ __forceinline__ int seg_and(int v)
+{
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) v = v & __shfl_xor(v, m, W);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double v) { return static_cast<unsigned long long>(__double_as_longlong(v)); }
+__device__ __forceinline__ double bitsd(unsigned long long v) { return __longlong_as_double(static_cast<long long>(v)); }
+
+}  // namespace gdx
+
+#endif  // GDX_DEVICE_H

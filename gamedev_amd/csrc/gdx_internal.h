@@ -1,0 +1,168 @@
+// Host-side internals of the collision library: device buffers, the operator context and the
+// launch wrappers implemented by geometry.hip and operator.hip.
+#ifndef GDX_INTERNAL_H
+#define GDX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gdx_model.h"
+
+namespace gdx {
+
+// library status, mapped 1:1 onto the reference's failure paths (see include/gdx_col.h)
+struct status_error : std::runtime_error
+{
+    int code;
+    status_error(int c, const std::string &what) : std::runtime_error(what), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what);
+
+// owning device allocation
+template <typename T>
+struct dbuf
+{
+    T *p = nullptr;
+    size_t n = 0;
+    dbuf() = default;
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    ~dbuf() { release(); }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) count = 1;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&p), sizeof(T)*count), "device allocation");
+        n = count;
+    }
+    // grow preserving the first `keep` entries
+    void grow(size_t count, size_t keep, hipStream_t s)
+    {
+        if (count <= n && p) return;
+        T *q = nullptr;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&q), sizeof(T)*count), "device allocation");
+        if (p && keep) hip_check(hipMemcpyAsync(q, p, sizeof(T)*keep, hipMemcpyDeviceToDevice, s), "grow copy");
+        hip_check(hipStreamSynchronize(s), "grow sync");
+        release();
+        p = q;
+        n = count;
+    }
+};
+
+// particle-order view of one bound state field: element i lives at base + i*stride bytes
+struct field
+{
+    const char *base = nullptr;
+    long long stride = 8;
+};
+struct bound_state
+{
+    field x, y, z, vx, vy, vz, size, numr;   // vx..vz optional (finiteness screen only)
+    bool bound = false;
+};
+
+// reusable scratch for radix sorts and scans
+struct sort_scratch
+{
+    dbuf<uint64_t> keys_alt;
+    dbuf<int> vals_alt;
+    dbuf<unsigned int> hist;
+    dbuf<unsigned int> partial;
+};
+
+struct morton_index
+{
+    int n = 0, dim = 3, max_level = 20, leaf_target = 128;
+    float ox = 0, oy = 0, oz = 0, width = 1;
+    dbuf<uint64_t> keys;       // sorted keys
+    dbuf<int> idx;             // sorted record -> source index
+    dbuf<float4> rec;          // sorted records {x, y, z, code}; code sign bit marks inactive
+    dbuf<gdx_node> nodes;
+    int node_count = 0;
+    int leaf_count = 0;
+};
+
+struct context
+{
+    // configuration
+    int n = 0;
+    double total_mass = 0.0;
+    uint64_t seed = 0;
+    int flags = 0;
+    hipStream_t stream = nullptr;
+    bound_state user;
+
+    // geometry epoch (particle order)
+    dbuf<float4> pts;          // {x, y, z, search cap}
+    dbuf<unsigned char> active;
+    dbuf<unsigned char> group;
+    dbuf<int> bad;             // first non-finite particle + 1
+    morton_index index;
+    sort_scratch scratch;
+    dbuf<int> pidx_slot;       // particle -> slot (-1 inactive)
+    dbuf<int> slot_pidx;       // slot -> particle
+    dbuf<unsigned long long> graph;   // per group: 64-bit dependency mask
+    dbuf<unsigned int> overflow;
+    int n_active = 0;
+    std::vector<int> group_beg, group_cnt, group_all;   // slot ranges of active owners; all particles per group
+    std::vector<uint32_t> edges;                        // LOCAL_GROUPS x LOCAL_WORDS
+    bool geometry_valid = false;
+    bool state_loaded = false;
+
+    // per-slot operator state (slot order)
+    dbuf<int> nbr;             // n_active x N_K partner slots, -1 empty
+    dbuf<double> measure;
+    dbuf<double2> st;          // {size, numr}
+    dbuf<double2> res;         // frozen reservoir {size_old, numr_old}
+    dbuf<double4> cached;      // bath-start {rate (-1 invalid), first, second, maximum}
+    dbuf<gdx_env> env;
+    dbuf<double> hgs;          // partner gas scale height (vertically integrated physical kernel)
+    dbuf<uint64_t> rng;        // per-slot draw counters
+    dbuf<uint64_t> rng_p;      // per-particle draw counters (particle order, persistent)
+
+    // controller scratch
+    dbuf<unsigned long long> scr_min, scr_max;
+    dbuf<double> lower, upper;
+    dbuf<gdx_raw_rate> raw_rate;
+    dbuf<gdx_raw_audit> raw_audit;
+    dbuf<int> err;
+    dbuf<unsigned long long> counters;   // events, chain owners, pair evaluations
+    std::vector<double> h_lower, h_upper;
+
+    // host controller memory (inc/swarm/_col_chain.cuh:46; persists across operator calls)
+    struct bath_state { double limit_scale = 1.0; int activity_streak = 0, quiet_streak = 0, floor_streak = 0; };
+    std::vector<bath_state> state;
+
+    std::string last_error;
+};
+
+// ---- geometry.hip ---------------------------------------------------------------------------
+void scan_exclusive_u32(const unsigned int *in, unsigned int *out, int n, sort_scratch &s, hipStream_t st);
+void radix_sort_pairs(uint64_t *keys, int *vals, int n, int begin_bit, int end_bit, sort_scratch &s, hipStream_t st);
+void morton_build(morton_index &ix, const float4 *points, const unsigned char *active, int n, float ox, float oy,
+                  float oz, float width, int dim, int leaf_target, int max_level, sort_scratch &s, hipStream_t st);
+void geometry_epoch(context &c);
+void load_state(context &c);
+void store_state(context &c);
+// test access: exact top-K over a built index for arbitrary queries and a runtime K
+void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int *codes, float *d2,
+               unsigned int *overflow, hipStream_t st);
+
+// ---- operator.hip ---------------------------------------------------------------------------
+void launch_reset_scratch(context &c);
+void launch_publish(context &c, const gdx_due &due);
+void launch_bath_rate(context &c, const gdx_due &due, double lambda_0);
+void launch_chain(context &c, const gdx_due &due, double lambda_0);
+void launch_gather_cached(context &c, double *out_particle_order);
+
+}  // namespace gdx
+
+#endif  // GDX_INTERNAL_H

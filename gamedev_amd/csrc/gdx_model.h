@@ -1,0 +1,75 @@
+// Derived compile-time constants of the active model directory (the -I path selects
+// gamedev_amd/mod/<model>/const_defs.h) and the shared plain records of the collision library.
+#ifndef GDX_MODEL_H
+#define GDX_MODEL_H
+
+#include <cstdint>
+
+#include "const_defs.h"
+
+namespace gdx {
+
+constexpr double PI = 3.14159265358979323846;
+constexpr int DIM = (N_Z > 1) ? 3 : 2;
+// an azimuthal wedge needs periodic images (inc/swarm/const_defs.cuh:56-57); only full disks are built
+constexpr bool X_WEDGE = N_X > 1
+    && static_cast<float>(X_MAX) - static_cast<float>(X_MIN) < 6.28318530717958647692f - 1.0e-06f;
+static_assert(!X_WEDGE, "azimuthal wedges (periodic Morton ghosts) are not built by this library");
+// spatial controller groups (inc/swarm/_col_types.cuh:123-125)
+constexpr int LOCAL_GROUPS = ((N_X > 1) ? COL_BIN_X : 1)*COL_BIN_Y*((N_Z > 1) ? COL_BIN_Z : 1);
+constexpr int LOCAL_WORDS = (LOCAL_GROUPS + 31) / 32;
+constexpr int RAW_BINS = LOCAL_GROUPS*COL_BIN_S;
+static_assert(LOCAL_GROUPS <= 64, "one 64-bit dependency mask per group");
+static_assert(COL_BIN_S <= 8, "per-block controller accumulators hold 8 size bins");
+// one wavefront holds 64/N_K owners, one neighbour slot per lane
+static_assert(N_K == 16 || N_K == 32 || N_K == 64, "N_K must be 16, 32 or 64");
+constexpr int SEG = N_K;
+constexpr int NSEG = 64 / N_K;
+// the physical kernel evaluated in a vertically integrated disk needs each partner's gas scale height
+constexpr bool PARTNER_HG = (COAG_KERNEL == 3 && N_Z == 1);
+
+}  // namespace gdx
+
+// owner-position gas coefficients, computed once per geometry epoch with the exact operation sequence
+// of the per-pair reference evaluation (inc/swarm/_col_types.cuh:15-18, _col_rates.cuh:13-24)
+struct gdx_env
+{
+    double Z, omega, vn, radial, strat, cs, re_inv_sqrt, vg_sq;
+};
+
+// Morton hierarchy node; children occupy [first_child, first_child + child_count) in code order
+struct gdx_node
+{
+    float lx, ly, lz, width;
+    int begin, count, first_child, child_count;
+};
+
+// per raw controller bin (spatial group x size bin) sums of one bath-start rating pass
+struct gdx_raw_rate
+{
+    double mass, weighted_rate, weighted_change, weighted_second;
+    int count, owner_count, invalid_count, pad;
+};
+
+// per raw controller bin sums of one post-bath audit pass (col_audit_accum field order)
+struct gdx_raw_audit
+{
+    double mass, predicted_f, predicted_e, predicted_var_f, predicted_var_e, predicted_g, predicted_var_g;
+    unsigned long long maximum_weight_bits, maximum_g_jump_bits;
+    double touched, event_weight, growth, end_mass;
+    int owner_count, invalid_count;
+};
+
+// the due groups of one refresh wave and their contiguous owner slot ranges
+struct gdx_due
+{
+    int n;                       // due groups with active owners
+    int total;                   // total owners
+    int gid[64];
+    int beg[64];
+    int cnt[64];
+    int bstart[65];              // first block of each due group for the launching kernel
+    double dt[64];               // bath end per due group
+};
+
+#endif  // GDX_MODEL_H

@@ -1,0 +1,56 @@
+"""Synthetic swarms for the benchmark and parity workloads (host numpy, particle order).
+
+plain_2d_state:     65,536 representatives, vertically integrated disk (N_Z = 1), r uniform on [0.5, 2.0)
+                    (equal weights with surface density ~ 1/r give dN/dr = const), phi uniform,
+                    monodisperse size 1e-4, equal weights, total mass 1e-4.
+fiducial_3d_state:  1e7 representatives by default, full 3D, r uniform on [0.3, 3.0), polar angle
+                    Gaussian about pi/2 with h/r = 0.05 r^0.25, log-normal sizes (median 1e-4,
+                    sigma_ln = 1, clipped to [1e-5, 1e-1]), equal-mass representatives, total dust
+                    mass = 0.01 x the gas mass 2 pi Sigma_0 (3.0 - 0.3).
+Velocities (Keplerian plus dispersion) do not enter any collision rate: the reference's relative speed
+is query-local and analytic (inc/swarm/_collision.cuh:275-325), so the state holds none.
+The size variable is the reference's par_size and takes the drawn value.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+RHO_0 = 1.0
+
+
+def grain_mass(size, rho=RHO_0):
+    return math.pi*rho*size**3/6.0
+
+
+def plain_2d_state(n: int = 65536, seed: int = 0, total_mass: float = 1e-4, size: float = 1e-4) -> dict:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = rng.uniform(0.5, 2.0, n)
+    phi = rng.uniform(-math.pi, math.pi, n)
+    z = np.full(n, 0.5*math.pi)
+    s = np.full(n, size)
+    w = total_mass/n
+    numr = w/grain_mass(s)
+    return {"x": phi, "y": r, "z": z, "size": s, "numr": numr, "total_mass": total_mass}
+
+
+def fiducial_3d_state(n: int = 10_000_000, seed: int = 0, r_min: float = 0.3, r_max: float = 3.0,
+                      sigma_ln: float = 1.0) -> dict:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = rng.uniform(r_min, r_max, n)
+    phi = rng.uniform(-math.pi, math.pi, n)
+    aspect = 0.05*r**0.25
+    theta = 0.5*math.pi + aspect*rng.standard_normal(n)
+    s = np.clip(1e-4*np.exp(sigma_ln*rng.standard_normal(n)), 1e-5, 1e-1)
+    gas_mass = 2.0*math.pi*1e-2*(r_max - r_min)
+    total_mass = 0.01*gas_mass
+    w = total_mass/n
+    numr = w/grain_mass(s)
+    return {"x": phi, "y": r, "z": theta, "size": s, "numr": numr, "total_mass": total_mass}
+
+
+def to_device(state: dict, device="cuda:0") -> dict:
+    import torch
+    return {k: torch.as_tensor(np.ascontiguousarray(v), dtype=torch.float64, device=device)
+            for k, v in state.items() if isinstance(v, np.ndarray)}

@@ -1,0 +1,140 @@
+/* gdx_col.h -- C ABI of the MI355X swarm collision operator.
+ *
+ * Drop-in boundary for GameDev's swarm collision operator (bijiaqing/GameDev). Each entry point names
+ * the reference interface it replaces. All state pointers are device pointers on the current HIP
+ * device; sizes are element counts. Every function returns a status code (GDX_COL_OK on success) and
+ * keeps a message for gdx_col_last_error(). Calls on one handle must not overlap.
+ *
+ * One library is built per compile-time model directory (gamedev_amd/mod/<model>/const_defs.h), the
+ * analogue of the reference's mod/<model>/const_defs.cuh (Makefile:129-151, README.md:201-321):
+ * N_K, the coagulation kernel, the disk and controller constants are constexpr in the library.
+ */
+#ifndef GDX_COL_H
+#define GDX_COL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: 1-6 are the chain errors of col_chain_run (src/swarm/col_chain_run.cu:258,264,
+ *      280,299,327,372) thrown as "local collision chain error N" (inc/swarm/_col_chain.cuh:403-405) */
+#define GDX_COL_OK                 0
+#define GDX_COL_ERR_CHAIN_RATE     1   /* invalid pair rate (NaN or negative) */
+#define GDX_COL_ERR_CHAIN_TOTAL    2   /* invalid total rate */
+#define GDX_COL_ERR_CHAIN_WAIT     3   /* invalid waiting time */
+#define GDX_COL_ERR_CHAIN_CLOCK    4   /* event time does not advance */
+#define GDX_COL_ERR_CHAIN_PARTNER  5   /* no positive partner slot */
+#define GDX_COL_ERR_CHAIN_MASS     6   /* mass conservation or invalid outcome */
+#define GDX_COL_ERR_CONTINUATION   7   /* continuation limit (_col_chain.cuh:381-384) */
+#define GDX_COL_ERR_RATE_MOMENTS   8   /* invalid change-based rate moments (_col_bound.cuh:22-33, 111) */
+#define GDX_COL_ERR_AUDIT          9   /* invalid audit state / no active mass (_col_bound.cuh:136-146; _col_chain.cuh:434) */
+#define GDX_COL_ERR_SCHEDULE      10   /* scheduler horizon or time resolution (_col_sched.cuh:30-49, 135-143) */
+#define GDX_COL_ERR_MORTON_OVERFLOW 11 /* Morton stack or node-index overflow (swarm_runtime.cu:441-444; morton_index.cuh:215) */
+#define GDX_COL_ERR_MORTON_ARGS   12   /* invalid Morton arguments (morton_index.cuh:172-176) */
+#define GDX_COL_ERR_ALLOC         13   /* device allocation failure */
+#define GDX_COL_ERR_NONFINITE     14   /* non-finite particle state (swarm_runtime.cu:361-411) */
+#define GDX_COL_ERR_ARG           15   /* invalid argument */
+#define GDX_COL_ERR_STATE         16   /* call order: geometry or state missing */
+#define GDX_COL_ERR_HIP           17   /* HIP runtime failure (inc/gpu.cuh:103-120 exits instead) */
+
+/* ---- flags ---- */
+#define GDX_COL_DEFER_WRITEBACK 1      /* keep size/number resident between operator calls; gdx_col_store_state writes back */
+
+typedef struct gdx_col_ctx *gdx_col_handle;
+
+/* compile-time constants of the model directory this library was built for */
+typedef struct {
+    char model[32];
+    int n_k, dim, coag_kernel, n_x, n_y, n_z;
+    int local_groups, col_bin_s, morton_leaf_target, morton_max_level;
+    double x_min, x_max, y_min, y_max, z_min, z_max;
+    double h_search, v_frag, col_bath_max, col_bath_eps, col_bath_alpha, init_smin, rho_0;
+} gdx_col_model;
+
+typedef struct {
+    int n;                          /* representatives: the runtime N_P (const_defs.cuh:39) */
+    double total_dust_mass;         /* lambda_0 = N / N_K / total_dust_mass (_col_chain.cuh:187) */
+    unsigned long long seed;        /* random-number key; replaces rngstate_init (src/swarm/rngstate_init.cu:11-18) */
+    int flags;                      /* GDX_COL_DEFER_WRITEBACK */
+} gdx_col_config;
+
+typedef struct {
+    int waves;                      /* refresh waves: the reference's count_col increments (_col_chain.cuh:480) */
+    int baths;                      /* audited group baths */
+    long long owner_updates;        /* owners advanced over all waves */
+    long long active_owners;        /* active representatives */
+    long long events;               /* accepted collision events */
+    long long chain_owners;         /* owners that entered the event chain */
+    long long pair_evals;           /* pair-rate evaluations */
+    double dt_col;                  /* smallest bath duration of the last wave (_col_chain.cuh:481-482) */
+    double clock_dyn;               /* operator clock at return: the horizon */
+} gdx_col_stats;
+
+/* library and model */
+int gdx_col_model_info(gdx_col_model *out);
+const char *gdx_col_last_error(gdx_col_handle h);
+
+/* construction: replaces the device allocations of main (src/swarm/swarm_runtime.cu:121-184) and
+ * local_workspace (inc/swarm/_col_chain.cuh:36-96) */
+int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out);
+int gdx_col_destroy(gdx_col_handle h);
+
+/* state binding: particle-order device arrays. SoA fields (positions are azimuth x, spherical radius y,
+ * polar angle z as in swarm.position, const_defs.cuh:194-203) or the reference's 64-byte swarm records */
+int gdx_col_bind_soa(gdx_col_handle h, const double *x, const double *y, const double *z,
+                     double *size, double *numr);
+int gdx_col_bind_swarm(gdx_col_handle h, void *swarm_records);
+int gdx_col_set_stream(gdx_col_handle h, void *hip_stream);
+
+/* geometry epoch: col_site_init + morton_ghost_index::build + col_cache_get + col_space_bin +
+ * col_dep_graph (src/swarm/swarm_runtime.cu:384-448; _col_chain.cuh:161-180). Also loads the state. */
+int gdx_col_geometry(gdx_col_handle h);
+/* finiteness screen of a reused geometry epoch (colstate_flag, src/swarm/colstate_flag.cu:13-29) */
+int gdx_col_check_state(gdx_col_handle h);
+/* gather bound size/number into the operator layout, or write them back */
+int gdx_col_load_state(gdx_col_handle h);
+int gdx_col_store_state(gdx_col_handle h);
+
+/* one operator call over `duration`: evolve_local_collisions (inc/swarm/_col_chain.cuh:108-539) */
+int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);
+/* controller memory reset at output boundaries (src/swarm/swarm_runtime.cu:491-496) */
+int gdx_col_reset_controller(gdx_col_handle h);
+int gdx_col_synchronize(gdx_col_handle h);
+
+/* ---- inspection (host buffers, particle order) ---- */
+int gdx_col_active_count(gdx_col_handle h, int *n_active);
+/* neighbour codes 3*j (-1 empty), n x N_K in (d^2, code) order, and the KNN measure (0 if inactive) */
+int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure);
+/* collision sites: float {x, y, z, search cap} x n, active flags, spatial group */
+int gdx_col_get_sites(gdx_col_handle h, float *pts4, unsigned char *active, unsigned char *group);
+/* Morton keys and record order after the stable sort (n each) */
+int gdx_col_get_morton(gdx_col_handle h, unsigned long long *sorted_keys, int *sorted_idx);
+/* bath-start rate moments {rate (-1 invalid), first, second, maximum} per particle (0 for inactive) */
+int gdx_col_get_cached(gdx_col_handle h, double *cached4);
+/* per-particle random-number draw counters (checkpoint/restore) */
+int gdx_col_get_rng(gdx_col_handle h, unsigned long long *counters);
+int gdx_col_set_rng(gdx_col_handle h, const unsigned long long *counters);
+/* dependency graph rows: one 64-bit group mask per group */
+int gdx_col_get_graph(gdx_col_handle h, unsigned long long *rows);
+/* bath-start rating pass only (publish + col_bath_rate for every owner); for rate parity checks */
+int gdx_col_rate_pass(gdx_col_handle h);
+
+/* ---- stand-alone adaptive Morton index: morton_index::build / view (inc/swarm/morton/morton_index.cuh:160-293) ---- */
+typedef struct gdx_morton_ctx *gdx_morton_handle;
+/* points: device float4 {x, y, z, unused} x n; active: device flags or NULL */
+int gdx_morton_build(const void *points4, const unsigned char *active, int n, float ox, float oy, float oz,
+                     float width, int dim, int leaf_target, int max_level, gdx_morton_handle *out);
+int gdx_morton_destroy(gdx_morton_handle h);
+int gdx_morton_node_count(gdx_morton_handle h, int *nodes);
+/* host copies: sorted keys/source indices (n); nodes in the reference layout (lower xyz, width,
+ * begin, count, child[8], child_count) */
+int gdx_morton_get(gdx_morton_handle h, unsigned long long *keys, int *idx, float *lower3, float *width,
+                   int *begin, int *count, int *child8, int *child_count);
+/* exact top-K for device float4 queries {x, y, z, search cap}: device outputs codes (3*idx, -1) and d^2 */
+int gdx_morton_knn(gdx_morton_handle h, const void *queries4, int nq, int k, int *codes, float *d2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GDX_COL_H */
