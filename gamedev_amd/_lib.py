@@ -15,6 +15,8 @@ STATUS = {
     12: "MORTON_ARGS", 13: "ALLOC", 14: "NONFINITE", 15: "ARG", 16: "STATE", 17: "HIP",
 }
 DEFER_WRITEBACK = 1
+PROFILE = 2
+KERNELS = ("publish", "bath_rate", "chain")
 
 
 class GdxError(RuntimeError):
@@ -61,6 +63,8 @@ _SIGS = {
     "gdx_col_operator": [P, D, C.POINTER(Stats)],
     "gdx_col_reset_controller": [P],
     "gdx_col_synchronize": [P],
+    "gdx_col_kernel_times": [P, P, P, P],
+    "gdx_col_reset_kernel_times": [P],
     "gdx_col_active_count": [P, C.POINTER(I)],
     "gdx_col_get_neighbors": [P, P, P],
     "gdx_col_get_sites": [P, P, P, P],
@@ -70,6 +74,10 @@ _SIGS = {
     "gdx_col_set_rng": [P, P],
     "gdx_col_get_graph": [P, P],
     "gdx_col_rate_pass": [P],
+    "gdx_col_host_binmap": [P, I, P, P],
+    "gdx_col_host_choose": [P, I, D, D, P],
+    "gdx_col_host_finish": [P, I, P, P],
+    "gdx_col_host_schedule": [I, D, P, P, I, P, P, P, P],
     "gdx_morton_build": [P, P, I, C.c_float, C.c_float, C.c_float, C.c_float, I, I, I, C.POINTER(P)],
     "gdx_morton_destroy": [P],
     "gdx_morton_node_count": [P, C.POINTER(I)],

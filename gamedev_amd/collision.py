@@ -35,13 +35,14 @@ def _np_ptr(a: np.ndarray) -> int:
 class CollisionOperator:
     """One swarm's collision operator on the current HIP device (one library per model directory)."""
 
-    def __init__(self, model: str, n: int, total_dust_mass: float, seed: int = 0, defer_writeback: bool = False):
+    def __init__(self, model: str, n: int, total_dust_mass: float, seed: int = 0, defer_writeback: bool = False,
+                 profile: bool = False):
         self.model = model
         self.lib = _lib.load(model)
         self.info = model_info(model)
         self.n = int(n)
-        cfg = _lib.Config(self.n, float(total_dust_mass), int(seed) & (2**64 - 1),
-                          _lib.DEFER_WRITEBACK if defer_writeback else 0)
+        flags = (_lib.DEFER_WRITEBACK if defer_writeback else 0) | (_lib.PROFILE if profile else 0)
+        cfg = _lib.Config(self.n, float(total_dust_mass), int(seed) & (2**64 - 1), flags)
         h = C.c_void_p()
         _lib.check(self.lib, None, self.lib.gdx_col_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -103,6 +104,18 @@ class CollisionOperator:
 
     def rate_pass(self) -> None:
         self._call(self.lib.gdx_col_rate_pass)
+
+    def kernel_times(self) -> dict:
+        """Accumulated HIP-event time (ms), launches and owners per operator kernel (profile=True)."""
+        ms = np.zeros(3, np.float64)
+        la = np.zeros(3, np.int64)
+        ow = np.zeros(3, np.int64)
+        self._call(self.lib.gdx_col_kernel_times, _np_ptr(ms), _np_ptr(la), _np_ptr(ow))
+        return {k: {"ms": float(ms[i]), "launches": int(la[i]), "owners": int(ow[i])}
+                for i, k in enumerate(_lib.KERNELS)}
+
+    def reset_kernel_times(self) -> None:
+        self._call(self.lib.gdx_col_reset_kernel_times)
 
     # ---- inspection ----
     def active_count(self) -> int:

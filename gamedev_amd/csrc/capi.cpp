@@ -90,6 +90,53 @@ void copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t st)
     hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "device to host copy");
 }
 
+// HIP-event timing of one launch on the library stream; resolved at the wave's next synchronisation
+struct kernel_timer
+{
+    context &c;
+    struct pending { hipEvent_t a, b; int kind; long long owners; };
+    std::vector<pending> open;
+    explicit kernel_timer(context &ctx) : c(ctx) {}
+    bool on() const { return (c.flags & GDX_COL_PROFILE) != 0; }
+    hipEvent_t take()
+    {
+        if (c.events.empty())
+        {
+            hipEvent_t e;
+            hip_check(hipEventCreate(&e), "create event");
+            return e;
+        }
+        hipEvent_t e = c.events.back();
+        c.events.pop_back();
+        return e;
+    }
+    template <typename F>
+    void time(int kind, long long owners, F &&launch)
+    {
+        if (!on() || owners == 0) { launch(); return; }
+        pending p{take(), take(), kind, owners};
+        hip_check(hipEventRecord(p.a, c.stream), "record event");
+        launch();
+        hip_check(hipEventRecord(p.b, c.stream), "record event");
+        open.push_back(p);
+    }
+    // call after the stream has been synchronised
+    void resolve()
+    {
+        for (const pending &p : open)
+        {
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, p.a, p.b), "event time");
+            c.timing.ms[p.kind] += ms;
+            c.timing.launches[p.kind] += 1;
+            c.timing.owners[p.kind] += p.owners;
+            c.events.push_back(p.a);
+            c.events.push_back(p.b);
+        }
+        open.clear();
+    }
+};
+
 }  // namespace
 
 // one operator call (inc/swarm/_col_chain.cuh:108-539)
@@ -99,6 +146,7 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
     if (!(duration > 0) || !std::isfinite(duration)) throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
     hipStream_t st = c.stream;
     const double lambda0 = c.n / static_cast<double>(N_K) / c.total_mass;
+    kernel_timer timer(c);
     gdx_col_stats stats{};
     stats.active_owners = c.n_active;
     hip_check(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long)*3, st), "zero counters");
@@ -114,10 +162,11 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
     // publish, rate and bin the due owners; returns the merged-bin count (_col_chain.cuh:190-243)
     auto rate_wave = [&](const gdx_due &due) {
         hip_check(hipMemsetAsync(c.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, st), "zero rate bins");
-        launch_publish(c, due);
-        launch_bath_rate(c, due, lambda0);
+        timer.time(GDX_COL_KERNEL_PUBLISH, due.total, [&] { launch_publish(c, due); });
+        timer.time(GDX_COL_KERNEL_RATE, due.total, [&] { launch_bath_rate(c, due, lambda0); });
         copy_d2h(raw_rate.data(), c.raw_rate.p, sizeof(gdx_raw_rate)*RAW_BINS, st);
         hip_check(hipStreamSynchronize(st), "rating pass");
+        timer.resolve();
         stats.pair_evals += (long long)due.total*N_K;
         for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count;
         int m = build_binmap(counts, binmap);
@@ -173,11 +222,12 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
         gdx_due due = make_due(c, groups, steps);
         hip_check(hipMemsetAsync(c.raw_audit.p, 0, sizeof(gdx_raw_audit)*RAW_BINS, st), "zero audit bins");
         hip_check(hipMemsetAsync(c.err.p, 0, sizeof(int), st), "zero error");
-        launch_chain(c, due, lambda0);
+        timer.time(GDX_COL_KERNEL_CHAIN, due.total, [&] { launch_chain(c, due, lambda0); });
         int err = 0;
         copy_d2h(&err, c.err.p, sizeof(int), st);
         copy_d2h(raw_audit.data(), c.raw_audit.p, sizeof(gdx_raw_audit)*RAW_BINS, st);
         hip_check(hipStreamSynchronize(st), "chain pass");
+        timer.resolve();
         if (err) throw status_error(err, "local collision chain error " + std::to_string(err));
 
         std::fill(audit.begin(), audit.begin() + merged, audit_bin{});
@@ -311,6 +361,7 @@ int gdx_col_destroy(gdx_col_handle h)
 {
     if (!h) return GDX_COL_ERR_ARG;
     (void)hipStreamSynchronize(h->c.stream);
+    for (hipEvent_t e : h->c.events) (void)hipEventDestroy(e);
     delete h;
     return GDX_COL_OK;
 }
@@ -404,6 +455,25 @@ int gdx_col_synchronize(gdx_col_handle h)
 {
     if (!h) return GDX_COL_ERR_ARG;
     return guard(&h->c, [&] { hip_check(hipStreamSynchronize(h->c.stream), "synchronize"); });
+}
+
+int gdx_col_kernel_times(gdx_col_handle h, double *ms, long long *launches, long long *owners)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    for (int k = 0; k < GDX_COL_KERNELS; ++k)
+    {
+        if (ms) ms[k] = h->c.timing.ms[k];
+        if (launches) launches[k] = h->c.timing.launches[k];
+        if (owners) owners[k] = h->c.timing.owners[k];
+    }
+    return GDX_COL_OK;
+}
+
+int gdx_col_reset_kernel_times(gdx_col_handle h)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    h->c.timing = context::timing_t{};
+    return GDX_COL_OK;
 }
 
 int gdx_col_active_count(gdx_col_handle h, int *n_active)
@@ -553,6 +623,82 @@ int gdx_col_rate_pass(gdx_col_handle h)
         launch_publish(c, due);
         launch_bath_rate(c, due, c.n / static_cast<double>(N_K) / c.total_mass);
         hip_check(hipStreamSynchronize(c.stream), "rating pass");
+    });
+}
+
+// ---- host controller and scheduler ----
+int gdx_col_host_binmap(const int *raw_count, int raw_bins, int *raw_to_merged, int *merged)
+{
+    if (!raw_count || !raw_to_merged || !merged || raw_bins <= 0 || raw_bins % COL_BIN_S) return GDX_COL_ERR_ARG;
+    return guard(nullptr, [&] {
+        std::vector<int> rc(raw_count, raw_count + raw_bins), map;
+        *merged = build_binmap(rc, map);
+        std::copy(map.begin(), map.end(), raw_to_merged);
+    });
+}
+
+int gdx_col_host_choose(const double *bins6, int n, double remaining, double limit_scale, double *duration)
+{
+    if (!bins6 || !duration || n < 0) return GDX_COL_ERR_ARG;
+    return guard(nullptr, [&] {
+        std::vector<rate_bin> b(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const double *v = bins6 + 6*i;
+            b[i] = rate_bin{v[0], v[1], v[2], v[3], int(v[4]), int(v[5])};
+        }
+        *duration = choose_bath(b.data(), n, remaining, limit_scale);
+    });
+}
+
+int gdx_col_host_finish(const double *audit15, int n, double *state4, double *result10)
+{
+    if (!audit15 || !state4 || !result10 || n < 0) return GDX_COL_ERR_ARG;
+    return guard(nullptr, [&] {
+        std::vector<audit_bin> b(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const double *v = audit15 + 15*i;
+            audit_bin &o = b[i];
+            o.mass = v[0]; o.predicted_f = v[1]; o.predicted_e = v[2]; o.predicted_var_f = v[3];
+            o.predicted_var_e = v[4]; o.predicted_g = v[5]; o.predicted_var_g = v[6]; o.maximum_weight = v[7];
+            o.maximum_g_jump = v[8]; o.touched = v[9]; o.event_weight = v[10]; o.growth = v[11];
+            o.end_mass = v[12]; o.owner_count = int(v[13]); o.invalid_count = int(v[14]);
+        }
+        bath_state st{state4[0], int(state4[1]), int(state4[2]), int(state4[3])};
+        bath_result r = finish_bath(b.data(), n, st);
+        state4[0] = st.limit_scale; state4[1] = st.activity_streak; state4[2] = st.quiet_streak;
+        state4[3] = st.floor_streak;
+        const double out[10] = {r.max_f, r.max_e, r.max_touched, r.max_events, r.max_g, r.max_g_upper, r.d_bath,
+                                double(r.activity_overshoot), double(r.distribution_overshoot),
+                                double(r.persistent_overshoot)};
+        std::copy(out, out + 10, result10);
+    });
+}
+
+int gdx_col_host_schedule(int n, double horizon, const double *requested, const unsigned int *edges, int max_waves,
+                          unsigned long long *ticks, int *due_counts, int *levels, int *waves)
+{
+    if (n <= 0 || !requested || !edges || !ticks || !due_counts || !levels || !waves) return GDX_COL_ERR_ARG;
+    return guard(nullptr, [&] {
+        std::vector<double> req(requested, requested + n);
+        std::vector<uint32_t> e(edges, edges + size_t(n)*((n + 31) / 32));
+        schedule s(horizon, req, e);
+        for (int c = 0; c < n; ++c) levels[c] = s.level[c];
+        std::vector<bool> passed(n, true);
+        int w = 0;
+        while (s.time() < s.end)
+        {
+            if (w >= max_waves) throw status_error(GDX_COL_ERR_ARG, "schedule exceeds max_waves");
+            uint64_t tick = s.time();
+            std::vector<int> groups = s.due(tick);
+            if (tick != 0) s.adapt(tick, groups, req, passed);
+            ticks[w] = tick;
+            due_counts[w] = int(groups.size());
+            ++w;
+            s.advance(groups);
+        }
+        *waves = w;
     });
 }
 

@@ -141,6 +141,14 @@ struct context
     struct bath_state { double limit_scale = 1.0; int activity_streak = 0, quiet_streak = 0, floor_streak = 0; };
     std::vector<bath_state> state;
 
+    // optional per-kernel timing (HIP events on the launch stream)
+    struct timing_t
+    {
+        double ms[3] = {0, 0, 0};
+        long long launches[3] = {0, 0, 0}, owners[3] = {0, 0, 0};
+    } timing;
+    std::vector<hipEvent_t> events;
+
     std::string last_error;
 };
 

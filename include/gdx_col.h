@@ -39,6 +39,13 @@ extern "C" {
 
 /* ---- flags ---- */
 #define GDX_COL_DEFER_WRITEBACK 1      /* keep size/number resident between operator calls; gdx_col_store_state writes back */
+#define GDX_COL_PROFILE         2      /* time every operator kernel with HIP events on the launch stream */
+
+/* kernels timed under GDX_COL_PROFILE */
+#define GDX_COL_KERNEL_PUBLISH   0
+#define GDX_COL_KERNEL_RATE      1
+#define GDX_COL_KERNEL_CHAIN     2
+#define GDX_COL_KERNELS          3
 
 typedef struct gdx_col_ctx *gdx_col_handle;
 
@@ -100,6 +107,10 @@ int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);
 /* controller memory reset at output boundaries (src/swarm/swarm_runtime.cu:491-496) */
 int gdx_col_reset_controller(gdx_col_handle h);
 int gdx_col_synchronize(gdx_col_handle h);
+/* accumulated kernel time (ms), launches and owners per GDX_COL_KERNEL_* since the last reset; no
+ * reference counterpart (the reference's COL_PERF_VAL hooks time host phases, _col_chain.cuh:215-241) */
+int gdx_col_kernel_times(gdx_col_handle h, double *ms, long long *launches, long long *owners);
+int gdx_col_reset_kernel_times(gdx_col_handle h);
 
 /* ---- inspection (host buffers, particle order) ---- */
 int gdx_col_active_count(gdx_col_handle h, int *n_active);
@@ -118,6 +129,19 @@ int gdx_col_set_rng(gdx_col_handle h, const unsigned long long *counters);
 int gdx_col_get_graph(gdx_col_handle h, unsigned long long *rows);
 /* bath-start rating pass only (publish + col_bath_rate for every owner); for rate parity checks */
 int gdx_col_rate_pass(gdx_col_handle h);
+
+/* ---- host controller and scheduler, callable without a GPU (same code the operator runs) ---- */
+/* _build_col_binmap (inc/swarm/_col_bound.cuh:45-98): returns the merged-bin count */
+int gdx_col_host_binmap(const int *raw_count, int raw_bins, int *raw_to_merged, int *merged);
+/* _choose_col_bath (_col_bound.cuh:101-123); bins: n x {mass, w_rate, w_change, w_second, owners, invalid} */
+int gdx_col_host_choose(const double *bins6, int n, double remaining, double limit_scale, double *duration);
+/* _finish_col_bath (_col_bound.cuh:126-216); audit: n x 15 (col_audit_accum order), state: {limit_scale,
+ * activity, quiet, floor}, result: {max_f, max_e, max_touched, max_events, max_g, max_g_upper, d_bath,
+ * activity_overshoot, distribution_overshoot, persistent_overshoot} */
+int gdx_col_host_finish(const double *audit15, int n, double *state4, double *result10);
+/* local_schedule (_col_sched.cuh:17-183) run with fixed requests and passing audits: due ticks per wave */
+int gdx_col_host_schedule(int n, double horizon, const double *requested, const unsigned int *edges, int max_waves,
+                          unsigned long long *ticks, int *due_counts, int *levels, int *waves);
 
 /* ---- stand-alone adaptive Morton index: morton_index::build / view (inc/swarm/morton/morton_index.cuh:160-293) ---- */
 typedef struct gdx_morton_ctx *gdx_morton_handle;

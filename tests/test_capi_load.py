@@ -1,0 +1,42 @@
+"""The C-ABI libraries load without a GPU and export every entry point include/gdx_col.h declares."""
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "gdx_col.h"
+
+
+def declared() -> list[str]:
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(gdx_\w+)\s*\(", text, re.M)))
+
+
+@pytest.mark.parametrize("model", ["plain_2d", "fiducial_3d"])
+def test_library_exports_header(model):
+    from gamedev_amd import _lib
+    lib = _lib.load(model)
+    names = declared()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTS)
+
+
+@pytest.mark.parametrize("model,k,dim,kernel", [("plain_2d", 16, 2, 0), ("fiducial_3d", 32, 3, 3)])
+def test_model_info(model, k, dim, kernel):
+    from gamedev_amd import model_info
+    info = model_info(model)
+    assert info["model"] == model and info["n_k"] == k and info["dim"] == dim and info["coag_kernel"] == kernel
+
+
+def test_argument_errors_without_gpu():
+    import ctypes as C
+    from gamedev_amd import _lib
+    lib = _lib.load("plain_2d")
+    cfg = _lib.Config(0, 1.0, 0, 0)
+    h = C.c_void_p()
+    assert lib.gdx_col_create(C.byref(cfg), C.byref(h)) == 15           # GDX_COL_ERR_ARG
+    assert b"representative count" in lib.gdx_col_last_error(None)
+    assert lib.gdx_col_operator(None, C.c_double(1.0), None) == 15
