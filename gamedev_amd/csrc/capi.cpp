@@ -510,7 +510,7 @@ int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure)
                 int v = -1;
                 if (s >= 0)
                 {
-                    int j = nbr[size_t(s)*N_K + k];
+                    int j = nbr[size_t(s)*N_K + bitrev_slot(k)];
                     v = j < 0 ? -1 : 3*slot_pidx[j];
                 }
                 if (codes) codes[size_t(p)*N_K + k] = v;

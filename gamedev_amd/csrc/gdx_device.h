@@ -314,33 +314,187 @@ __device__ __forceinline__ double sample_outcome(double si, double sj, bool high
     return fmin(si, fmax(INIT_SMIN, root*root));
 }
 
-// one pair term of _get_col_chain_rate (_col_rates.cuh:167-223) before division by the measure
-__device__ __forceinline__ double pair_numer(const gdx_env &e, double hg_i, double hg_j, double size_i,
-                                             double size_j, double numr_j, double lambda_0, double &vrel)
+// RN(a/b) from y = RN(1/b): q = a y, r = a - b q (exact with one FMA), q' = RN(q + r y) is the
+// correctly rounded quotient (Markstein's theorem; valid without underflow or overflow). Used where
+// the divisor is owner-uniform or a constant, so the reciprocal is paid once.
+__device__ __forceinline__ double div_rn(double a, double b, double y)
 {
-    vrel = 0.0;
-    if constexpr (COAG_KERNEL == 0)
+    double q = a*y;
+    double r = fma(-q, b, a);
+    return fma(r, y, q);
+}
+
+constexpr double INV_S0 = 1.0 / S_0;
+constexpr double INV_THREE = 1.0 / 3.0;
+constexpr double INV_SMIN = 1.0 / INIT_SMIN;
+
+// owner-only factors of one bath-start or chain evaluation; every value is produced by the same
+// operations, in the same order, as the reference's per-pair expressions
+struct owner_terms
+{
+    double size_i, m, inv_m, inv_size;
+    double si, fi, ai, mi, vn, zo, ris, vg_sq, radial, inv_radial, strat, inv_strat, cs, hg;
+};
+
+__device__ __forceinline__ owner_terms make_owner(const gdx_env &e, double size_i, double m, double hg_i)
+{
+    owner_terms t{};
+    t.size_i = size_i;
+    t.m = m;
+    t.inv_m = 1.0 / m;
+    t.inv_size = 1.0 / size_i;
+    if constexpr (COAG_KERNEL == 3)
     {
-        (void)e; (void)hg_i; (void)hg_j; (void)size_i; (void)size_j;
-        return lambda_0*numr_j;
-    }
-    else if constexpr (COAG_KERNEL == 1)
-    {
-        (void)e; (void)hg_i; (void)hg_j;
-        return lambda_0*numr_j*(grain_mass(size_i) + grain_mass(size_j));
-    }
-    else if constexpr (COAG_KERNEL == 2)
-    {
-        (void)e; (void)hg_i; (void)hg_j;
-        return lambda_0*numr_j*grain_mass(size_i)*grain_mass(size_j);
+        t.si = stokes(e, size_i);
+        t.fi = 1.0 / (1.0 + t.si*t.si);
+        t.ai = t.si*t.fi;
+        t.mi = fmin(t.si, 0.5);
+        t.vn = e.vn;
+        t.zo = e.Z*e.omega;
+        t.ris = e.re_inv_sqrt;
+        t.vg_sq = e.vg_sq;
+        t.radial = e.radial;
+        t.inv_radial = 1.0 / e.radial;
+        t.strat = e.strat;
+        t.inv_strat = 1.0 / e.strat;
+        t.cs = e.cs;
+        t.hg = hg_i;
     }
     else
     {
-        vrel = vrel_pair(e, size_i, size_j);
-        double rate = numr_j*vrel*PI*(size_i + size_j)*(size_i + size_j) / 4.0;
-        if constexpr (N_Z == 1) rate /= sqrt(2.0*PI*(hg_i*hg_i + hg_j*hg_j));
-        return rate;
+        (void)e; (void)hg_i;
     }
+    return t;
+}
+
+// _get_stokes for a partner size with the owner's scalings: same three divisions, uniform divisors
+__device__ __forceinline__ double stokes_partner(const owner_terms &t, double size)
+{
+    double st = STOKES_0*div_rn(size, S_0, INV_S0);
+#ifndef CONST_ST
+    st = div_rn(st, t.radial, t.inv_radial);
+    st = div_rn(st, t.strat, t.inv_strat);
+#endif
+    return st;
+}
+
+__device__ __forceinline__ double vrel_pair_terms(const owner_terms &t, double size_j)
+{
+    double sj = stokes_partner(t, size_j);
+    double fj = 1.0 / (1.0 + sj*sj);
+    double dvr = 2.0*t.vn*(t.ai - sj*fj);
+    double dvphi = t.vn*(t.fi - fj);
+    double dvz = t.zo*(t.mi - fmin(sj, 0.5));
+    gdx_env e{};
+    e.re_inv_sqrt = t.ris;
+    e.vg_sq = t.vg_sq;
+    double vt = vrel_t(e, t.si, sj);
+    double vb = 0.0;
+#ifndef CODE_UNIT
+    double mi = grain_mass(t.size_i);
+    double mj = grain_mass(size_j);
+    vb = fmin(sqrt(8.0*t.cs*t.cs*M_MOL*(mi + mj) / (PI*mi*mj)), t.cs);
+#endif
+    return sqrt(dvr*dvr + dvphi*dvphi + dvz*dvz + vt*vt + vb*vb);
+}
+
+// _erosion_outcome_moments with owner-uniform and constant divisors (_col_event.cuh:31-74)
+__device__ __forceinline__ double outcome_moments_terms(const owner_terms &t, double sj, bool high_speed,
+                                                        double &mean, double &second, double &maximum)
+{
+    const double si = t.size_i;
+    double ratio = div_rn(sj, si, t.inv_size);
+    double q = ratio*ratio*ratio;
+    double Gp = sticking_packet(q);
+    if (!high_speed)
+    {
+        mean = div_rn(log1p(Gp*q), 3.0, INV_THREE);
+        second = mean*mean;
+        maximum = mean;
+        return Gp == 1.0 ? 1.0 : 1.0 / Gp;
+    }
+    if (q <= 0.1)
+    {
+        double remnant = Gp == 1.0 ? (1.0 - q) : (1.0 - q) / Gp;
+        double debris = q;
+        double factor = remnant + debris;
+        double jr = div_rn(-log1p(-Gp*q), 3.0, INV_THREE);
+        double jd = div_rn(-log(q), 3.0, INV_THREE);
+        mean = (remnant*jr + debris*jd) / factor;
+        second = (remnant*jr*jr + debris*jd*jd) / factor;
+        maximum = fmax(jr, jd);
+        return factor;
+    }
+    double L = 0.5*log(div_rn(si, INIT_SMIN, INV_SMIN));
+    if (L < 1.0e-03)
+    {
+        mean = L - L*L / 6.0 + L*L*L*L / 360.0;
+        second = L*L*(4.0/3.0 - L / 3.0 + L*L / 90.0 + L*L*L / 180.0);
+    }
+    else
+    {
+        double tail = exp(-L) / (-expm1(-L));
+        mean = 2.0*(1.0 - L*tail);
+        second = 8.0 - (4.0*L*L + 8.0*L)*tail;
+    }
+    maximum = 2.0*L;
+    return 1.0;
+}
+
+// one slot's pair term (_get_col_chain_rate / measure x outcome factor; col_bath_rate.cu:40-57)
+struct pair_term
+{
+    double rate, j1, j2, jmax, vrel, size_j;
+};
+
+__device__ __forceinline__ pair_term eval_pair_terms(const owner_terms &t, int j, const double2 *res,
+                                                     const double *hgs, double lambda_0)
+{
+    pair_term p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j < 0) return p;
+    double2 pj = res[j];
+    const double size_i = t.size_i, size_j = pj.x, numr_j = pj.y;
+    double vrel = 0.0;
+    double rate;
+    if constexpr (COAG_KERNEL == 0)
+    {
+        (void)hgs;
+        rate = lambda_0*numr_j;
+    }
+    else if constexpr (COAG_KERNEL == 1)
+    {
+        (void)hgs;
+        rate = lambda_0*numr_j*(grain_mass(size_i) + grain_mass(size_j));
+    }
+    else if constexpr (COAG_KERNEL == 2)
+    {
+        (void)hgs;
+        rate = lambda_0*numr_j*grain_mass(size_i)*grain_mass(size_j);
+    }
+    else
+    {
+        vrel = vrel_pair_terms(t, size_j);
+        rate = numr_j*vrel*PI*(size_i + size_j)*(size_i + size_j) / 4.0;
+        if constexpr (N_Z == 1)
+        {
+            double hg_j = hgs[j];
+            rate /= sqrt(2.0*PI*(t.hg*t.hg + hg_j*hg_j));
+        }
+        else
+        {
+            (void)hgs;
+        }
+    }
+    double r = div_rn(rate, t.m, t.inv_m);
+    double mean = 0.0, second = 0.0, maximum = 0.0;
+    r *= outcome_moments_terms(t, size_j, vrel >= V_FRAG, mean, second, maximum);
+    p.rate = r;
+    p.j1 = r*mean;
+    p.j2 = r*second;
+    p.jmax = r > 0.0 ? maximum : 0.0;
+    p.vrel = vrel;
+    p.size_j = size_j;
+    return p;
 }
 
 // moving logarithmic size bin (inc/swarm/_col_sizes.cuh:12-21)

@@ -25,6 +25,15 @@ static_assert(COL_BIN_S <= 8, "per-block controller accumulators hold 8 size bin
 static_assert(N_K == 16 || N_K == 32 || N_K == 64, "N_K must be 16, 32 or 64");
 constexpr int SEG = N_K;
 constexpr int NSEG = 64 / N_K;
+constexpr int LOG2K = (N_K == 16) ? 4 : (N_K == 32) ? 5 : 6;
+// neighbour rows are stored in bit-reversed slot order: position t holds slot bitrev(t), so that
+// streaming positions 0..K-1 pairs slots exactly as the upper-half fold does (j with j + K/2, ...)
+constexpr int bitrev_slot(int t)
+{
+    int r = 0;
+    for (int b = 0; b < LOG2K; ++b) r |= ((t >> b) & 1) << (LOG2K - 1 - b);
+    return r;
+}
 // the physical kernel evaluated in a vertically integrated disk needs each partner's gas scale height
 constexpr bool PARTNER_HG = (COAG_KERNEL == 3 && N_Z == 1);
 

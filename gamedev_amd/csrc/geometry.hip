@@ -537,7 +537,9 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
         }
         int4 *dst = reinterpret_cast<int4 *>(nbr + (long long)s*N_K);
 #pragma unroll
-        for (int k = 0; k < N_K; k += 4) dst[k / 4] = make_int4(row[k], row[k + 1], row[k + 2], row[k + 3]);
+        for (int t = 0; t < N_K; t += 4)
+            dst[t / 4] = make_int4(row[bitrev_slot(t)], row[bitrev_slot(t + 1)], row[bitrev_slot(t + 2)],
+                                   row[bitrev_slot(t + 3)]);
         double radius = sqrt(static_cast<double>(max_d2));
         measure[s] = ball_measure(ld(u.y, p), ld(u.z, p), radius);
     }
@@ -598,20 +600,33 @@ __global__ __launch_bounds__(256) void k_slot_maps(const int *order, const unsig
                                                   const unsigned char *active, int n, int *slot_pidx, int *pidx_slot,
                                                   int *group_active, int *group_all)
 {
+    // per-block histograms: one global atomic per (block, group) instead of one per particle
+    __shared__ int hist[2][LOCAL_GROUPS];
+    for (int i = threadIdx.x; i < 2*LOCAL_GROUPS; i += blockDim.x) (&hist[0][0])[i] = 0;
+    __syncthreads();
     long long s = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    int p = order[s];
-    if (active[p])
+    if (s < n)
     {
-        slot_pidx[s] = p;
-        pidx_slot[p] = int(s);
-        atomicAdd(group_active + group[p], 1);
+        int p = order[s];
+        int g = group[p];
+        if (active[p])
+        {
+            slot_pidx[s] = p;
+            pidx_slot[p] = int(s);
+            atomicAdd(&hist[0][g], 1);
+        }
+        else
+        {
+            pidx_slot[p] = -1;
+        }
+        atomicAdd(&hist[1][g], 1);
     }
-    else
+    __syncthreads();
+    for (int i = threadIdx.x; i < LOCAL_GROUPS; i += blockDim.x)
     {
-        pidx_slot[p] = -1;
+        if (hist[0][i]) atomicAdd(group_active + i, hist[0][i]);
+        if (hist[1][i]) atomicAdd(group_all + i, hist[1][i]);
     }
-    atomicAdd(group_all + group[p], 1);
 }
 
 __global__ __launch_bounds__(256) void k_gather_state(bound_state u, const int *slot_pidx, int n_active,

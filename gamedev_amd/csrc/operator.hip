@@ -25,8 +25,8 @@ namespace {
 constexpr int WAVE = 64;
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / WAVE;
-constexpr int PUBLISH_OWNERS = 256;
-constexpr int RATE_OWNERS = 64;
+constexpr int PUBLISH_OWNERS = 1024;
+constexpr int RATE_OWNERS = 256;
 constexpr int CHAIN_OWNERS = 256;
 // events one owner may take in one bath: the reference allows COL_EVENT_CAP events per launch and
 // 1e6 continuation launches (inc/swarm/_col_chain.cuh:381-384); exceeding this bound is status 7
@@ -41,8 +41,20 @@ __device__ __forceinline__ bool due_block(const gdx_due &due, int per_block, int
     int local = b - due.bstart[gi];
     s0 = due.beg[gi] + local*per_block;
     s1 = min(due.beg[gi] + due.cnt[gi], s0 + per_block);
-    (void)per_block;
     return s0 < s1;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, WAVE));
+    return v;
 }
 
 __global__ void k_reset_scratch(unsigned long long *scr_min, unsigned long long *scr_max)
@@ -55,20 +67,21 @@ __global__ void k_reset_scratch(unsigned long long *scr_min, unsigned long long 
     }
 }
 
+// col_bath_init + col_comp_zero + col_size_scan (col_bath_init.cu:11-23, col_size_scan.cu:13-25)
 __global__ __launch_bounds__(BLOCK) void k_publish(gdx_due due, const double2 *st, double2 *res,
                                                   unsigned long long *scr_min, unsigned long long *scr_max)
 {
+    __shared__ unsigned long long red[2][WAVES];
     int gi, s0, s1;
     if (!due_block(due, PUBLISH_OWNERS, gi, s0, s1)) return;
     const int g = due.gid[gi];
-    int s = s0 + threadIdx.x;
     unsigned long long lo = ~0ull, hi = 0ull;
-    if (s < s1)
+    for (int s = s0 + threadIdx.x; s < s1; s += BLOCK)
     {
         double2 v = st[s];
         res[s] = v;
         // positive doubles order like their bit patterns (col_size_scan.cu:20-24)
-        if (v.x > 0 && isfinite(v.x)) { lo = dbits(v.x); hi = lo; }
+        if (v.x > 0 && isfinite(v.x)) { lo = min(lo, dbits(v.x)); hi = max(hi, dbits(v.x)); }
     }
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1)
@@ -76,39 +89,32 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_due due, const double2 *s
         lo = min(lo, __shfl_xor(lo, m, WAVE));
         hi = max(hi, __shfl_xor(hi, m, WAVE));
     }
-    if (threadIdx.x % WAVE == 0 && hi != 0ull)
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (lane == 0) { red[0][wave] = lo; red[1][wave] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
-        atomicMin(scr_min + g, lo);
-        atomicMax(scr_max + g, hi);
+        for (int w = 1; w < WAVES; ++w) { lo = min(lo, red[0][w]); hi = max(hi, red[1][w]); }
+        if (hi != 0ull)
+        {
+            atomicMin(scr_min + g, lo);
+            atomicMax(scr_max + g, hi);
+        }
     }
 }
 
-// one slot's pair term at owner size size_i (col_bath_rate.cu:30-58, col_chain_run.cu:136-165)
-struct pair_term
+// ---------------------------------------------------------------------------------------------
+// bath-start rates, one lane per owner (col_bath_rate.cu:13-129, col_count_bin.cu, col_rate_bins.cu)
+// The lane streams its K slots from the bit-reversed row, four pair terms per chunk: positions
+// (4c, 4c+1) and (4c+2, 4c+3) are slot pairs (j, j + K/2), the chunk sum is a level-2 partial of the
+// reference's upper-half fold, and higher levels combine through a carry stack whose branches
+// depend only on the chunk index (wavefront-uniform). The sums equal the fold bit for bit.
+// ---------------------------------------------------------------------------------------------
+struct fold3
 {
-    double rate, j1, j2, jmax, vrel, size_j;
+    double r, a, b;
 };
-
-__device__ __forceinline__ pair_term eval_pair(int j, const gdx_env &e, double hg_i, const double *hgs,
-                                               const double2 *res, double size_i, double m, double lambda_0)
-{
-    pair_term t{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (j < 0) return t;
-    double2 pj = res[j];
-    double hg_j = 0.0;
-    if constexpr (PARTNER_HG) hg_j = hgs[j];
-    double vrel = 0.0;
-    double r = pair_numer(e, hg_i, hg_j, size_i, pj.x, pj.y, lambda_0, vrel) / m;
-    double mean = 0.0, second = 0.0, maximum = 0.0;
-    r *= outcome_moments(size_i, pj.x, vrel >= V_FRAG, mean, second, maximum);
-    t.rate = r;
-    t.j1 = r*mean;
-    t.j2 = r*second;
-    t.jmax = r > 0.0 ? maximum : 0.0;
-    t.vrel = vrel;
-    t.size_j = pj.x;
-    return t;
-}
+__device__ __forceinline__ fold3 add3(const fold3 &x, const fold3 &y) { return {x.r + y.r, x.a + y.a, x.b + y.b}; }
 
 __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr, const double2 *st,
                                                     const double2 *res, const double *measure, const gdx_env *env,
@@ -117,8 +123,8 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
                                                     const unsigned long long *scr_max, double *lower, double *upper,
                                                     double4 *cached, gdx_raw_rate *raw)
 {
-    __shared__ double accd[COL_BIN_S][4];
-    __shared__ int acci[COL_BIN_S][3];
+    __shared__ double accd[COL_BIN_S][4];     // mass, weighted rate, change, second
+    __shared__ int acci[COL_BIN_S][3];        // count, owner count, invalid count
     int gi, s0, s1;
     if (!due_block(due, RATE_OWNERS, gi, s0, s1)) return;
     const int g = due.gid[gi];
@@ -131,49 +137,80 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
         up = 8.0*bitsd(mx);
         if (threadIdx.x == 0 && blockIdx.x == due.bstart[gi]) { lower[g] = lo; upper[g] = up; }
     }
-    if (threadIdx.x < COL_BIN_S*4) accd[threadIdx.x / 4][threadIdx.x % 4] = 0.0;
-    if (threadIdx.x < COL_BIN_S*3) acci[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    if (threadIdx.x < COL_BIN_S*4) (&accd[0][0])[threadIdx.x] = 0.0;
+    if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
     __syncthreads();
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    const int seg = lane / SEG, kl = lane % SEG;
-    for (int base = s0 + wave*NSEG; base < s1; base += WAVES*NSEG)
+    const int lane = threadIdx.x % WAVE;
+    const int s = s0 + int(threadIdx.x);
+    const bool own = s < s1;
+    double2 me = own ? st[s] : make_double2(1.0, 0.0);
+    const double m = own ? measure[s] : 0.0;
+    gdx_env e{};
+    double hg_i = 0.0;
+    if constexpr (COAG_KERNEL == 3) { if (own) e = env[s]; }
+    if constexpr (PARTNER_HG) { if (own) hg_i = hgs[s]; }
+    const owner_terms t = make_owner(e, me.x, m, hg_i);
+    const bool live = own && m > 0.0;
+    const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : s0)*N_K);
+
+    fold3 stack[LOG2K];
+    fold3 total{0.0, 0.0, 0.0};
+    double JM = 0.0;
+    bool V = true;
+    for (int c = 0; c < N_K / 4; ++c)
     {
-        const int s = base + seg;
-        const bool valid = s < s1;
-        double2 me = valid ? st[s] : make_double2(0.0, 0.0);
-        double m = valid ? measure[s] : 0.0;
-        int j = (valid && m > 0.0) ? nbr[(long long)s*N_K + kl] : -1;
-        gdx_env e{};
-        double hg_i = 0.0;
-        if constexpr (COAG_KERNEL == 3) { if (valid) e = env[s]; }
-        if constexpr (PARTNER_HG) { if (valid) hg_i = hgs[s]; }
-        pair_term t = eval_pair(j, e, hg_i, hgs, res, me.x, m, lambda_0);
-        int ok = (isfinite(t.rate) && t.rate >= 0.0) ? 1 : 0;
-        double R = seg_sum<SEG>(t.rate);
-        double J1 = seg_sum<SEG>(t.j1);
-        double J2 = seg_sum<SEG>(t.j2);
-        double JM = seg_max<SEG>(t.jmax);
-        int V = seg_and<SEG>(ok);
-        if (valid && kl == 0)
+        int4 jj = live ? row[c] : make_int4(-1, -1, -1, -1);
+        pair_term p0 = eval_pair_terms(t, jj.x, res, hgs, lambda_0);
+        pair_term p1 = eval_pair_terms(t, jj.y, res, hgs, lambda_0);
+        pair_term p2 = eval_pair_terms(t, jj.z, res, hgs, lambda_0);
+        pair_term p3 = eval_pair_terms(t, jj.w, res, hgs, lambda_0);
+        V = V && isfinite(p0.rate) && p0.rate >= 0.0 && isfinite(p1.rate) && p1.rate >= 0.0
+              && isfinite(p2.rate) && p2.rate >= 0.0 && isfinite(p3.rate) && p3.rate >= 0.0;
+        JM = fmax(fmax(JM, fmax(p0.jmax, p1.jmax)), fmax(p2.jmax, p3.jmax));
+        fold3 part = add3(add3({p0.rate, p0.j1, p0.j2}, {p1.rate, p1.j1, p1.j2}),
+                          add3({p2.rate, p2.j1, p2.j2}, {p3.rate, p3.j1, p3.j2}));
+        bool placed = false;
+#pragma unroll
+        for (int lv = 2; lv < LOG2K; ++lv)
         {
-            cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
-            int sb = size_bin(me.x, lo, up);
-            atomicAdd(&acci[sb][0], 1);
-            atomicAdd(&acci[sb][1], 1);
-            double weight = me.y*grain_mass(me.x);
-            if (!isfinite(weight) || !(weight > 0.0) || !isfinite(R) || R < 0.0 || !isfinite(J1) || J1 < 0.0
-                || !isfinite(J2) || J2 < 0.0)
+            if (!placed)
             {
-                atomicAdd(&acci[sb][2], 1);
-            }
-            else
-            {
-                atomicAdd(&accd[sb][0], weight);
-                atomicAdd(&accd[sb][1], weight*R);
-                atomicAdd(&accd[sb][2], weight*J1);
-                atomicAdd(&accd[sb][3], weight*J2);
+                if (((c >> (lv - 2)) & 1) == 0) { stack[lv] = part; placed = true; }
+                else part = add3(stack[lv], part);
             }
         }
+        if (!placed) total = part;
+    }
+    const double R = total.r, J1 = total.a, J2 = total.b;
+    if (own) cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
+
+    // controller-bin sums, reduced over each distinct bin of the wavefront
+    int sb = own ? size_bin(me.x, lo, up) : -1;
+    double weight = me.y*grain_mass(me.x);
+    bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(R) || R < 0.0 || !isfinite(J1) || J1 < 0.0
+        || !isfinite(J2) || J2 < 0.0;
+    unsigned long long todo = __ballot(own);
+    while (todo)
+    {
+        int leader = __builtin_ctzll(todo);
+        int b = __builtin_amdgcn_readlane(sb, leader);
+        bool in = own && sb == b;
+        bool good = in && !bad;
+        double w = good ? weight : 0.0;
+        double a0 = wave_sum(w), a1 = wave_sum(good ? weight*R : 0.0);
+        double a2 = wave_sum(good ? weight*J1 : 0.0), a3 = wave_sum(good ? weight*J2 : 0.0);
+        int n_in = __popcll(__ballot(in)), n_bad = __popcll(__ballot(in && bad));
+        if (lane == leader)
+        {
+            atomicAdd(&acci[b][0], n_in);
+            atomicAdd(&acci[b][1], n_in);
+            atomicAdd(&acci[b][2], n_bad);
+            atomicAdd(&accd[b][0], a0);
+            atomicAdd(&accd[b][1], a1);
+            atomicAdd(&accd[b][2], a2);
+            atomicAdd(&accd[b][3], a3);
+        }
+        todo &= ~__ballot(in);
     }
     __syncthreads();
     if (threadIdx.x < COL_BIN_S*4)
@@ -190,7 +227,9 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
     }
 }
 
-// per-block audit accumulators of col_audit_bin (col_audit_bin.cu:31-59)
+// ---------------------------------------------------------------------------------------------
+// screen + event chain + audit (col_skip_scan.cu:13-59, col_chain_run.cu:21-404, col_audit_bin.cu:13-60)
+// ---------------------------------------------------------------------------------------------
 struct audit_lds
 {
     double d[COL_BIN_S][11];   // mass, pf, pe, pvf, pve, pg, pvg, touched, event_weight, growth, end_mass
@@ -263,13 +302,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
     {
         const int s = s0 + wave*WAVE + lane;
         const bool valid = s < s1;
-        bool chain = false;
+        bool chain = false, done = false;
+        double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
+        double2 old = make_double2(1.0, 0.0);
         if (valid)
         {
             double4 c = cached[s];
             double m = measure[s];
-            double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
-            bool done = false;
             if (!(m > 0.0))
             {
                 done = true;
@@ -297,16 +336,54 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
                     // otherwise the draw stays uncommitted: the chain repeats the same deviate
                 }
             }
+            chain = !done;
             if (done)
             {
-                double2 old = res[s];
-                double rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
-                audit_owner(a, old, old.x, rate0, end, hazard, j1, j2, jmax, 0, lo, up);
+                old = res[s];
+                rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
             }
-            else
+        }
+        // audit sums of the screened owners (no events: start bin = end bin), reduced per distinct bin
+        int bs = done ? size_bin(old.x, lo, up) : -1;
+        double weight = old.y*grain_mass(old.x);
+        double probability = -expm1(-rate0*end);
+        bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard)
+            || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
+            || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
+        unsigned long long todo = __ballot(done);
+        while (todo)
+        {
+            int leader = __builtin_ctzll(todo);
+            int b = __builtin_amdgcn_readlane(bs, leader);
+            bool in = done && bs == b;
+            bool good = in && !bad;
+            double w = good ? weight : 0.0;
+            double v0 = wave_sum(w);
+            double v1 = wave_sum(w*probability);
+            double v2 = wave_sum(w*hazard);
+            double v3 = wave_sum(w*w*probability*(1.0 - probability));
+            double v4 = wave_sum(w*w*hazard);
+            double v5 = wave_sum(w*j1);
+            double v6 = wave_sum(w*w*j2);
+            double x0 = wave_max(w);
+            double x1 = wave_max(w*jmax);
+            int n_in = __popcll(__ballot(in)), n_bad = __popcll(__ballot(in && bad));
+            if (lane == leader)
             {
-                chain = true;
+                atomicAdd(&a.cnt[b][0], n_in);
+                if (n_bad) atomicAdd(&a.cnt[b][1], n_bad);
+                atomicAdd(&a.d[b][0], v0);
+                atomicAdd(&a.d[b][1], v1);
+                atomicAdd(&a.d[b][2], v2);
+                atomicAdd(&a.d[b][3], v3);
+                atomicAdd(&a.d[b][4], v4);
+                atomicAdd(&a.d[b][5], v5);
+                atomicAdd(&a.d[b][6], v6);
+                atomicAdd(&a.d[b][10], v0);
+                if (x0 > 0.0) atomicMax(&a.mx[b][0], dbits(x0));
+                if (x1 > 0.0) atomicMax(&a.mx[b][1], dbits(x1));
             }
+            todo &= ~__ballot(in);
         }
         unsigned long long ball = __ballot(chain);
         if (chain) queue[wave][__popcll(ball & ((1ull << lane) - 1ull))] = s;
@@ -320,11 +397,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
     int next = NSEG;
     bool loaded = false;
     int s = 0, events = 0;
-    double size_i = 0, numr_i = 0, time = 0, hazard = 0, j1 = 0, j2 = 0, jmax = 0, m = 0, rate0 = 0, hg_i = 0;
+    double numr_i = 0, time = 0, hazard = 0, j1 = 0, j2 = 0, jmax = 0, rate0 = 0;
     double2 old = make_double2(0.0, 0.0);
     uint64_t ctr = 0;
     uint32_t pidx = 0;
     gdx_env e{};
+    owner_terms t{};
     while (true)
     {
         const bool has = item < nq;
@@ -336,26 +414,26 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
             {
                 s = queue[wave][item];
                 double2 cur = st[s];
-                size_i = cur.x;
                 numr_i = cur.y;
                 old = res[s];
-                m = measure[s];
                 ctr = rng[s];
                 pidx = uint32_t(slot_pidx[s]);
                 rate0 = cached[s].x;
+                double hg_i = 0.0;
                 if constexpr (COAG_KERNEL == 3) e = env[s];
                 if constexpr (PARTNER_HG) hg_i = hgs[s];
+                t = make_owner(e, cur.x, measure[s], hg_i);
                 time = hazard = j1 = j2 = jmax = 0.0;
                 events = 0;
                 loaded = true;
             }
-            int j = nbr[(long long)s*N_K + kl];
-            pair_term t = eval_pair(j, e, hg_i, hgs, res, size_i, m, lambda_0);
-            int last = (!isfinite(t.rate) || t.rate < 0.0) ? -2 : (t.rate > 0.0 ? kl : -1);
-            double R = seg_sum<SEG>(t.rate);
-            double J1 = seg_sum<SEG>(t.j1);
-            double J2 = seg_sum<SEG>(t.j2);
-            double JM = seg_max<SEG>(t.jmax);
+            int j = nbr[(long long)s*N_K + bitrev_slot(kl)];
+            pair_term p = eval_pair_terms(t, j, res, hgs, lambda_0);
+            int last = (!isfinite(p.rate) || p.rate < 0.0) ? -2 : (p.rate > 0.0 ? kl : -1);
+            double R = seg_sum<SEG>(p.rate);
+            double J1 = seg_sum<SEG>(p.j1);
+            double J2 = seg_sum<SEG>(p.j2);
+            double JM = seg_max<SEG>(p.jmax);
             int LAST = seg_last<SEG>(last);
             n_pairs += (kl == 0) ? N_K : 0;
             int code = 0;
@@ -392,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
                         int slot = -1;
                         for (int k = 0; k < N_K; ++k)
                         {
-                            double rk = __shfl(t.rate, segbase + k, WAVE);
+                            double rk = __shfl(p.rate, segbase + k, WAVE);
                             if (rk > 0.0)
                             {
                                 cumulative += rk;
@@ -403,19 +481,20 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
                         if (slot < 0) { code = 5; fin = true; }
                         else
                         {
-                            double size_j = __shfl(t.size_j, segbase + slot, WAVE);
-                            double vrel = __shfl(t.vrel, segbase + slot, WAVE);
+                            double size_i = t.size_i;
+                            double size_j = __shfl(p.size_j, segbase + slot, WAVE);
+                            double vrel = __shfl(p.vrel, segbase + slot, WAVE);
                             double mass_before = numr_i*size_i*size_i*size_i;
                             bool high_speed = vrel >= V_FRAG;
                             double sample = high_speed ? col_uniform(pidx, seed, ctr) : 0.0;
                             double size_new = sample_outcome(size_i, size_j, high_speed, sample);
                             numr_i = mass_before / (size_new*size_new*size_new);
-                            size_i = size_new;
                             time = event_time;
                             ++events;
-                            double mass_after = numr_i*size_i*size_i*size_i;
+                            double mass_after = numr_i*size_new*size_new*size_new;
                             double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
-                            if (!isfinite(size_i) || !isfinite(numr_i) || !(size_i > 0.0) || !(numr_i > 0.0)
+                            t = make_owner(e, size_new, t.m, t.hg);
+                            if (!isfinite(size_new) || !isfinite(numr_i) || !(size_new > 0.0) || !(numr_i > 0.0)
                                 || fabs(mass_after - mass_before) > 2.0e-12*mass_scale)
                             { code = 6; fin = true; }
                             else if (events >= EVENT_LIMIT) { code = 7; fin = true; }
@@ -427,10 +506,10 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
             {
                 if (kl == 0)
                 {
-                    if (events > 0) st[s] = make_double2(size_i, numr_i);
+                    if (events > 0) st[s] = make_double2(t.size_i, numr_i);
                     rng[s] = ctr;
                     err_code = max(err_code, code);
-                    if (!code) audit_owner(a, old, size_i, rate0, end, hazard, j1, j2, jmax, events, lo, up);
+                    if (!code) audit_owner(a, old, t.size_i, rate0, end, hazard, j1, j2, jmax, events, lo, up);
                     n_events += events;
                     n_chain += 1;
                 }
