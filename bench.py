@@ -30,11 +30,24 @@ sys.path.insert(0, str(ROOT))
 METRIC = "collision updates/s at 1e7 reps, K=32: learned+batched vs plain, same GPU"
 HBM_PEAK_GBS = 8000.0
 K = 32
-# algorithmic bytes per owner and launch (K = 32): the plain-update budget 20K + 201 split by kernel
-#   publish:   owner (s, N) read 16 + reservoir publish 16
-#   bath_rate: codes 4K + partner (s, N) 16K + measure 8 + owner (y, z) 16 + group/active 5
-#   chain:     owner (s, N) write 16 + clock/events/compensators 44 + RNG state read + write 80
-KERNEL_BYTES = {"publish": 32, "bath_rate": 20*K + 29, "chain": 140}
+# algorithmic bytes (K = 32), per owner of the launch (all owners of the due groups) unless noted
+#   publish:     owner (s, N) read 16 + reservoir publish 16
+#   bath_rate:   codes 4K + partner (s, N) 16K + measure 8 + owner (y, z) 16 + group/active 5
+#   screen:      cached rate 8 + measure 8 + RNG counter read + write 16 + particle index 4 + published (s, N) 16
+#   chain:       per chained owner 140 (state, clock, RNG, sums), plus codes 4 + partner (s, N) 16 per pair term
+#   chain_audit: per chained owner: queue entry 4 + chain sums 36 + published and final state 24
+KERNEL_BYTES = {"publish": 32, "bath_rate": 20*K + 29, "screen": 52}
+CHAIN_OWNER_BYTES, CHAIN_PAIR_BYTES, AUDIT_OWNER_BYTES = 140, 20, 64
+
+
+def kernel_bytes(kernel: str, rec: dict, tot: dict) -> float:
+    """Algorithmic bytes moved by all timed launches of one kernel."""
+    if kernel in KERNEL_BYTES:
+        return rec["owners"]*KERNEL_BYTES[kernel]
+    chain_pairs = tot["pair_evals"] - K*tot["owner_updates"]
+    if kernel == "chain":
+        return tot["chain_owners"]*CHAIN_OWNER_BYTES + chain_pairs*CHAIN_PAIR_BYTES
+    return tot["chain_owners"]*AUDIT_OWNER_BYTES
 
 
 def log(msg: str) -> None:
@@ -144,7 +157,7 @@ def main() -> int:
 
     dom = max(kt, key=lambda k: kt[k]["ms"])
     rec = kt[dom]
-    bytes_per_launch = rec["owners"]*KERNEL_BYTES[dom]/max(1, rec["launches"])
+    bytes_per_launch = kernel_bytes(dom, rec, tot)/max(1, rec["launches"])
     avg_s = rec["ms"]/max(1, rec["launches"])*1e-3
     achieved = bytes_per_launch/avg_s/1e9 if avg_s > 0 else 0.0
     traffic = None
@@ -153,7 +166,7 @@ def main() -> int:
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                "bytes_per_owner": KERNEL_BYTES[dom], "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}
+                "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}
 
     cpu = None
     if not args.no_cpu:

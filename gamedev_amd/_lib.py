@@ -16,7 +16,7 @@ STATUS = {
 }
 DEFER_WRITEBACK = 1
 PROFILE = 2
-KERNELS = ("publish", "bath_rate", "chain")
+KERNELS = ("publish", "bath_rate", "screen", "chain", "chain_audit")
 
 
 class GdxError(RuntimeError):

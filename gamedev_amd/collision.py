@@ -107,9 +107,10 @@ class CollisionOperator:
 
     def kernel_times(self) -> dict:
         """Accumulated HIP-event time (ms), launches and owners per operator kernel (profile=True)."""
-        ms = np.zeros(3, np.float64)
-        la = np.zeros(3, np.int64)
-        ow = np.zeros(3, np.int64)
+        nk = len(_lib.KERNELS)
+        ms = np.zeros(nk, np.float64)
+        la = np.zeros(nk, np.int64)
+        ow = np.zeros(nk, np.int64)
         self._call(self.lib.gdx_col_kernel_times, _np_ptr(ms), _np_ptr(la), _np_ptr(ow))
         return {k: {"ms": float(ms[i]), "launches": int(la[i]), "owners": int(ow[i])}
                 for i, k in enumerate(_lib.KERNELS)}

@@ -222,7 +222,9 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
         gdx_due due = make_due(c, groups, steps);
         hip_check(hipMemsetAsync(c.raw_audit.p, 0, sizeof(gdx_raw_audit)*RAW_BINS, st), "zero audit bins");
         hip_check(hipMemsetAsync(c.err.p, 0, sizeof(int), st), "zero error");
+        timer.time(GDX_COL_KERNEL_SCREEN, due.total, [&] { launch_screen(c, due); });
         timer.time(GDX_COL_KERNEL_CHAIN, due.total, [&] { launch_chain(c, due, lambda0); });
+        timer.time(GDX_COL_KERNEL_AUDIT, due.total, [&] { launch_chain_audit(c, due); });
         int err = 0;
         copy_d2h(&err, c.err.p, sizeof(int), st);
         copy_d2h(raw_audit.data(), c.raw_audit.p, sizeof(gdx_raw_audit)*RAW_BINS, st);

@@ -447,43 +447,50 @@ struct pair_term
     double rate, j1, j2, jmax, vrel, size_j;
 };
 
-__device__ __forceinline__ pair_term eval_pair_terms(const owner_terms &t, int j, const double2 *res,
-                                                     const double *hgs, double lambda_0)
+// partner data of one slot, loaded ahead of its evaluation
+struct partner
+{
+    double size_j, numr_j, hg_j;
+    bool valid;
+};
+
+__device__ __forceinline__ partner load_partner(int j, const double2 *res, const double *hgs)
+{
+    partner q{0.0, 0.0, 0.0, false};
+    if (j < 0) return q;
+    double2 pj = res[j];
+    q.size_j = pj.x;
+    q.numr_j = pj.y;
+    if constexpr (PARTNER_HG) q.hg_j = hgs[j];
+    else (void)hgs;
+    q.valid = true;
+    return q;
+}
+
+__device__ __forceinline__ pair_term eval_pair_terms(const owner_terms &t, const partner &q, double lambda_0)
 {
     pair_term p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (j < 0) return p;
-    double2 pj = res[j];
-    const double size_i = t.size_i, size_j = pj.x, numr_j = pj.y;
+    if (!q.valid) return p;
+    const double size_i = t.size_i, size_j = q.size_j, numr_j = q.numr_j;
     double vrel = 0.0;
     double rate;
     if constexpr (COAG_KERNEL == 0)
     {
-        (void)hgs;
         rate = lambda_0*numr_j;
     }
     else if constexpr (COAG_KERNEL == 1)
     {
-        (void)hgs;
         rate = lambda_0*numr_j*(grain_mass(size_i) + grain_mass(size_j));
     }
     else if constexpr (COAG_KERNEL == 2)
     {
-        (void)hgs;
         rate = lambda_0*numr_j*grain_mass(size_i)*grain_mass(size_j);
     }
     else
     {
         vrel = vrel_pair_terms(t, size_j);
         rate = numr_j*vrel*PI*(size_i + size_j)*(size_i + size_j) / 4.0;
-        if constexpr (N_Z == 1)
-        {
-            double hg_j = hgs[j];
-            rate /= sqrt(2.0*PI*(t.hg*t.hg + hg_j*hg_j));
-        }
-        else
-        {
-            (void)hgs;
-        }
+        if constexpr (N_Z == 1) rate /= sqrt(2.0*PI*(t.hg*t.hg + q.hg_j*q.hg_j));
     }
     double r = div_rn(rate, t.m, t.inv_m);
     double mean = 0.0, second = 0.0, maximum = 0.0;
@@ -539,6 +546,21 @@ __device__ __forceinline__ int seg_and(int v)
 #pragma unroll
     for (int m = W / 2; m > 0; m >>= 1) v = v & __shfl_xor(v, m, W);
     return v;
+}
+
+// gfx950 hands workgroups to its 8 XCDs round-robin; each XCD has its own L2. Launching
+// XCDS*ceil(n/XCDS) groups and mapping group b to logical block (b % XCDS)*per + b / XCDS gives each
+// XCD one contiguous range of logical blocks, so owners that share partners share an L2.
+constexpr int XCDS = 8;
+__host__ __device__ constexpr int xcd_grid(int logical_blocks)
+{
+    return XCDS*((logical_blocks + XCDS - 1) / XCDS);
+}
+__device__ __forceinline__ int xcd_block()
+{
+    int per = int(gridDim.x) / XCDS;
+    int b = int(blockIdx.x);
+    return (b % XCDS)*per + b / XCDS;
 }
 
 __device__ __forceinline__ unsigned long long dbits(double v) { return static_cast<unsigned long long>(__double_as_longlong(v)); }

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/gdx_col.h"
 #include "gdx_model.h"
 
 namespace gdx {
@@ -134,6 +135,11 @@ struct context
     dbuf<gdx_raw_rate> raw_rate;
     dbuf<gdx_raw_audit> raw_audit;
     dbuf<int> err;
+    dbuf<int> queue;           // event owners of one wave, in their group's slot range
+    dbuf<int> qcount;          // per-group queue lengths + the chain work counter
+    dbuf<double4> qsum;        // chain sums (hazard, j1, j2, jmax) per queue position
+    dbuf<int> qevents;         // chain events per queue position, -1 after an error
+    int chain_blocks = 0;      // resident blocks of the persistent chain kernel
     dbuf<unsigned long long> counters;   // events, chain owners, pair evaluations
     std::vector<double> h_lower, h_upper;
 
@@ -144,8 +150,8 @@ struct context
     // optional per-kernel timing (HIP events on the launch stream)
     struct timing_t
     {
-        double ms[3] = {0, 0, 0};
-        long long launches[3] = {0, 0, 0}, owners[3] = {0, 0, 0};
+        double ms[GDX_COL_KERNELS] = {};
+        long long launches[GDX_COL_KERNELS] = {}, owners[GDX_COL_KERNELS] = {};
     } timing;
     std::vector<hipEvent_t> events;
 
@@ -168,7 +174,9 @@ void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int
 void launch_reset_scratch(context &c);
 void launch_publish(context &c, const gdx_due &due);
 void launch_bath_rate(context &c, const gdx_due &due, double lambda_0);
+void launch_screen(context &c, const gdx_due &due);
 void launch_chain(context &c, const gdx_due &due, double lambda_0);
+void launch_chain_audit(context &c, const gdx_due &due);
 void launch_gather_cached(context &c, double *out_particle_order);
 
 }  // namespace gdx

@@ -500,7 +500,7 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
 {
     __shared__ int stacks[TOPK_BLOCK / WAVE][TOPK_STACK];
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    long long m = ((long long)blockIdx.x*(TOPK_BLOCK / WAVE) + wave)*WAVE + lane;
+    long long m = ((long long)xcd_block()*(TOPK_BLOCK / WAVE) + wave)*WAVE + lane;
     bool valid = m < n;
     float4 me = valid ? rec[m] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
     int code = __float_as_int(me.w);
@@ -861,7 +861,7 @@ void geometry_epoch(context &c)
     hip_check(hipMemsetAsync(c.graph.p, 0, sizeof(unsigned long long)*LOCAL_GROUPS, st), "zero graph");
     hip_check(hipMemsetAsync(c.overflow.p, 0, sizeof(unsigned int), st), "zero overflow");
     int waves = blocks_for(n, WAVE);
-    k_col_topk<<<blocks_for(waves, TOPK_BLOCK / WAVE), TOPK_BLOCK, 0, st>>>(
+    k_col_topk<<<xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE)), TOPK_BLOCK, 0, st>>>(
         c.index.rec.p, c.index.nodes.p, n, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p, c.group.p,
         c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p);
     hip_check(hipGetLastError(), "launch top-K");
@@ -888,6 +888,10 @@ void load_state(context &c)
     c.res.alloc(na);
     c.cached.alloc(na);
     c.rng.alloc(na);
+    c.queue.alloc(na);
+    c.qcount.alloc(LOCAL_GROUPS + 1);
+    c.qsum.alloc(na);
+    c.qevents.alloc(na);
     if (COAG_KERNEL == 3) c.env.alloc(na);
     if (PARTNER_HG) c.hgs.alloc(na);
     if (c.n_active > 0)

@@ -16,6 +16,7 @@
 #include "gdx_device.h"
 
 #include <algorithm>
+#include <cstdint>
 
 #pragma clang fp contract(off)
 
@@ -34,7 +35,7 @@ constexpr int EVENT_LIMIT = 1 << 22;
 
 __device__ __forceinline__ bool due_block(const gdx_due &due, int per_block, int &gi, int &s0, int &s1)
 {
-    int b = blockIdx.x;
+    int b = xcd_block();
     gi = 0;
     while (gi < due.n && b >= due.bstart[gi + 1]) ++gi;
     if (gi >= due.n) return false;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
     {
         lo = 0.5*bitsd(scr_min[g]);
         up = 8.0*bitsd(mx);
-        if (threadIdx.x == 0 && blockIdx.x == due.bstart[gi]) { lower[g] = lo; upper[g] = up; }
+        if (threadIdx.x == 0 && xcd_block() == due.bstart[gi]) { lower[g] = lo; upper[g] = up; }
     }
     if (threadIdx.x < COL_BIN_S*4) (&accd[0][0])[threadIdx.x] = 0.0;
     if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
@@ -157,13 +158,15 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
     fold3 total{0.0, 0.0, 0.0};
     double JM = 0.0;
     bool V = true;
+    // the neighbour indices run one chunk ahead of the partner loads
+    int4 jj = live ? row[0] : make_int4(-1, -1, -1, -1);
     for (int c = 0; c < N_K / 4; ++c)
     {
-        int4 jj = live ? row[c] : make_int4(-1, -1, -1, -1);
-        pair_term p0 = eval_pair_terms(t, jj.x, res, hgs, lambda_0);
-        pair_term p1 = eval_pair_terms(t, jj.y, res, hgs, lambda_0);
-        pair_term p2 = eval_pair_terms(t, jj.z, res, hgs, lambda_0);
-        pair_term p3 = eval_pair_terms(t, jj.w, res, hgs, lambda_0);
+        int4 jn = (live && c + 1 < N_K / 4) ? row[c + 1] : make_int4(-1, -1, -1, -1);
+        pair_term p0 = eval_pair_terms(t, load_partner(jj.x, res, hgs), lambda_0);
+        pair_term p1 = eval_pair_terms(t, load_partner(jj.y, res, hgs), lambda_0);
+        pair_term p2 = eval_pair_terms(t, load_partner(jj.z, res, hgs), lambda_0);
+        pair_term p3 = eval_pair_terms(t, load_partner(jj.w, res, hgs), lambda_0);
         V = V && isfinite(p0.rate) && p0.rate >= 0.0 && isfinite(p1.rate) && p1.rate >= 0.0
               && isfinite(p2.rate) && p2.rate >= 0.0 && isfinite(p3.rate) && p3.rate >= 0.0;
         JM = fmax(fmax(JM, fmax(p0.jmax, p1.jmax)), fmax(p2.jmax, p3.jmax));
@@ -180,6 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
             }
         }
         if (!placed) total = part;
+        jj = jn;
     }
     const double R = total.r, J1 = total.a, J2 = total.b;
     if (own) cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
 }
 
 // ---------------------------------------------------------------------------------------------
-// screen + event chain + audit (col_skip_scan.cu:13-59, col_chain_run.cu:21-404, col_audit_bin.cu:13-60)
+// screen, event chains and audit (col_skip_scan.cu:13-59, col_chain_run.cu:21-404, col_audit_bin.cu:13-60)
 // ---------------------------------------------------------------------------------------------
 struct audit_lds
 {
@@ -237,308 +241,14 @@ struct audit_lds
     int cnt[COL_BIN_S][2];     // owner_count, invalid_count
 };
 
-// one owner's audit terms: start bin from the published size, end mass to the end bin
-__device__ __forceinline__ void audit_owner(audit_lds &a, double2 old, double size_end, double rate0, double dur,
-                                            double hazard, double j1, double j2, double jmax, int events, double lo,
-                                            double up)
+__device__ __forceinline__ void clear_audit(audit_lds &a)
 {
-    int bs = size_bin(old.x, lo, up);
-    int be = size_bin(size_end, lo, up);
-    double weight = old.y*grain_mass(old.x);
-    double probability = -expm1(-rate0*dur);
-    double ev = static_cast<double>(events);
-    atomicAdd(&a.cnt[bs][0], 1);
-    if (!isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard) || hazard < 0.0
-        || !isfinite(ev) || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax) || jmax < 0.0
-        || !isfinite(size_end) || !(size_end > 0.0))
-    {
-        atomicAdd(&a.cnt[bs][1], 1);
-        return;
-    }
-    atomicAdd(&a.d[bs][0], weight);
-    atomicAdd(&a.d[bs][1], weight*probability);
-    atomicAdd(&a.d[bs][2], weight*hazard);
-    atomicAdd(&a.d[bs][3], weight*weight*probability*(1.0 - probability));
-    atomicAdd(&a.d[bs][4], weight*weight*hazard);
-    atomicAdd(&a.d[bs][5], weight*j1);
-    atomicAdd(&a.d[bs][6], weight*weight*j2);
-    atomicMax(&a.mx[bs][0], dbits(weight));
-    atomicMax(&a.mx[bs][1], dbits(weight*jmax));
-    if (events > 0) atomicAdd(&a.d[bs][7], weight);
-    atomicAdd(&a.d[bs][8], weight*ev);
-    atomicAdd(&a.d[bs][9], weight*fabs(log(size_end / old.x)));
-    atomicAdd(&a.d[be][10], weight);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, double2 *st, const double2 *res,
-                                                const double4 *cached, const double *measure, const gdx_env *env,
-                                                const double *hgs, const int *slot_pidx, uint64_t *rng,
-                                                uint64_t seed, double lambda_0, const double *lower,
-                                                const double *upper, unsigned long long *scr_min,
-                                                unsigned long long *scr_max, gdx_raw_audit *raw, int *err,
-                                                unsigned long long *counters)
-{
-    __shared__ audit_lds a;
-    __shared__ int queue[WAVES][WAVE];
-    __shared__ unsigned long long stat[3];
-    int gi, s0, s1;
-    if (!due_block(due, CHAIN_OWNERS, gi, s0, s1)) return;
-    const int g = due.gid[gi];
-    const double end = due.dt[gi];
-    const double lo = lower[g], up = upper[g];
     for (int i = threadIdx.x; i < COL_BIN_S*11; i += BLOCK) (&a.d[0][0])[i] = 0.0;
     if (threadIdx.x < COL_BIN_S*2) { (&a.mx[0][0])[threadIdx.x] = 0ull; (&a.cnt[0][0])[threadIdx.x] = 0; }
-    if (threadIdx.x < 3) stat[threadIdx.x] = 0ull;
-    // this group's extrema scratch was consumed by k_bath_rate; clear it for the group's next refresh
-    if (threadIdx.x == 0 && blockIdx.x == due.bstart[gi]) { scr_min[g] = dbits(__builtin_inf()); scr_max[g] = 0ull; }
-    __syncthreads();
+}
 
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    unsigned long long n_events = 0, n_chain = 0, n_pairs = 0;
-    int err_code = 0;
-
-    // ---- screen: one lane per owner (col_skip_scan.cu:19-58) ----
-    int nq;
-    {
-        const int s = s0 + wave*WAVE + lane;
-        const bool valid = s < s1;
-        bool chain = false, done = false;
-        double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
-        double2 old = make_double2(1.0, 0.0);
-        if (valid)
-        {
-            double4 c = cached[s];
-            double m = measure[s];
-            if (!(m > 0.0))
-            {
-                done = true;
-            }
-            else if (isfinite(c.x) && c.x >= 0.0)
-            {
-                if (c.x == 0.0)
-                {
-                    done = true;
-                }
-                else
-                {
-                    uint64_t trial = rng[s];
-                    double wait = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / c.x;
-                    double remaining = end;
-                    if (isfinite(wait) && wait > 0.0 && wait >= remaining)
-                    {
-                        hazard = c.x*remaining;
-                        j1 = c.y*remaining;
-                        j2 = c.z*remaining;
-                        jmax = fmax(0.0, c.w);
-                        done = true;
-                        rng[s] = trial;
-                    }
-                    // otherwise the draw stays uncommitted: the chain repeats the same deviate
-                }
-            }
-            chain = !done;
-            if (done)
-            {
-                old = res[s];
-                rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
-            }
-        }
-        // audit sums of the screened owners (no events: start bin = end bin), reduced per distinct bin
-        int bs = done ? size_bin(old.x, lo, up) : -1;
-        double weight = old.y*grain_mass(old.x);
-        double probability = -expm1(-rate0*end);
-        bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard)
-            || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
-            || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
-        unsigned long long todo = __ballot(done);
-        while (todo)
-        {
-            int leader = __builtin_ctzll(todo);
-            int b = __builtin_amdgcn_readlane(bs, leader);
-            bool in = done && bs == b;
-            bool good = in && !bad;
-            double w = good ? weight : 0.0;
-            double v0 = wave_sum(w);
-            double v1 = wave_sum(w*probability);
-            double v2 = wave_sum(w*hazard);
-            double v3 = wave_sum(w*w*probability*(1.0 - probability));
-            double v4 = wave_sum(w*w*hazard);
-            double v5 = wave_sum(w*j1);
-            double v6 = wave_sum(w*w*j2);
-            double x0 = wave_max(w);
-            double x1 = wave_max(w*jmax);
-            int n_in = __popcll(__ballot(in)), n_bad = __popcll(__ballot(in && bad));
-            if (lane == leader)
-            {
-                atomicAdd(&a.cnt[b][0], n_in);
-                if (n_bad) atomicAdd(&a.cnt[b][1], n_bad);
-                atomicAdd(&a.d[b][0], v0);
-                atomicAdd(&a.d[b][1], v1);
-                atomicAdd(&a.d[b][2], v2);
-                atomicAdd(&a.d[b][3], v3);
-                atomicAdd(&a.d[b][4], v4);
-                atomicAdd(&a.d[b][5], v5);
-                atomicAdd(&a.d[b][6], v6);
-                atomicAdd(&a.d[b][10], v0);
-                if (x0 > 0.0) atomicMax(&a.mx[b][0], dbits(x0));
-                if (x1 > 0.0) atomicMax(&a.mx[b][1], dbits(x1));
-            }
-            todo &= ~__ballot(in);
-        }
-        unsigned long long ball = __ballot(chain);
-        if (chain) queue[wave][__popcll(ball & ((1ull << lane) - 1ull))] = s;
-        nq = __popcll(ball);
-        __builtin_amdgcn_wave_barrier();
-    }
-
-    // ---- event chains: one owner per segment, each run to its bath end (col_chain_run.cu:127-382) ----
-    const int seg = lane / SEG, kl = lane % SEG, segbase = seg*SEG;
-    int item = seg;
-    int next = NSEG;
-    bool loaded = false;
-    int s = 0, events = 0;
-    double numr_i = 0, time = 0, hazard = 0, j1 = 0, j2 = 0, jmax = 0, rate0 = 0;
-    double2 old = make_double2(0.0, 0.0);
-    uint64_t ctr = 0;
-    uint32_t pidx = 0;
-    gdx_env e{};
-    owner_terms t{};
-    while (true)
-    {
-        const bool has = item < nq;
-        if (__ballot(has) == 0ull) break;
-        bool fin = false;
-        if (has)
-        {
-            if (!loaded)
-            {
-                s = queue[wave][item];
-                double2 cur = st[s];
-                numr_i = cur.y;
-                old = res[s];
-                ctr = rng[s];
-                pidx = uint32_t(slot_pidx[s]);
-                rate0 = cached[s].x;
-                double hg_i = 0.0;
-                if constexpr (COAG_KERNEL == 3) e = env[s];
-                if constexpr (PARTNER_HG) hg_i = hgs[s];
-                t = make_owner(e, cur.x, measure[s], hg_i);
-                time = hazard = j1 = j2 = jmax = 0.0;
-                events = 0;
-                loaded = true;
-            }
-            int j = nbr[(long long)s*N_K + bitrev_slot(kl)];
-            pair_term p = eval_pair_terms(t, j, res, hgs, lambda_0);
-            int last = (!isfinite(p.rate) || p.rate < 0.0) ? -2 : (p.rate > 0.0 ? kl : -1);
-            double R = seg_sum<SEG>(p.rate);
-            double J1 = seg_sum<SEG>(p.j1);
-            double J2 = seg_sum<SEG>(p.j2);
-            double JM = seg_max<SEG>(p.jmax);
-            int LAST = seg_last<SEG>(last);
-            n_pairs += (kl == 0) ? N_K : 0;
-            int code = 0;
-            if (LAST == -2) { code = 1; fin = true; }
-            else if (!isfinite(R) || R < 0.0) { code = 2; fin = true; }
-            else if (R == 0.0) { time = end; fin = true; }
-            else
-            {
-                double wait = -log(col_uniform(pidx, seed, ctr)) / R;
-                double remaining = end - time;
-                if (!isfinite(wait) || !(wait > 0.0)) { code = 3; fin = true; }
-                else if (wait >= remaining)
-                {
-                    hazard += R*remaining;
-                    j1 += J1*remaining;
-                    j2 += J2*remaining;
-                    jmax = fmax(jmax, JM);
-                    time = end;
-                    fin = true;
-                }
-                else
-                {
-                    double event_time = time + wait;
-                    if (!(event_time > time)) { code = 4; fin = true; }
-                    else
-                    {
-                        hazard += R*wait;
-                        j1 += J1*wait;
-                        j2 += J2*wait;
-                        jmax = fmax(jmax, JM);
-                        // partner by the cumulative walk in slot order, last positive slot on roundoff
-                        double target = col_uniform(pidx, seed, ctr)*R;
-                        double cumulative = 0.0;
-                        int slot = -1;
-                        for (int k = 0; k < N_K; ++k)
-                        {
-                            double rk = __shfl(p.rate, segbase + k, WAVE);
-                            if (rk > 0.0)
-                            {
-                                cumulative += rk;
-                                if (slot < 0 && cumulative > target) slot = k;
-                            }
-                        }
-                        if (slot < 0) slot = LAST;
-                        if (slot < 0) { code = 5; fin = true; }
-                        else
-                        {
-                            double size_i = t.size_i;
-                            double size_j = __shfl(p.size_j, segbase + slot, WAVE);
-                            double vrel = __shfl(p.vrel, segbase + slot, WAVE);
-                            double mass_before = numr_i*size_i*size_i*size_i;
-                            bool high_speed = vrel >= V_FRAG;
-                            double sample = high_speed ? col_uniform(pidx, seed, ctr) : 0.0;
-                            double size_new = sample_outcome(size_i, size_j, high_speed, sample);
-                            numr_i = mass_before / (size_new*size_new*size_new);
-                            time = event_time;
-                            ++events;
-                            double mass_after = numr_i*size_new*size_new*size_new;
-                            double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
-                            t = make_owner(e, size_new, t.m, t.hg);
-                            if (!isfinite(size_new) || !isfinite(numr_i) || !(size_new > 0.0) || !(numr_i > 0.0)
-                                || fabs(mass_after - mass_before) > 2.0e-12*mass_scale)
-                            { code = 6; fin = true; }
-                            else if (events >= EVENT_LIMIT) { code = 7; fin = true; }
-                        }
-                    }
-                }
-            }
-            if (fin)
-            {
-                if (kl == 0)
-                {
-                    if (events > 0) st[s] = make_double2(t.size_i, numr_i);
-                    rng[s] = ctr;
-                    err_code = max(err_code, code);
-                    if (!code) audit_owner(a, old, t.size_i, rate0, end, hazard, j1, j2, jmax, events, lo, up);
-                    n_events += events;
-                    n_chain += 1;
-                }
-                loaded = false;
-            }
-        }
-        // finished segments take the next queued owners in order
-        unsigned long long fb = __ballot(has && fin && kl == 0);
-        if (has && fin) item = next + __popcll(fb & ((1ull << segbase) - 1ull));
-        next += __popcll(fb);
-    }
-
-    // ---- flush block sums ----
-    if (err_code) atomicMax(err, err_code);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-    {
-        n_events += __shfl_xor(n_events, o, WAVE);
-        n_chain += __shfl_xor(n_chain, o, WAVE);
-        n_pairs += __shfl_xor(n_pairs, o, WAVE);
-    }
-    if (lane == 0)
-    {
-        atomicAdd(&stat[0], n_events);
-        atomicAdd(&stat[1], n_chain);
-        atomicAdd(&stat[2], n_pairs);
-    }
-    __syncthreads();
-    gdx_raw_audit *dst = raw + g*COL_BIN_S;
+__device__ __forceinline__ void flush_audit(const audit_lds &a, gdx_raw_audit *dst)
+{
     for (int i = threadIdx.x; i < COL_BIN_S*11; i += BLOCK)
     {
         int b = i / 11, f = i % 11;
@@ -555,7 +265,392 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *nbr, do
         int cnt = a.cnt[b][f];
         if (cnt) atomicAdd(f == 0 ? &dst[b].owner_count : &dst[b].invalid_count, cnt);
     }
-    if (threadIdx.x < 3 && stat[threadIdx.x]) atomicAdd(&counters[threadIdx.x], stat[threadIdx.x]);
+}
+
+// audit terms of one owner per lane (col_audit_bin.cu:31-59): every field but the end mass goes to the
+// start bin bs, the end mass to the end bin be; each distinct bin of the wavefront takes one LDS atomic
+// per field. Without events (EVENTS false) the end bin is the start bin and the event fields are zero.
+template <bool EVENTS>
+__device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int be, double weight,
+                                           double probability, double hazard, double j1, double j2, double jmax,
+                                           double ev, double growth, bool bad)
+{
+    const int lane = threadIdx.x % WAVE;
+    unsigned long long todo = __ballot(take);
+    while (todo)
+    {
+        int leader = __builtin_ctzll(todo);
+        int b = __builtin_amdgcn_readlane(bs, leader);
+        bool in = take && bs == b;
+        bool good = in && !bad;
+        double w = good ? weight : 0.0;
+        double v0 = wave_sum(w);
+        double v1 = wave_sum(good ? w*probability : 0.0);
+        double v2 = wave_sum(good ? w*hazard : 0.0);
+        double v3 = wave_sum(good ? w*w*probability*(1.0 - probability) : 0.0);
+        double v4 = wave_sum(good ? w*w*hazard : 0.0);
+        double v5 = wave_sum(good ? w*j1 : 0.0);
+        double v6 = wave_sum(good ? w*w*j2 : 0.0);
+        double x0 = wave_max(w);
+        double x1 = wave_max(good ? w*jmax : 0.0);
+        double v7 = 0.0, v8 = 0.0, v9 = 0.0;
+        if constexpr (EVENTS)
+        {
+            v7 = wave_sum(good && ev > 0.0 ? w : 0.0);
+            v8 = wave_sum(good ? w*ev : 0.0);
+            v9 = wave_sum(good ? w*growth : 0.0);
+        }
+        int n_in = __popcll(__ballot(in)), n_bad = __popcll(__ballot(in && bad));
+        if (lane == leader)
+        {
+            atomicAdd(&a.cnt[b][0], n_in);
+            if (n_bad) atomicAdd(&a.cnt[b][1], n_bad);
+            atomicAdd(&a.d[b][0], v0);
+            atomicAdd(&a.d[b][1], v1);
+            atomicAdd(&a.d[b][2], v2);
+            atomicAdd(&a.d[b][3], v3);
+            atomicAdd(&a.d[b][4], v4);
+            atomicAdd(&a.d[b][5], v5);
+            atomicAdd(&a.d[b][6], v6);
+            if (x0 > 0.0) atomicMax(&a.mx[b][0], dbits(x0));
+            if (x1 > 0.0) atomicMax(&a.mx[b][1], dbits(x1));
+            if constexpr (EVENTS)
+            {
+                atomicAdd(&a.d[b][7], v7);
+                atomicAdd(&a.d[b][8], v8);
+                atomicAdd(&a.d[b][9], v9);
+            }
+            else
+            {
+                atomicAdd(&a.d[b][10], v0);
+            }
+        }
+        todo &= ~__ballot(in);
+    }
+    if constexpr (EVENTS)
+    {
+        todo = __ballot(take && !bad);
+        while (todo)
+        {
+            int leader = __builtin_ctzll(todo);
+            int b = __builtin_amdgcn_readlane(be, leader);
+            bool in = take && !bad && be == b;
+            double v = wave_sum(in ? weight : 0.0);
+            if (lane == leader) atomicAdd(&a.d[b][10], v);
+            todo &= ~__ballot(in);
+        }
+    }
+}
+
+// screen (col_skip_scan.cu:13-59): one lane per owner. Owners with no event in the bath finish here
+// and enter the audit sums; the rest are appended to their group's queue region [beg, beg + cnt).
+__global__ __launch_bounds__(BLOCK) void k_screen(gdx_due due, const double4 *cached, const double *measure,
+                                                 const double2 *res, const int *slot_pidx, uint64_t *rng,
+                                                 uint64_t seed, const double *lower, const double *upper,
+                                                 unsigned long long *scr_min, unsigned long long *scr_max,
+                                                 gdx_raw_audit *raw, int *queue, int *qcount)
+{
+    __shared__ audit_lds a;
+    int gi, s0, s1;
+    if (!due_block(due, CHAIN_OWNERS, gi, s0, s1)) return;
+    const int g = due.gid[gi];
+    const double end = due.dt[gi];
+    const double lo = lower[g], up = upper[g];
+    clear_audit(a);
+    // this group's extrema scratch was consumed by k_bath_rate; clear it for the group's next refresh
+    if (threadIdx.x == 0 && xcd_block() == due.bstart[gi]) { scr_min[g] = dbits(__builtin_inf()); scr_max[g] = 0ull; }
+    __syncthreads();
+
+    const int lane = threadIdx.x % WAVE;
+    const int s = s0 + int(threadIdx.x);
+    const bool valid = s < s1;
+    bool chain = false, done = false;
+    double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
+    double2 old = make_double2(1.0, 0.0);
+    if (valid)
+    {
+        double4 c = cached[s];
+        double m = measure[s];
+        if (!(m > 0.0))
+        {
+            done = true;
+        }
+        else if (isfinite(c.x) && c.x >= 0.0)
+        {
+            if (c.x == 0.0)
+            {
+                done = true;
+            }
+            else
+            {
+                uint64_t trial = rng[s];
+                double wait = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / c.x;
+                double remaining = end;
+                if (isfinite(wait) && wait > 0.0 && wait >= remaining)
+                {
+                    hazard = c.x*remaining;
+                    j1 = c.y*remaining;
+                    j2 = c.z*remaining;
+                    jmax = fmax(0.0, c.w);
+                    done = true;
+                    rng[s] = trial;
+                }
+                // otherwise the draw stays uncommitted: the chain repeats the same deviate
+            }
+        }
+        chain = !done;
+        if (done)
+        {
+            old = res[s];
+            rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
+        }
+    }
+    int bs = done ? size_bin(old.x, lo, up) : -1;
+    double weight = old.y*grain_mass(old.x);
+    double probability = -expm1(-rate0*end);
+    bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard)
+        || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
+        || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
+    audit_wave<false>(a, done, bs, bs, weight, probability, hazard, j1, j2, jmax, 0.0, 0.0, bad);
+    // queue the owners with an event, one atomic per wavefront
+    unsigned long long ball = __ballot(chain);
+    if (ball)
+    {
+        int leader = __builtin_ctzll(ball);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(qcount + g, __popcll(ball));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (chain) queue[due.beg[gi] + base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
+    }
+    __syncthreads();
+    flush_audit(a, raw + g*COL_BIN_S);
+}
+
+// event chains (col_chain_run.cu:127-382): a persistent grid pulls the queued owners from one global
+// counter. Each K-lane segment of a wavefront runs one owner to its bath end, then takes the next;
+// the chain sums go to the owner's queue position for the audit pass.
+__global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount, const int *queue, const int *nbr,
+                                                double2 *st, const double2 *res, const double4 *cached,
+                                                const double *measure, const gdx_env *env, const double *hgs,
+                                                const int *slot_pidx, uint64_t *rng, uint64_t seed, double lambda_0,
+                                                double4 *qsum, int *qevents, int *err,
+                                                unsigned long long *counters, int *work)
+{
+    __shared__ int prefix[WAVES][65];
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    // prefix of the due groups' queue lengths, one lane per due group
+    int inc = lane < due.n ? qcount[due.gid[lane]] : 0;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1)
+    {
+        int t = __shfl_up(inc, o, WAVE);
+        if (lane >= o) inc += t;
+    }
+    prefix[wave][lane + 1] = inc;
+    if (lane == 0) prefix[wave][0] = 0;
+    const int total = __shfl(inc, WAVE - 1, WAVE);
+    __builtin_amdgcn_wave_barrier();
+
+    const int seg = lane / SEG, kl = lane % SEG, segbase = seg*SEG;
+    const unsigned long long seg_lead = (N_K == 64) ? 1ull : (N_K == 32) ? 0x0000000100000001ull
+                                                                         : 0x0001000100010001ull;
+    unsigned long long n_events = 0, n_chain = 0, n_pairs = 0;
+    int err_code = 0;
+    bool more = true, active = false;
+    int s = 0, pos = 0, gi = 0, events = 0;
+    double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
+    uint64_t ctr = 0;
+    uint32_t pidx = 0;
+    gdx_env e{};
+    owner_terms t{};
+    while (true)
+    {
+        // segments without an owner take the next queued owners, one counter atomic per wavefront
+        unsigned long long need = __ballot(!active) & seg_lead;
+        if (more && need)
+        {
+            int n = __popcll(need);
+            int base = 0;
+            if (lane == 0) base = atomicAdd(work, n);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base + n >= total) more = false;
+            if (!active)
+            {
+                int v = base + __popcll(need & ((1ull << segbase) - 1ull));
+                if (v < total)
+                {
+                    gi = 0;
+                    while (prefix[wave][gi + 1] <= v) ++gi;
+                    pos = due.beg[gi] + (v - prefix[wave][gi]);
+                    s = queue[pos];
+                    end = due.dt[gi];
+                    double2 cur = st[s];
+                    numr_i = cur.y;
+                    ctr = rng[s];
+                    pidx = uint32_t(slot_pidx[s]);
+                    double hg_i = 0.0;
+                    if constexpr (COAG_KERNEL == 3) e = env[s];
+                    if constexpr (PARTNER_HG) hg_i = hgs[s];
+                    t = make_owner(e, cur.x, measure[s], hg_i);
+                    time = hazard = j1 = j2 = jmax = 0.0;
+                    events = 0;
+                    active = true;
+                }
+            }
+        }
+        if (!__ballot(active)) break;
+        if (!active) continue;
+        partner q = load_partner(nbr[(long long)s*N_K + bitrev_slot(kl)], res, hgs);
+        pair_term p = eval_pair_terms(t, q, lambda_0);
+        int last = (!isfinite(p.rate) || p.rate < 0.0) ? -2 : (p.rate > 0.0 ? kl : -1);
+        double R = seg_sum<SEG>(p.rate);
+        double J1 = seg_sum<SEG>(p.j1);
+        double J2 = seg_sum<SEG>(p.j2);
+        double JM = seg_max<SEG>(p.jmax);
+        int LAST = seg_last<SEG>(last);
+        n_pairs += (kl == 0) ? N_K : 0;
+        bool fin = false;
+        int code = 0;
+        if (LAST == -2) { code = 1; fin = true; }
+        else if (!isfinite(R) || R < 0.0) { code = 2; fin = true; }
+        else if (R == 0.0) { time = end; fin = true; }
+        else
+        {
+            double wait = -log(col_uniform(pidx, seed, ctr)) / R;
+            double remaining = end - time;
+            if (!isfinite(wait) || !(wait > 0.0)) { code = 3; fin = true; }
+            else if (wait >= remaining)
+            {
+                hazard += R*remaining;
+                j1 += J1*remaining;
+                j2 += J2*remaining;
+                jmax = fmax(jmax, JM);
+                time = end;
+                fin = true;
+            }
+            else
+            {
+                double event_time = time + wait;
+                if (!(event_time > time)) { code = 4; fin = true; }
+                else
+                {
+                    hazard += R*wait;
+                    j1 += J1*wait;
+                    j2 += J2*wait;
+                    jmax = fmax(jmax, JM);
+                    // partner by the cumulative walk in slot order, last positive slot on roundoff
+                    double target = col_uniform(pidx, seed, ctr)*R;
+                    double cumulative = 0.0;
+                    int slot = -1;
+                    for (int k = 0; k < N_K; ++k)
+                    {
+                        double rk = __shfl(p.rate, segbase + k, WAVE);
+                        if (rk > 0.0)
+                        {
+                            cumulative += rk;
+                            if (slot < 0 && cumulative > target) slot = k;
+                        }
+                    }
+                    if (slot < 0) slot = LAST;
+                    if (slot < 0) { code = 5; fin = true; }
+                    else
+                    {
+                        double size_i = t.size_i;
+                        double size_j = __shfl(p.size_j, segbase + slot, WAVE);
+                        double vrel = __shfl(p.vrel, segbase + slot, WAVE);
+                        double mass_before = numr_i*size_i*size_i*size_i;
+                        bool high_speed = vrel >= V_FRAG;
+                        double sample = high_speed ? col_uniform(pidx, seed, ctr) : 0.0;
+                        double size_new = sample_outcome(size_i, size_j, high_speed, sample);
+                        numr_i = mass_before / (size_new*size_new*size_new);
+                        time = event_time;
+                        ++events;
+                        double mass_after = numr_i*size_new*size_new*size_new;
+                        double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
+                        t = make_owner(e, size_new, t.m, t.hg);
+                        if (!isfinite(size_new) || !isfinite(numr_i) || !(size_new > 0.0) || !(numr_i > 0.0)
+                            || fabs(mass_after - mass_before) > 2.0e-12*mass_scale)
+                        { code = 6; fin = true; }
+                        else if (events >= EVENT_LIMIT) { code = 7; fin = true; }
+                    }
+                }
+            }
+        }
+        if (fin)
+        {
+            if (kl == 0)
+            {
+                if (events > 0) st[s] = make_double2(t.size_i, numr_i);
+                rng[s] = ctr;
+                err_code = max(err_code, code);
+                qsum[pos] = make_double4(hazard, j1, j2, jmax);
+                qevents[pos] = code ? -1 : events;
+                n_events += events;
+                n_chain += 1;
+            }
+            active = false;
+        }
+    }
+    if (err_code) atomicMax(err, err_code);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        n_events += __shfl_xor(n_events, o, WAVE);
+        n_chain += __shfl_xor(n_chain, o, WAVE);
+        n_pairs += __shfl_xor(n_pairs, o, WAVE);
+    }
+    if (lane == 0)
+    {
+        if (n_events) atomicAdd(&counters[0], n_events);
+        if (n_chain) atomicAdd(&counters[1], n_chain);
+        if (n_pairs) atomicAdd(&counters[2], n_pairs);
+    }
+}
+
+// audit of the chained owners, one lane per queue position of each due group (col_audit_bin.cu:13-60)
+__global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_due due, const int *qcount, const int *queue,
+                                                      const double2 *st, const double2 *res, const double4 *cached,
+                                                      const double4 *qsum, const int *qevents, const double *lower,
+                                                      const double *upper, gdx_raw_audit *raw)
+{
+    __shared__ audit_lds a;
+    int gi, s0, s1;
+    if (!due_block(due, CHAIN_OWNERS, gi, s0, s1)) return;
+    const int g = due.gid[gi];
+    const int q1 = due.beg[gi] + qcount[g];
+    if (s0 >= q1) return;
+    const double end = due.dt[gi];
+    const double lo = lower[g], up = upper[g];
+    clear_audit(a);
+    __syncthreads();
+    const int pos = s0 + int(threadIdx.x);
+    // owners that ended in an error are left out: the operator call fails on them
+    bool take = pos < s1 && pos < q1 && qevents[pos] >= 0;
+    double2 old = make_double2(1.0, 0.0);
+    double size_end = 1.0, rate0 = 0.0;
+    double4 sum = make_double4(0.0, 0.0, 0.0, 0.0);
+    int events = 0;
+    if (take)
+    {
+        int s = queue[pos];
+        old = res[s];
+        size_end = st[s].x;
+        rate0 = cached[s].x;
+        sum = qsum[pos];
+        events = qevents[pos];
+    }
+    int bs = take ? size_bin(old.x, lo, up) : -1;
+    int be = take ? size_bin(size_end, lo, up) : -1;
+    double weight = old.y*grain_mass(old.x);
+    double probability = -expm1(-rate0*end);
+    double ev = static_cast<double>(events);
+    bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(sum.x) || sum.x < 0.0
+        || !isfinite(ev) || !isfinite(sum.y) || sum.y < 0.0 || !isfinite(sum.z) || sum.z < 0.0 || !isfinite(sum.w)
+        || sum.w < 0.0 || !isfinite(size_end) || !(size_end > 0.0);
+    double growth = fabs(log(size_end / old.x));
+    audit_wave<true>(a, take, bs, be, weight, probability, sum.x, sum.y, sum.z, sum.w, ev, growth, bad);
+    __syncthreads();
+    flush_audit(a, raw + g*COL_BIN_S);
 }
 
 __global__ __launch_bounds__(256) void k_cached_to_particles(const double4 *cached, const int *slot_pidx, int n_active,
@@ -593,7 +688,7 @@ void launch_publish(context &c, const gdx_due &due_in)
     gdx_due due = due_in;
     int nb = blocks_for_due(due, PUBLISH_OWNERS);
     if (!nb) return;
-    k_publish<<<nb, BLOCK, 0, c.stream>>>(due, c.st.p, c.res.p, c.scr_min.p, c.scr_max.p);
+    k_publish<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.st.p, c.res.p, c.scr_min.p, c.scr_max.p);
     hip_check(hipGetLastError(), "launch publish");
 }
 
@@ -602,21 +697,51 @@ void launch_bath_rate(context &c, const gdx_due &due_in, double lambda_0)
     gdx_due due = due_in;
     int nb = blocks_for_due(due, RATE_OWNERS);
     if (!nb) return;
-    k_bath_rate<<<nb, BLOCK, 0, c.stream>>>(due, c.nbr.p, c.st.p, c.res.p, c.measure.p, c.env.p, c.hgs.p, lambda_0,
+    k_bath_rate<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.nbr.p, c.st.p, c.res.p, c.measure.p, c.env.p, c.hgs.p, lambda_0,
                                            c.scr_min.p, c.scr_max.p, c.lower.p, c.upper.p, c.cached.p,
                                            c.raw_rate.p);
     hip_check(hipGetLastError(), "launch bath rate");
 }
 
-void launch_chain(context &c, const gdx_due &due_in, double lambda_0)
+void launch_screen(context &c, const gdx_due &due_in)
+{
+    gdx_due due = due_in;
+    int nb = blocks_for_due(due, CHAIN_OWNERS);
+    hip_check(hipMemsetAsync(c.qcount.p, 0, sizeof(int)*(LOCAL_GROUPS + 1), c.stream), "zero queue counters");
+    if (!nb) return;
+    k_screen<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.cached.p, c.measure.p, c.res.p, c.slot_pidx.p, c.rng.p,
+                                                  c.seed, c.lower.p, c.upper.p, c.scr_min.p, c.scr_max.p,
+                                                  c.raw_audit.p, c.queue.p, c.qcount.p);
+    hip_check(hipGetLastError(), "launch screen");
+}
+
+void launch_chain(context &c, const gdx_due &due, double lambda_0)
+{
+    if (!due.total) return;
+    // a persistent grid of as many blocks as stay resident together, fewer for small waves
+    if (!c.chain_blocks)
+    {
+        int per_cu = 0, cus = 0, dev = 0;
+        hip_check(hipGetDevice(&dev), "get device");
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain, BLOCK, 0), "chain occupancy");
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+        c.chain_blocks = std::max(1, per_cu*cus);
+    }
+    int grid = std::max(1, std::min(c.chain_blocks, (due.total + 4*NSEG - 1) / (4*NSEG)));
+    k_chain<<<grid, BLOCK, 0, c.stream>>>(due, c.qcount.p, c.queue.p, c.nbr.p, c.st.p, c.res.p, c.cached.p,
+                                         c.measure.p, c.env.p, c.hgs.p, c.slot_pidx.p, c.rng.p, c.seed, lambda_0,
+                                         c.qsum.p, c.qevents.p, c.err.p, c.counters.p, c.qcount.p + LOCAL_GROUPS);
+    hip_check(hipGetLastError(), "launch chain");
+}
+
+void launch_chain_audit(context &c, const gdx_due &due_in)
 {
     gdx_due due = due_in;
     int nb = blocks_for_due(due, CHAIN_OWNERS);
     if (!nb) return;
-    k_chain<<<nb, BLOCK, 0, c.stream>>>(due, c.nbr.p, c.st.p, c.res.p, c.cached.p, c.measure.p, c.env.p, c.hgs.p,
-                                       c.slot_pidx.p, c.rng.p, c.seed, lambda_0, c.lower.p, c.upper.p, c.scr_min.p,
-                                       c.scr_max.p, c.raw_audit.p, c.err.p, c.counters.p);
-    hip_check(hipGetLastError(), "launch chain");
+    k_chain_audit<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.qcount.p, c.queue.p, c.st.p, c.res.p, c.cached.p,
+                                                       c.qsum.p, c.qevents.p, c.lower.p, c.upper.p, c.raw_audit.p);
+    hip_check(hipGetLastError(), "launch chain audit");
 }
 
 void launch_gather_cached(context &c, double *out)

@@ -44,8 +44,10 @@ extern "C" {
 /* kernels timed under GDX_COL_PROFILE */
 #define GDX_COL_KERNEL_PUBLISH   0
 #define GDX_COL_KERNEL_RATE      1
-#define GDX_COL_KERNEL_CHAIN     2
-#define GDX_COL_KERNELS          3
+#define GDX_COL_KERNEL_SCREEN    2
+#define GDX_COL_KERNEL_CHAIN     3
+#define GDX_COL_KERNEL_AUDIT     4
+#define GDX_COL_KERNELS          5
 
 typedef struct gdx_col_ctx *gdx_col_handle;
 

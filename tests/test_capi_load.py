@@ -40,3 +40,14 @@ def test_argument_errors_without_gpu():
     assert lib.gdx_col_create(C.byref(cfg), C.byref(h)) == 15           # GDX_COL_ERR_ARG
     assert b"representative count" in lib.gdx_col_last_error(None)
     assert lib.gdx_col_operator(None, C.c_double(1.0), None) == 15
+
+
+def test_kernel_timer_ids_match_header():
+    """The host mirror sizes its kernel-time buffers by the header's GDX_COL_KERNELS."""
+    from gamedev_amd import _lib
+    text = HEADER.read_text()
+    n = int(re.search(r"#define\s+GDX_COL_KERNELS\s+(\d+)", text).group(1))
+    ids = {m.group(1).lower(): int(m.group(2))
+           for m in re.finditer(r"#define\s+GDX_COL_KERNEL_(\w+)\s+(\d+)", text)}
+    assert len(_lib.KERNELS) == n == len(ids)
+    assert sorted(ids.values()) == list(range(n))

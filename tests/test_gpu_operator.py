@@ -146,3 +146,24 @@ def test_swarm_record_binding_matches_soa(gpu):
     out = drec.cpu().numpy()
     assert np.array_equal(out[:, 6], d["size"].cpu().numpy()) and np.array_equal(out[:, 7], d["numr"].cpu().numpy())
     assert np.array_equal(out[:, :6], rec[:, :6])      # positions and velocities untouched
+
+
+def test_profile_mode_same_result(gpu):
+    """Kernel timing (HIP events around every launch) changes nothing in the evolved state and reports
+    a launch count and a positive time for every operator kernel."""
+    from gamedev_amd import CollisionOperator, _lib, synth
+    outs = []
+    for profile in (False, True):
+        st = synth.fiducial_3d_state(40000, seed=21)
+        dev = synth.to_device(st, gpu)
+        op = CollisionOperator("fiducial_3d", len(st["x"]), st["total_mass"], seed=3, profile=profile)
+        op.bind_soa(dev["x"], dev["y"], dev["z"], dev["size"], dev["numr"])
+        op.geometry()
+        ev = sum(op.evolve(0.05)["events"] for _ in range(2))
+        outs.append((dev["size"].cpu().numpy().tobytes(), op.rng().tobytes(), ev))
+        if profile:
+            kt = op.kernel_times()
+            assert set(kt) == set(_lib.KERNELS)
+            for name, rec in kt.items():
+                assert rec["launches"] > 0 and rec["ms"] > 0.0 and rec["owners"] > 0, name
+    assert outs[0] == outs[1] and outs[0][2] > 0
