@@ -102,6 +102,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--geometry-steps", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (a known-size dispatch that calibrates "
+                         "hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the collision operator shards by independent realizations or "
@@ -112,6 +115,14 @@ def main() -> int:
     from gamedev_amd import CollisionOperator, synth
 
     dev = torch.device("cuda:0")
+    if args.copy_mib > 0:
+        a = torch.ones(args.copy_mib*(1 << 20)//8, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log(f"copied {args.copy_mib} MiB")
     log(f"generating {args.n} representatives")
     st = synth.fiducial_3d_state(args.n, seed=0)
     d = synth.to_device(st, dev)
@@ -163,7 +174,10 @@ def main() -> int:
     traffic = None
     tfile = ROOT/"profiles"/f"traffic_{dom}_fiducial_3d.json"
     if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        # HBM bytes per owner from the FETCH_SIZE / WRITE_SIZE counter runs, scaled to this run's launches
+        per_owner = json.loads(tfile.read_text()).get("bytes_per_owner")
+        if per_owner:
+            traffic = per_owner*rec["owners"]/max(1, rec["launches"])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}

@@ -37,6 +37,21 @@ def lib_path(model: str) -> Path:
     return LIB / f"libgdx_col_{model}.so"
 
 
+def check_path(model: str) -> Path:
+    return LIB / f"pair_forms_check_{model}"
+
+
+def build_check(model: str) -> Path:
+    """Host-only build of the pair-form check program of one model (no GPU code, no HIP runtime use)."""
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = check_path(model)
+    tmp = out.with_suffix(".tmp")
+    _run([HIPCC, "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", "-Wall", "-Wno-unused-function",
+          f"-I{MOD / model}", f"-I{CSRC}", str(CSRC / "pair_forms_check.cpp"), "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def _run(cmd: list[str]) -> None:
     proc = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (proc.stdout + proc.stderr).splitlines() if "packed-fp32-ops" not in l)
@@ -62,7 +77,10 @@ def build(targets: list[str] | None = None, jobs: int | None = None) -> list[Pat
     jobs = jobs or max(1, min(len(targets)*len(SOURCES), os.cpu_count() or 4, 8))
     with cf.ThreadPoolExecutor(jobs) as ex:
         futs = {(m, s): ex.submit(_compile, m, s) for m in targets for s in SOURCES}
+        checks = [ex.submit(build_check, m) for m in targets]
         objs = {k: f.result() for k, f in futs.items()}
+        for f in checks:
+            f.result()
     out = []
     for m in targets:
         so = lib_path(m)

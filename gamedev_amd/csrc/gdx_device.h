@@ -16,6 +16,10 @@
 
 #pragma clang fp contract(off)
 
+// the pair physics also compiles for the host, where a check program compares its hoisted and
+// merged forms with the direct per-pair forms bit for bit
+#define GDX_FN __host__ __device__ __forceinline__
+
 namespace gdx {
 
 // ---------------------------------------------------------------------------------------------
@@ -59,16 +63,16 @@ __device__ __forceinline__ double col_uniform(uint32_t pidx, uint64_t seed, uint
 // ---------------------------------------------------------------------------------------------
 // disk and grain profiles (inc/swarm/param_phys.cuh:16-210, param_grid.cuh:23-27)
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ double grain_mass(double size) { return PI*RHO_0*size*size*size / 6.0; }
-__device__ __forceinline__ double omegaK(double R) { return sqrt(G*M_S / R / R / R); }
-__device__ __forceinline__ double hg(double R) { return ASPR_0*pow(R / R_0, 0.5*(IDX_Q + 1.0)); }
-__device__ __forceinline__ double eta(double R, double Z, double h)
+GDX_FN double grain_mass(double size) { return PI*RHO_0*size*size*size / 6.0; }
+GDX_FN double omegaK(double R) { return sqrt(G*M_S / R / R / R); }
+GDX_FN double hg(double R) { return ASPR_0*pow(R / R_0, 0.5*(IDX_Q + 1.0)); }
+GDX_FN double eta(double R, double Z, double h)
 { return -0.5*((IDX_P + 0.5*IDX_Q - 1.5)*h*h + IDX_Q*(1.0 - R / sqrt(R*R + Z*Z))); }
-__device__ __forceinline__ double gas_strat(double R, double Z, double h)
+GDX_FN double gas_strat(double R, double Z, double h)
 { return exp((R / sqrt(R*R + Z*Z) - 1.0) / (h*h)); }
-__device__ __forceinline__ double sigma_g(double R) { return SIGMA_0*pow(R / R_0, IDX_P); }
-__device__ __forceinline__ double sound_speed(double R, double h) { return h*omegaK(R)*R; }
-__device__ __forceinline__ double alpha_of(double R, double h)
+GDX_FN double sigma_g(double R) { return SIGMA_0*pow(R / R_0, IDX_P); }
+GDX_FN double sound_speed(double R, double h) { return h*omegaK(R)*R; }
+GDX_FN double alpha_of(double R, double h)
 {
 #ifdef CONST_NU
     return NU / (h*h*R*R*omegaK(R));
@@ -77,11 +81,11 @@ __device__ __forceinline__ double alpha_of(double R, double h)
     return ALPHA;
 #endif
 }
-__device__ __forceinline__ double cyl_R(double y, double z) { return (N_Z > 1) ? y*sin(z) : y; }
-__device__ __forceinline__ double cyl_Z(double y, double z) { return (N_Z > 1) ? y*cos(z) : 0.0; }
+GDX_FN double cyl_R(double y, double z) { return (N_Z > 1) ? y*sin(z) : y; }
+GDX_FN double cyl_Z(double y, double z) { return (N_Z > 1) ? y*cos(z) : 0.0; }
 
 // _get_re_inv_sqrt (inc/swarm/_collision.cuh:136-150)
-__device__ __forceinline__ double re_inv_sqrt(double alpha, double sig)
+GDX_FN double re_inv_sqrt(double alpha, double sig)
 {
 #ifdef CODE_UNIT
     double alpha_0 = alpha_of(R_0, ASPR_0);
@@ -93,7 +97,7 @@ __device__ __forceinline__ double re_inv_sqrt(double alpha, double sig)
 }
 
 // owner environment: every owner-only factor of _get_vrel_pair (_collision.cuh:285-317), in order
-__device__ __forceinline__ gdx_env make_env(double y, double z)
+GDX_FN gdx_env make_env(double y, double z)
 {
     double R = cyl_R(y, z);
     double Z = cyl_Z(y, z);
@@ -119,7 +123,7 @@ __device__ __forceinline__ gdx_env make_env(double y, double z)
 }
 
 // _get_stokes for analytic gas (param_phys.cuh:170-206): divide by the radial and vertical scalings
-__device__ __forceinline__ double stokes(const gdx_env &e, double size)
+GDX_FN double stokes(const gdx_env &e, double size)
 {
     double st = STOKES_0*(size / S_0);
 #ifndef CONST_ST
@@ -130,7 +134,7 @@ __device__ __forceinline__ double stokes(const gdx_env &e, double size)
 }
 
 // Ormel & Cuzzi (2007) turbulent relative speed, regimes of _get_vrel_t (_collision.cuh:152-273)
-__device__ __forceinline__ double vrel_t(const gdx_env &e, double stokes_i, double stokes_j)
+GDX_FN double vrel_t(const gdx_env &e, double stokes_i, double stokes_j)
 {
     double ris = e.re_inv_sqrt;
     double vg_sq = e.vg_sq;
@@ -181,8 +185,73 @@ __device__ __forceinline__ double vrel_t(const gdx_env &e, double stokes_i, doub
     return v < 0.0 ? __builtin_nan("") : sqrt(v);
 }
 
+// vrel_t for the kernels: regimes 3, 4 and 5 share their first three divisions, with operands
+// selected per lane (regime 3: (sl - ss)/(sl + ss), sl/(1 + y_a), ss^2/(ss + y_a sl); regimes 4-5:
+// eps = ss/sl, 2/(1 + eps), eps^3/(y + eps)), so a wavefront whose lanes straddle sl = 5 Re^-1/2 or
+// sl = 0.2 pays two more divisions instead of a second branch. Every value is the same operation on
+// the same operands as in vrel_t; ss^2/(y_a sl + ss) equals ss^2/(ss + y_a sl) (addition commutes).
+GDX_FN double vrel_t_merged(double ris, double vg_sq, double stokes_i, double stokes_j)
+{
+    double sl, ss;
+    if (stokes_i >= stokes_j) { sl = stokes_i; ss = stokes_j; }
+    else { sl = stokes_j; ss = stokes_i; }
+    const double y_a = 1.6;
+    const bool small = sl < 5.0*ris;
+    double v;
+    if (small && sl < ris / y_a)
+    {
+        if (sl < 0.2*ris)
+        {
+            v = vg_sq*(sl - ss)*(sl - ss) / ris;
+        }
+        else
+        {
+            v = vg_sq*(sl - ss) / (sl + ss);
+            v *= (sl / (1.0 + ris / sl) - ss / (1.0 + ris / ss));
+        }
+    }
+    else if (small || sl < 1.0)
+    {
+        const bool r3 = small;
+        double y = y_a;
+        if (!(sl < 0.2))
+        {
+            y = 1.6015125;
+            y += -0.63119577*sl;
+            y +=  0.32938936*sl*sl;
+            y += -0.29847604*sl*sl*sl;
+        }
+        double d1 = (r3 ? sl - ss : ss) / (r3 ? sl + ss : sl);
+        double eps = d1;
+        double d2 = (r3 ? sl : 2.0) / (r3 ? 1.0 + y_a : 1.0 + eps);
+        double d3 = (r3 ? ss*ss : eps*eps*eps) / (r3 ? ss + y_a*sl : y + eps);
+        if (r3)
+        {
+            double coeff = d1;
+            coeff *= (d2 - d3);
+            coeff += 2.0*(y_a*sl - ris) + d2;
+            coeff -= sl*sl / (sl + ris);
+            coeff += d3;
+            coeff -= ss*ss / (ss + ris);
+            v = vg_sq*coeff;
+        }
+        else
+        {
+            double inv = 1.0 / (1.0 + y_a);
+            if (!(sl < 0.2)) inv = 1.0 / (1.0 + y);
+            v = vg_sq*sl;
+            v *= (2.0*y - (1.0 + eps) + d2*(inv + d3));
+        }
+    }
+    else
+    {
+        v = vg_sq*(1.0 / (1.0 + sl) + 1.0 / (1.0 + ss));
+    }
+    return v < 0.0 ? __builtin_nan("") : sqrt(v);
+}
+
 // query-local drift + settling + turbulence (+ Brownian) speed (_collision.cuh:275-325, _col_rates.cuh:127-141)
-__device__ __forceinline__ double vrel_pair(const gdx_env &e, double size_i, double size_j)
+GDX_FN double vrel_pair(const gdx_env &e, double size_i, double size_j)
 {
     double si = stokes(e, size_i);
     double sj = stokes(e, size_j);
@@ -202,7 +271,7 @@ __device__ __forceinline__ double vrel_pair(const gdx_env &e, double size_i, dou
 }
 
 // KNN ball measure (_collision.cuh:66-115)
-__device__ __forceinline__ double ball_measure(double y, double z, double radius)
+GDX_FN double ball_measure(double y, double z, double radius)
 {
 #ifdef GDX_UNIT_MEASURE
     (void)y; (void)z;
@@ -251,21 +320,87 @@ __device__ __forceinline__ double ball_measure(double y, double z, double radius
 }
 
 // ---------------------------------------------------------------------------------------------
+// logarithms of the pair outcome moments, after fdlibm's e_log.c / s_log1p.c (Sun Microsystems,
+// 1993): 1 + x = 2^k (1 + f) with 1 + f in [sqrt(1/2), sqrt(2)), s = f / (2 + f), and
+// log(1 + f) = f - (hfsq - s (hfsq + R(s^2))) with fdlibm's degree-7 minimax polynomial R (Lg1-Lg7).
+// Branch-free (every lane runs one instruction stream) and under one ulp; about a third of the
+// instructions of a double-double library log1p.
+// ---------------------------------------------------------------------------------------------
+constexpr double LN2_HI = 6.93147180369123816490e-01;   // 32 significant bits: k LN2_HI is exact
+constexpr double LN2_LO = 1.90821492927058770002e-10;
+constexpr double LG1 = 6.666666666666735130e-01, LG2 = 3.999999999940941908e-01;
+constexpr double LG3 = 2.857142874366239149e-01, LG4 = 2.222219843214978396e-01;
+constexpr double LG5 = 1.818357216161805012e-01, LG6 = 1.531383769920937332e-01;
+constexpr double LG7 = 1.479819860511658591e-01;
+constexpr double SQRT_HALF = 7.07106781186547524401e-01;
+
+// log((1 + f) 2^k) + c for 1 + f in [sqrt(1/2), sqrt(2)) and a small correction c
+GDX_FN double log_kernel(double f, int k, double c)
+{
+    double s = f / (2.0 + f);
+    double z = s*s;
+    double R = z*fma(z, fma(z, fma(z, fma(z, fma(z, fma(z, LG7, LG6), LG5), LG4), LG3), LG2), LG1);
+    double hfsq = 0.5*f*f;
+    double dk = static_cast<double>(k);
+    return dk*LN2_HI - ((hfsq - (s*(hfsq + R) + (dk*LN2_LO + c))) - f);
+}
+
+// 1 + x = 2^k m with m in [sqrt(1/2), sqrt(2)); returns m
+GDX_FN double log_reduce(double u, int &k)
+{
+    double m = frexp(u, &k);
+    bool low = m < SQRT_HALF;
+    m = low ? 2.0*m : m;
+    k = low ? k - 1 : k;
+    return m;
+}
+
+GDX_FN double ulog1p(double x)
+{
+    double u = 1.0 + x;
+    // u + e = 1 + x exactly (Fast2Sum with the larger operand first)
+    double e = (fabs(x) <= 1.0) ? x - (u - 1.0) : 1.0 - (u - x);
+    int k;
+    double m = log_reduce(u, k);
+    // k = 0: f = x exactly; otherwise f = m - 1 (exact) and c ~ e / u, needed to a few bits only
+    double r = 2.0 - m;
+    r = r*(2.0 - m*r);
+    double f = (k == 0) ? x : m - 1.0;
+    double c = (k == 0) ? 0.0 : ldexp(e*r, -k);
+    double y = log_kernel(f, k, c);
+    if (!(x < __builtin_inf())) y = x;                 // NaN, +inf
+    if (x == -1.0) y = -__builtin_inf();
+    if (x < -1.0) y = __builtin_nan("");
+    return y;
+}
+
+GDX_FN double ulog(double x)
+{
+    int k;
+    double m = log_reduce(x, k);
+    double y = log_kernel(m - 1.0, k, 0.0);
+    if (!(x < __builtin_inf())) y = x;                 // NaN, +inf
+    if (x == 0.0) y = -__builtin_inf();
+    if (x < 0.0) y = __builtin_nan("");
+    return y;
+}
+
+// ---------------------------------------------------------------------------------------------
 // outcomes (inc/swarm/_col_event.cuh:15-109)
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ double sticking_packet(double q)
+GDX_FN double sticking_packet(double q)
 { return q > 0.0 && q <= 1.0e-06 ? fmax(1.0, floor(1.0e-04 / q)) : 1.0; }
-__device__ __forceinline__ double mass_ratio(double size_i, double size_j)
+GDX_FN double mass_ratio(double size_i, double size_j)
 { double r = size_j / size_i; return r*r*r; }
 
-__device__ __forceinline__ double outcome_moments(double si, double sj, bool high_speed,
+GDX_FN double outcome_moments(double si, double sj, bool high_speed,
                                                   double &mean, double &second, double &maximum)
 {
     double q = mass_ratio(si, sj);
     double Gp = sticking_packet(q);
     if (!high_speed)
     {
-        mean = log1p(Gp*q) / 3.0;
+        mean = ulog1p(Gp*q) / 3.0;
         second = mean*mean;
         maximum = mean;
         return 1.0 / Gp;
@@ -275,8 +410,8 @@ __device__ __forceinline__ double outcome_moments(double si, double sj, bool hig
         double remnant = (1.0 - q) / Gp;
         double debris = q;
         double factor = remnant + debris;
-        double jr = -log1p(-Gp*q) / 3.0;
-        double jd = -log(q) / 3.0;
+        double jr = -ulog1p(-Gp*q) / 3.0;
+        double jd = -ulog(q) / 3.0;
         mean = (remnant*jr + debris*jd) / factor;
         second = (remnant*jr*jr + debris*jd*jd) / factor;
         maximum = fmax(jr, jd);
@@ -298,7 +433,7 @@ __device__ __forceinline__ double outcome_moments(double si, double sj, bool hig
     return 1.0;
 }
 
-__device__ __forceinline__ double sample_outcome(double si, double sj, bool high_speed, double u)
+GDX_FN double sample_outcome(double si, double sj, bool high_speed, double u)
 {
     double q = mass_ratio(si, sj);
     double Gp = sticking_packet(q);
@@ -317,7 +452,7 @@ __device__ __forceinline__ double sample_outcome(double si, double sj, bool high
 // RN(a/b) from y = RN(1/b): q = a y, r = a - b q (exact with one FMA), q' = RN(q + r y) is the
 // correctly rounded quotient (Markstein's theorem; valid without underflow or overflow). Used where
 // the divisor is owner-uniform or a constant, so the reciprocal is paid once.
-__device__ __forceinline__ double div_rn(double a, double b, double y)
+GDX_FN double div_rn(double a, double b, double y)
 {
     double q = a*y;
     double r = fma(-q, b, a);
@@ -336,7 +471,7 @@ struct owner_terms
     double si, fi, ai, mi, vn, zo, ris, vg_sq, radial, inv_radial, strat, inv_strat, cs, hg;
 };
 
-__device__ __forceinline__ owner_terms make_owner(const gdx_env &e, double size_i, double m, double hg_i)
+GDX_FN owner_terms make_owner(const gdx_env &e, double size_i, double m, double hg_i)
 {
     owner_terms t{};
     t.size_i = size_i;
@@ -368,7 +503,7 @@ __device__ __forceinline__ owner_terms make_owner(const gdx_env &e, double size_
 }
 
 // _get_stokes for a partner size with the owner's scalings: same three divisions, uniform divisors
-__device__ __forceinline__ double stokes_partner(const owner_terms &t, double size)
+GDX_FN double stokes_partner(const owner_terms &t, double size)
 {
     double st = STOKES_0*div_rn(size, S_0, INV_S0);
 #ifndef CONST_ST
@@ -378,17 +513,14 @@ __device__ __forceinline__ double stokes_partner(const owner_terms &t, double si
     return st;
 }
 
-__device__ __forceinline__ double vrel_pair_terms(const owner_terms &t, double size_j)
+GDX_FN double vrel_pair_terms(const owner_terms &t, double size_j)
 {
     double sj = stokes_partner(t, size_j);
     double fj = 1.0 / (1.0 + sj*sj);
     double dvr = 2.0*t.vn*(t.ai - sj*fj);
     double dvphi = t.vn*(t.fi - fj);
     double dvz = t.zo*(t.mi - fmin(sj, 0.5));
-    gdx_env e{};
-    e.re_inv_sqrt = t.ris;
-    e.vg_sq = t.vg_sq;
-    double vt = vrel_t(e, t.si, sj);
+    double vt = vrel_t_merged(t.ris, t.vg_sq, t.si, sj);
     double vb = 0.0;
 #ifndef CODE_UNIT
     double mi = grain_mass(t.size_i);
@@ -398,17 +530,56 @@ __device__ __forceinline__ double vrel_pair_terms(const owner_terms &t, double s
     return sqrt(dvr*dvr + dvphi*dvphi + dvz*dvz + vt*vt + vb*vb);
 }
 
-// _erosion_outcome_moments with owner-uniform and constant divisors (_col_event.cuh:31-74)
-__device__ __forceinline__ double outcome_moments_terms(const owner_terms &t, double sj, bool high_speed,
-                                                        double &mean, double &second, double &maximum)
+// catastrophic-fragmentation moments (_col_event.cuh:62-73): they depend on the owner size alone
+struct cat_moments
+{
+    double mean, second, maximum;
+    GDX_FN void get(double &a, double &b, double &c) const { a = mean; b = second; c = maximum; }
+};
+
+// the same moments read from shared memory only where a lane needs them (volatile: neither
+// forwarded into registers nor hoisted out of the pair loop)
+struct cat_moments_lds
+{
+    const volatile double *p;
+    __device__ __forceinline__ void get(double &a, double &b, double &c) const { a = p[0]; b = p[1]; c = p[2]; }
+};
+
+GDX_FN cat_moments catastrophic_moments(double si)
+{
+    cat_moments c{0.0, 0.0, 0.0};
+    if constexpr (!CAN_FRAGMENT) return c;
+    double L = 0.5*log(div_rn(si, INIT_SMIN, INV_SMIN));
+    if (L < 1.0e-03)
+    {
+        c.mean = L - L*L / 6.0 + L*L*L*L / 360.0;
+        c.second = L*L*(4.0/3.0 - L / 3.0 + L*L / 90.0 + L*L*L / 180.0);
+    }
+    else
+    {
+        double tail = exp(-L) / (-expm1(-L));
+        c.mean = 2.0*(1.0 - L*tail);
+        c.second = 8.0 - (4.0*L*L + 8.0*L)*tail;
+    }
+    c.maximum = 2.0*L;
+    return c;
+}
+
+// _erosion_outcome_moments with owner-uniform and constant divisors (_col_event.cuh:31-74). The
+// sticking (log1p(Gp q)) and erosion remnant (log1p(-Gp q)) forms share one log1p call with the
+// argument selected per lane; the catastrophic branch reads the owner's precomputed moments.
+template <class CM>
+GDX_FN double outcome_moments_terms(const owner_terms &t, const CM &cm, double sj, bool high_speed,
+                                    double &mean, double &second, double &maximum)
 {
     const double si = t.size_i;
     double ratio = div_rn(sj, si, t.inv_size);
     double q = ratio*ratio*ratio;
     double Gp = sticking_packet(q);
+    double lp = ulog1p(high_speed ? -Gp*q : Gp*q);
     if (!high_speed)
     {
-        mean = div_rn(log1p(Gp*q), 3.0, INV_THREE);
+        mean = div_rn(lp, 3.0, INV_THREE);
         second = mean*mean;
         maximum = mean;
         return Gp == 1.0 ? 1.0 : 1.0 / Gp;
@@ -418,26 +589,14 @@ __device__ __forceinline__ double outcome_moments_terms(const owner_terms &t, do
         double remnant = Gp == 1.0 ? (1.0 - q) : (1.0 - q) / Gp;
         double debris = q;
         double factor = remnant + debris;
-        double jr = div_rn(-log1p(-Gp*q), 3.0, INV_THREE);
-        double jd = div_rn(-log(q), 3.0, INV_THREE);
+        double jr = div_rn(-lp, 3.0, INV_THREE);
+        double jd = div_rn(-ulog(q), 3.0, INV_THREE);
         mean = (remnant*jr + debris*jd) / factor;
         second = (remnant*jr*jr + debris*jd*jd) / factor;
         maximum = fmax(jr, jd);
         return factor;
     }
-    double L = 0.5*log(div_rn(si, INIT_SMIN, INV_SMIN));
-    if (L < 1.0e-03)
-    {
-        mean = L - L*L / 6.0 + L*L*L*L / 360.0;
-        second = L*L*(4.0/3.0 - L / 3.0 + L*L / 90.0 + L*L*L / 180.0);
-    }
-    else
-    {
-        double tail = exp(-L) / (-expm1(-L));
-        mean = 2.0*(1.0 - L*tail);
-        second = 8.0 - (4.0*L*L + 8.0*L)*tail;
-    }
-    maximum = 2.0*L;
+    cm.get(mean, second, maximum);
     return 1.0;
 }
 
@@ -454,7 +613,7 @@ struct partner
     bool valid;
 };
 
-__device__ __forceinline__ partner load_partner(int j, const double2 *res, const double *hgs)
+GDX_FN partner load_partner(int j, const double2 *res, const double *hgs)
 {
     partner q{0.0, 0.0, 0.0, false};
     if (j < 0) return q;
@@ -467,7 +626,8 @@ __device__ __forceinline__ partner load_partner(int j, const double2 *res, const
     return q;
 }
 
-__device__ __forceinline__ pair_term eval_pair_terms(const owner_terms &t, const partner &q, double lambda_0)
+template <class CM>
+GDX_FN pair_term eval_pair_terms(const owner_terms &t, const CM &cm, const partner &q, double lambda_0)
 {
     pair_term p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (!q.valid) return p;
@@ -494,7 +654,7 @@ __device__ __forceinline__ pair_term eval_pair_terms(const owner_terms &t, const
     }
     double r = div_rn(rate, t.m, t.inv_m);
     double mean = 0.0, second = 0.0, maximum = 0.0;
-    r *= outcome_moments_terms(t, size_j, vrel >= V_FRAG, mean, second, maximum);
+    r *= outcome_moments_terms(t, cm, size_j, vrel >= V_FRAG, mean, second, maximum);
     p.rate = r;
     p.j1 = r*mean;
     p.j2 = r*second;
@@ -505,7 +665,7 @@ __device__ __forceinline__ pair_term eval_pair_terms(const owner_terms &t, const
 }
 
 // moving logarithmic size bin (inc/swarm/_col_sizes.cuh:12-21)
-__device__ __forceinline__ int size_bin(double size, double lower, double upper)
+GDX_FN int size_bin(double size, double lower, double upper)
 {
     if (!isfinite(size) || !(size > 0)) return 0;
     double fraction = log(size / lower) / log(upper / lower);

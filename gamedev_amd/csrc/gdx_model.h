@@ -34,6 +34,9 @@ constexpr int bitrev_slot(int t)
     for (int b = 0; b < LOG2K; ++b) r |= ((t >> b) & 1) << (LOG2K - 1 - b);
     return r;
 }
+// pairs at or above the fragmentation speed exist only for the physical kernel (other kernels carry
+// vrel = 0 into the outcome test)
+constexpr bool CAN_FRAGMENT = (COAG_KERNEL == 3) ? (V_FRAG < 1.0e300) : (V_FRAG <= 0.0);
 // the physical kernel evaluated in a vertically integrated disk needs each partner's gas scale height
 constexpr bool PARTNER_HG = (COAG_KERNEL == 3 && N_Z == 1);
 

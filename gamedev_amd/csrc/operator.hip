@@ -117,7 +117,7 @@ struct fold3
 };
 __device__ __forceinline__ fold3 add3(const fold3 &x, const fold3 &y) { return {x.r + y.r, x.a + y.a, x.b + y.b}; }
 
-__global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr, const double2 *st,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_due due, const int *nbr, const double2 *st,
                                                     const double2 *res, const double *measure, const gdx_env *env,
                                                     const double *hgs, double lambda_0,
                                                     const unsigned long long *scr_min,
@@ -129,15 +129,6 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
     int gi, s0, s1;
     if (!due_block(due, RATE_OWNERS, gi, s0, s1)) return;
     const int g = due.gid[gi];
-    // moving bounds from this wave's extrema (col_size_bnds.cu:12-21)
-    double lo = lower[g], up = upper[g];
-    unsigned long long mx = scr_max[g];
-    if (mx != 0ull)
-    {
-        lo = 0.5*bitsd(scr_min[g]);
-        up = 8.0*bitsd(mx);
-        if (threadIdx.x == 0 && xcd_block() == due.bstart[gi]) { lower[g] = lo; upper[g] = up; }
-    }
     if (threadIdx.x < COL_BIN_S*4) (&accd[0][0])[threadIdx.x] = 0.0;
     if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
     __syncthreads();
@@ -151,6 +142,15 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
     if constexpr (COAG_KERNEL == 3) { if (own) e = env[s]; }
     if constexpr (PARTNER_HG) { if (own) hg_i = hgs[s]; }
     const owner_terms t = make_owner(e, me.x, m, hg_i);
+    // read only by lanes with a catastrophic pair: kept in LDS rather than in registers
+    __shared__ double cats[BLOCK][3];
+    {
+        cat_moments c = catastrophic_moments(me.x);
+        cats[threadIdx.x][0] = c.mean;
+        cats[threadIdx.x][1] = c.second;
+        cats[threadIdx.x][2] = c.maximum;
+    }
+    const cat_moments_lds cm{&cats[threadIdx.x][0]};
     const bool live = own && m > 0.0;
     const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : s0)*N_K);
 
@@ -163,15 +163,26 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
     for (int c = 0; c < N_K / 4; ++c)
     {
         int4 jn = (live && c + 1 < N_K / 4) ? row[c + 1] : make_int4(-1, -1, -1, -1);
-        pair_term p0 = eval_pair_terms(t, load_partner(jj.x, res, hgs), lambda_0);
-        pair_term p1 = eval_pair_terms(t, load_partner(jj.y, res, hgs), lambda_0);
-        pair_term p2 = eval_pair_terms(t, load_partner(jj.z, res, hgs), lambda_0);
-        pair_term p3 = eval_pair_terms(t, load_partner(jj.w, res, hgs), lambda_0);
-        V = V && isfinite(p0.rate) && p0.rate >= 0.0 && isfinite(p1.rate) && p1.rate >= 0.0
-              && isfinite(p2.rate) && p2.rate >= 0.0 && isfinite(p3.rate) && p3.rate >= 0.0;
-        JM = fmax(fmax(JM, fmax(p0.jmax, p1.jmax)), fmax(p2.jmax, p3.jmax));
-        fold3 part = add3(add3({p0.rate, p0.j1, p0.j2}, {p1.rate, p1.j1, p1.j2}),
-                          add3({p2.rate, p2.j1, p2.j2}, {p3.rate, p3.j1, p3.j2}));
+        // each term is folded as soon as it is formed: (t0 + t1) + (t2 + t3), the level-2 partial
+        const partner q0 = load_partner(jj.x, res, hgs), q1 = load_partner(jj.y, res, hgs);
+        const partner q2 = load_partner(jj.z, res, hgs), q3 = load_partner(jj.w, res, hgs);
+        pair_term p = eval_pair_terms(t, cm, q0, lambda_0);
+        fold3 a{p.rate, p.j1, p.j2};
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        p = eval_pair_terms(t, cm, q1, lambda_0);
+        a = add3(a, {p.rate, p.j1, p.j2});
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        p = eval_pair_terms(t, cm, q2, lambda_0);
+        fold3 b{p.rate, p.j1, p.j2};
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        p = eval_pair_terms(t, cm, q3, lambda_0);
+        b = add3(b, {p.rate, p.j1, p.j2});
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        fold3 part = add3(a, b);
         bool placed = false;
 #pragma unroll
         for (int lv = 2; lv < LOG2K; ++lv)
@@ -187,6 +198,17 @@ __global__ __launch_bounds__(BLOCK) void k_bath_rate(gdx_due due, const int *nbr
     }
     const double R = total.r, J1 = total.a, J2 = total.b;
     if (own) cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
+
+    // moving bounds from this wave's extrema (col_size_bnds.cu:12-21); read after the pair loop so
+    // they hold no registers during it
+    double lo = lower[g], up = upper[g];
+    unsigned long long mx = scr_max[g];
+    if (mx != 0ull)
+    {
+        lo = 0.5*bitsd(scr_min[g]);
+        up = 8.0*bitsd(mx);
+        if (threadIdx.x == 0 && xcd_block() == due.bstart[gi]) { lower[g] = lo; upper[g] = up; }
+    }
 
     // controller-bin sums, reduced over each distinct bin of the wavefront
     int sb = own ? size_bin(me.x, lo, up) : -1;
@@ -463,6 +485,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
     uint32_t pidx = 0;
     gdx_env e{};
     owner_terms t{};
+    cat_moments cm{0.0, 0.0, 0.0};
     while (true)
     {
         // segments without an owner take the next queued owners, one counter atomic per wavefront
@@ -492,6 +515,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                     if constexpr (COAG_KERNEL == 3) e = env[s];
                     if constexpr (PARTNER_HG) hg_i = hgs[s];
                     t = make_owner(e, cur.x, measure[s], hg_i);
+                    cm = catastrophic_moments(cur.x);
                     time = hazard = j1 = j2 = jmax = 0.0;
                     events = 0;
                     active = true;
@@ -501,7 +525,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
         if (!__ballot(active)) break;
         if (!active) continue;
         partner q = load_partner(nbr[(long long)s*N_K + bitrev_slot(kl)], res, hgs);
-        pair_term p = eval_pair_terms(t, q, lambda_0);
+        pair_term p = eval_pair_terms(t, cm, q, lambda_0);
         int last = (!isfinite(p.rate) || p.rate < 0.0) ? -2 : (p.rate > 0.0 ? kl : -1);
         double R = seg_sum<SEG>(p.rate);
         double J1 = seg_sum<SEG>(p.j1);
@@ -568,6 +592,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                         double mass_after = numr_i*size_new*size_new*size_new;
                         double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
                         t = make_owner(e, size_new, t.m, t.hg);
+                        cm = catastrophic_moments(size_new);
                         if (!isfinite(size_new) || !isfinite(numr_i) || !(size_new > 0.0) || !(numr_i > 0.0)
                             || fabs(mass_after - mass_before) > 2.0e-12*mass_scale)
                         { code = 6; fin = true; }
