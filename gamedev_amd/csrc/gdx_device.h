@@ -123,15 +123,18 @@ GDX_FN gdx_env make_env(double y, double z)
 }
 
 // _get_stokes for analytic gas (param_phys.cuh:170-206): divide by the radial and vertical scalings
-GDX_FN double stokes(const gdx_env &e, double size)
+GDX_FN double stokes_scaled(double size, double radial, double strat)
 {
     double st = STOKES_0*(size / S_0);
 #ifndef CONST_ST
-    st /= e.radial;
-    st /= e.strat;
+    st /= radial;
+    st /= strat;
+#else
+    (void)radial; (void)strat;
 #endif
     return st;
 }
+GDX_FN double stokes(const gdx_env &e, double size) { return stokes_scaled(size, e.radial, e.strat); }
 
 // Ormel & Cuzzi (2007) turbulent relative speed, regimes of _get_vrel_t (_collision.cuh:152-273)
 GDX_FN double vrel_t(const gdx_env &e, double stokes_i, double stokes_j)
@@ -471,19 +474,27 @@ struct owner_terms
     double si, fi, ai, mi, vn, zo, ris, vg_sq, radial, inv_radial, strat, inv_strat, cs, hg;
 };
 
-GDX_FN owner_terms make_owner(const gdx_env &e, double size_i, double m, double hg_i)
+// the owner-size factors, recomputed after each event from the position factors already held
+GDX_FN void owner_resize(owner_terms &t, double size_i)
 {
-    owner_terms t{};
     t.size_i = size_i;
-    t.m = m;
-    t.inv_m = 1.0 / m;
     t.inv_size = 1.0 / size_i;
     if constexpr (COAG_KERNEL == 3)
     {
-        t.si = stokes(e, size_i);
+        t.si = stokes_scaled(size_i, t.radial, t.strat);
         t.fi = 1.0 / (1.0 + t.si*t.si);
         t.ai = t.si*t.fi;
         t.mi = fmin(t.si, 0.5);
+    }
+}
+
+GDX_FN owner_terms make_owner(const gdx_env &e, double size_i, double m, double hg_i)
+{
+    owner_terms t{};
+    t.m = m;
+    t.inv_m = 1.0 / m;
+    if constexpr (COAG_KERNEL == 3)
+    {
         t.vn = e.vn;
         t.zo = e.Z*e.omega;
         t.ris = e.re_inv_sqrt;
@@ -499,6 +510,7 @@ GDX_FN owner_terms make_owner(const gdx_env &e, double size_i, double m, double 
     {
         (void)e; (void)hg_i;
     }
+    owner_resize(t, size_i);
     return t;
 }
 

@@ -45,17 +45,42 @@ __device__ __forceinline__ bool due_block(const gdx_due &due, int per_block, int
     return s0 < s1;
 }
 
+// wavefront reductions of a double through DPP lane moves (no LDS): quad exchanges, row shifts by 4
+// and 8 leave each 16-lane row's total in its lane 15, row broadcasts 15 and 31 carry them to lane
+// 63. Disabled or out-of-row sources contribute 0, so the inputs must be >= 0 for wave_max. Every
+// lane of the wavefront must be active; the fixed tree makes the result deterministic.
+template <int CTRL, int ROWS, bool BOUND>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, BOUND);
+    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, BOUND);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane63(double v)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, WAVE);
-    return v;
+    v += dpp_move<0xb1, 0xf, true>(v);      // quad_perm [1,0,3,2]
+    v += dpp_move<0x4e, 0xf, true>(v);      // quad_perm [2,3,0,1]
+    v += dpp_move<0x114, 0xf, true>(v);     // row_shr:4
+    v += dpp_move<0x118, 0xf, true>(v);     // row_shr:8
+    v += dpp_move<0x142, 0xa, false>(v);    // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc, false>(v);    // row_bcast:31 into rows 2 and 3
+    return lane63(v);
 }
 __device__ __forceinline__ double wave_max(double v)
 {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, WAVE));
-    return v;
+    v = fmax(v, dpp_move<0xb1, 0xf, true>(v));
+    v = fmax(v, dpp_move<0x4e, 0xf, true>(v));
+    v = fmax(v, dpp_move<0x114, 0xf, true>(v));
+    v = fmax(v, dpp_move<0x118, 0xf, true>(v));
+    v = fmax(v, dpp_move<0x142, 0xa, false>(v));
+    v = fmax(v, dpp_move<0x143, 0xc, false>(v));
+    return lane63(v);
 }
 
 __global__ void k_reset_scratch(unsigned long long *scr_min, unsigned long long *scr_max)
@@ -483,9 +508,9 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
     double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
     uint64_t ctr = 0;
     uint32_t pidx = 0;
-    gdx_env e{};
     owner_terms t{};
     cat_moments cm{0.0, 0.0, 0.0};
+    partner q{0.0, 0.0, 0.0, false};
     while (true)
     {
         // segments without an owner take the next queued owners, one counter atomic per wavefront
@@ -512,10 +537,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                     ctr = rng[s];
                     pidx = uint32_t(slot_pidx[s]);
                     double hg_i = 0.0;
+                    gdx_env e{};
                     if constexpr (COAG_KERNEL == 3) e = env[s];
                     if constexpr (PARTNER_HG) hg_i = hgs[s];
                     t = make_owner(e, cur.x, measure[s], hg_i);
                     cm = catastrophic_moments(cur.x);
+                    // the partners are the published states: fixed for the whole chain
+                    q = load_partner(nbr[(long long)s*N_K + bitrev_slot(kl)], res, hgs);
                     time = hazard = j1 = j2 = jmax = 0.0;
                     events = 0;
                     active = true;
@@ -524,7 +552,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
         }
         if (!__ballot(active)) break;
         if (!active) continue;
-        partner q = load_partner(nbr[(long long)s*N_K + bitrev_slot(kl)], res, hgs);
         pair_term p = eval_pair_terms(t, cm, q, lambda_0);
         int last = (!isfinite(p.rate) || p.rate < 0.0) ? -2 : (p.rate > 0.0 ? kl : -1);
         double R = seg_sum<SEG>(p.rate);
@@ -591,7 +618,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                         ++events;
                         double mass_after = numr_i*size_new*size_new*size_new;
                         double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
-                        t = make_owner(e, size_new, t.m, t.hg);
+                        owner_resize(t, size_new);
                         cm = catastrophic_moments(size_new);
                         if (!isfinite(size_new) || !isfinite(numr_i) || !(size_new > 0.0) || !(numr_i > 0.0)
                             || fabs(mass_after - mass_before) > 2.0e-12*mass_scale)
