@@ -144,9 +144,11 @@ def main() -> int:
     t0 = time.perf_counter()
     tot = {"owner_updates": 0, "events": 0, "waves": 0, "baths": 0, "pair_evals": 0, "chain_owners": 0}
     updates = 0
+    longest_chain = 0
     for i in range(args.steps):
         s = op.evolve(args.horizon)
         updates += s["active_owners"]
+        longest_chain = max(longest_chain, s["max_chain_events"])
         for k in tot:
             tot[k] += s[k]
         if (i + 1) % 20 == 0:
@@ -206,6 +208,7 @@ def main() -> int:
             "mean_baths_per_owner_step": tot["owner_updates"]/max(1, updates),
             "waves_per_step": tot["waves"]/args.steps,
             "chain_owners_per_step": tot["chain_owners"]/args.steps,
+            "longest_chain_events": longest_chain,
             "kernel_ms_per_step": {k: v["ms"]/args.steps for k, v in kt.items()},
             "active_owners": n_active,
         },

@@ -16,6 +16,7 @@ STATUS = {
 }
 DEFER_WRITEBACK = 1
 PROFILE = 2
+SEARCH_MORTON, SEARCH_KDTREE = 0, 1
 KERNELS = ("publish", "bath_rate", "screen", "chain", "chain_audit")
 
 
@@ -34,13 +35,15 @@ class ModelInfo(C.Structure):
 
 
 class Config(C.Structure):
-    _fields_ = [("n", C.c_int), ("total_dust_mass", C.c_double), ("seed", C.c_ulonglong), ("flags", C.c_int)]
+    _fields_ = [("n", C.c_int), ("total_dust_mass", C.c_double), ("seed", C.c_ulonglong), ("flags", C.c_int),
+                ("search", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("waves", C.c_int), ("baths", C.c_int), ("owner_updates", C.c_longlong),
                 ("active_owners", C.c_longlong), ("events", C.c_longlong), ("chain_owners", C.c_longlong),
-                ("pair_evals", C.c_longlong), ("dt_col", C.c_double), ("clock_dyn", C.c_double)]
+                ("pair_evals", C.c_longlong), ("dt_col", C.c_double), ("clock_dyn", C.c_double),
+                ("max_chain_events", C.c_longlong)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -69,6 +72,7 @@ _SIGS = {
     "gdx_col_get_neighbors": [P, P, P],
     "gdx_col_get_sites": [P, P, P, P],
     "gdx_col_get_morton": [P, P, P],
+    "gdx_col_get_kdtree": [P, P, P, P],
     "gdx_col_get_cached": [P, P],
     "gdx_col_get_rng": [P, P],
     "gdx_col_set_rng": [P, P],

@@ -36,13 +36,16 @@ class CollisionOperator:
     """One swarm's collision operator on the current HIP device (one library per model directory)."""
 
     def __init__(self, model: str, n: int, total_dust_mass: float, seed: int = 0, defer_writeback: bool = False,
-                 profile: bool = False):
+                 profile: bool = False, search: str = "morton"):
         self.model = model
         self.lib = _lib.load(model)
         self.info = model_info(model)
         self.n = int(n)
         flags = (_lib.DEFER_WRITEBACK if defer_writeback else 0) | (_lib.PROFILE if profile else 0)
-        cfg = _lib.Config(self.n, float(total_dust_mass), int(seed) & (2**64 - 1), flags)
+        modes = {"morton": _lib.SEARCH_MORTON, "kdtree": _lib.SEARCH_KDTREE}
+        if search not in modes:
+            raise ValueError(f"search must be one of {sorted(modes)}")
+        cfg = _lib.Config(self.n, float(total_dust_mass), int(seed) & (2**64 - 1), flags, modes[search])
         h = C.c_void_p()
         _lib.check(self.lib, None, self.lib.gdx_col_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -143,6 +146,14 @@ class CollisionOperator:
         idx = np.empty(self.n, dtype=np.int32)
         self._call(self.lib.gdx_col_get_morton, _np_ptr(keys), _np_ptr(idx))
         return keys, idx
+
+    def kdtree(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """k-d tree of the last geometry epoch: particle index and split dimension per node, world box."""
+        idx = np.empty(self.n, dtype=np.int32)
+        dim = np.empty(self.n, dtype=np.int32)
+        world = np.empty(6, dtype=np.float32)
+        self._call(self.lib.gdx_col_get_kdtree, _np_ptr(idx), _np_ptr(dim), _np_ptr(world))
+        return idx, dim, world
 
     def cached(self) -> np.ndarray:
         out = np.empty((self.n, 4), dtype=np.float64)

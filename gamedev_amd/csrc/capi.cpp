@@ -149,7 +149,7 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
     kernel_timer timer(c);
     gdx_col_stats stats{};
     stats.active_owners = c.n_active;
-    hip_check(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long)*3, st), "zero counters");
+    hip_check(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long)*4, st), "zero counters");
     launch_reset_scratch(c);
 
     std::vector<gdx_raw_rate> raw_rate(RAW_BINS);
@@ -277,13 +277,14 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
         for (int g : groups) dt_col = std::min(dt_col, steps[g]);
         sched.advance(groups);
     }
-    unsigned long long counters[3] = {0, 0, 0};
+    unsigned long long counters[4] = {0, 0, 0, 0};
     copy_d2h(counters, c.counters.p, sizeof(counters), st);
     if (!(c.flags & GDX_COL_DEFER_WRITEBACK)) store_state(c);
     hip_check(hipStreamSynchronize(st), "operator");
     stats.events = (long long)counters[0];
     stats.chain_owners = (long long)counters[1];
     stats.pair_evals += (long long)counters[2];
+    stats.max_chain_events = (long long)counters[3];
     stats.dt_col = dt_col;
     stats.clock_dyn = duration;
     if (out) *out = stats;
@@ -328,6 +329,11 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         g_last_error = "total dust mass must be positive and finite";
         return GDX_COL_ERR_ARG;
     }
+    if (cfg->search != GDX_COL_SEARCH_MORTON && cfg->search != GDX_COL_SEARCH_KDTREE)
+    {
+        g_last_error = "search must be GDX_COL_SEARCH_MORTON or GDX_COL_SEARCH_KDTREE";
+        return GDX_COL_ERR_ARG;
+    }
     gdx_col_ctx *h = nullptr;
     int rc = guard(nullptr, [&] {
         h = new gdx_col_ctx();
@@ -336,6 +342,7 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         c.total_mass = cfg->total_dust_mass;
         c.seed = cfg->seed;
         c.flags = cfg->flags;
+        c.search = cfg->search;
         c.state.assign(LOCAL_GROUPS, context::bath_state{});
         c.rng_p.alloc(size_t(c.n));
         hip_check(hipMemset(c.rng_p.p, 0, sizeof(uint64_t)*c.n), "zero counters");
@@ -348,7 +355,7 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         c.raw_rate.alloc(RAW_BINS);
         c.raw_audit.alloc(RAW_BINS);
         c.err.alloc(1);
-        c.counters.alloc(3);
+        c.counters.alloc(4);
     });
     if (rc != GDX_COL_OK)
     {
@@ -544,6 +551,32 @@ int gdx_col_get_morton(gdx_col_handle h, unsigned long long *keys, int *idx)
         if (keys) copy_d2h(keys, c.index.keys.p, sizeof(uint64_t)*c.n, c.stream);
         if (idx) copy_d2h(idx, c.index.idx.p, sizeof(int)*c.n, c.stream);
         hip_check(hipStreamSynchronize(c.stream), "read morton");
+    });
+}
+
+int gdx_col_get_kdtree(gdx_col_handle h, int *node_idx, int *node_dim, float *world6)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(c.geometry_valid && c.search == GDX_COL_SEARCH_KDTREE, GDX_COL_ERR_STATE,
+                "no k-d tree: geometry not built with GDX_COL_SEARCH_KDTREE");
+        const int n = c.kd.n;
+        if (node_idx) copy_d2h(node_idx, c.kd.idx.p, sizeof(int)*n, c.stream);
+        if (world6) copy_d2h(world6, c.kd.world.p, sizeof(float)*6, c.stream);
+        if (node_dim)
+        {
+            std::vector<float4> nodes(n);
+            copy_d2h(nodes.data(), c.kd.node.p, sizeof(float4)*n, c.stream);
+            hip_check(hipStreamSynchronize(c.stream), "read k-d nodes");
+            for (int i = 0; i < n; ++i)
+            {
+                int d;
+                std::memcpy(&d, &nodes[i].w, sizeof d);
+                node_dim[i] = d;
+            }
+        }
+        hip_check(hipStreamSynchronize(c.stream), "read k-d tree");
     });
 }
 

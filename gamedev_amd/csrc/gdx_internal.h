@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/gdx_col.h"
@@ -70,6 +71,22 @@ struct bound_state
     bool bound = false;
 };
 
+template <typename T>
+void swap_buf(dbuf<T> &a, dbuf<T> &b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+
+// left-balanced k-d tree over all collision sites, node order: {x, y, z, split dim bits}
+struct kd_tree
+{
+    int n = 0;
+    dbuf<float4> node;
+    dbuf<int> idx;             // particle index per node
+    dbuf<float> world;         // data bounds {lo xyz, hi xyz}
+};
+
 // reusable scratch for radix sorts and scans
 struct sort_scratch
 {
@@ -98,6 +115,7 @@ struct context
     double total_mass = 0.0;
     uint64_t seed = 0;
     int flags = 0;
+    int search = GDX_COL_SEARCH_MORTON;
     hipStream_t stream = nullptr;
     bound_state user;
 
@@ -107,6 +125,7 @@ struct context
     dbuf<unsigned char> group;
     dbuf<int> bad;             // first non-finite particle + 1
     morton_index index;
+    kd_tree kd;
     sort_scratch scratch;
     dbuf<int> pidx_slot;       // particle -> slot (-1 inactive)
     dbuf<int> slot_pidx;       // slot -> particle
@@ -172,6 +191,8 @@ void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int
 
 // ---- operator.hip ---------------------------------------------------------------------------
 void launch_reset_scratch(context &c);
+void kd_build(context &c);
+void kd_query(context &c);
 void launch_publish(context &c, const gdx_due &due);
 void launch_bath_rate(context &c, const gdx_due &due, double lambda_0);
 void launch_screen(context &c, const gdx_due &due);

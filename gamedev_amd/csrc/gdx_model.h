@@ -30,10 +30,9 @@ constexpr int LOG2K = (N_K == 16) ? 4 : (N_K == 32) ? 5 : 6;
 // streaming positions 0..K-1 pairs slots exactly as the upper-half fold does (j with j + K/2, ...)
 constexpr int bitrev_slot(int t)
 {
-    int r = 0;
-    for (int b = 0; b < LOG2K; ++b) r |= ((t >> b) & 1) << (LOG2K - 1 - b);
-    return r;
+    return static_cast<int>(__builtin_bitreverse32(static_cast<unsigned>(t)) >> (32 - LOG2K));
 }
+static_assert(bitrev_slot(1) == N_K / 2 && bitrev_slot(N_K - 2) == N_K / 2 - 1, "bit-reversed slot order");
 // pairs at or above the fragmentation speed exist only for the physical kernel (other kernels carry
 // vrel = 0 into the outcome test)
 constexpr bool CAN_FRAGMENT = (COAG_KERNEL == 3) ? (V_FRAG < 1.0e300) : (V_FRAG <= 0.0);

@@ -860,17 +860,25 @@ void geometry_epoch(context &c)
     c.measure.alloc(na);
     hip_check(hipMemsetAsync(c.graph.p, 0, sizeof(unsigned long long)*LOCAL_GROUPS, st), "zero graph");
     hip_check(hipMemsetAsync(c.overflow.p, 0, sizeof(unsigned int), st), "zero overflow");
-    int waves = blocks_for(n, WAVE);
-    k_col_topk<<<xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE)), TOPK_BLOCK, 0, st>>>(
-        c.index.rec.p, c.index.nodes.p, n, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p, c.group.p,
-        c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p);
-    hip_check(hipGetLastError(), "launch top-K");
+    if (c.search == GDX_COL_SEARCH_KDTREE)
+    {
+        kd_build(c);
+        kd_query(c);
+    }
+    else
+    {
+        int waves = blocks_for(n, WAVE);
+        k_col_topk<<<xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE)), TOPK_BLOCK, 0, st>>>(
+            c.index.rec.p, c.index.nodes.p, n, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
+            c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p);
+        hip_check(hipGetLastError(), "launch top-K");
+    }
     unsigned int ovf = 0;
     std::vector<unsigned long long> graph(LOCAL_GROUPS);
     hip_check(hipMemcpyAsync(&ovf, c.overflow.p, sizeof(ovf), hipMemcpyDeviceToHost, st), "read overflow");
     hip_check(hipMemcpyAsync(graph.data(), c.graph.p, sizeof(unsigned long long)*LOCAL_GROUPS, hipMemcpyDeviceToHost, st), "read graph");
     hip_check(hipStreamSynchronize(st), "top-K");
-    if (ovf) throw status_error(11, "Morton traversal stack overflow in col_cache_get");
+    if (ovf) throw status_error(11, "traversal stack overflow in col_cache_get");
     c.edges.assign(size_t(LOCAL_GROUPS)*LOCAL_WORDS, 0u);
     for (int g = 0; g < LOCAL_GROUPS; ++g)
         for (int d = 0; d < LOCAL_GROUPS; ++d)

@@ -29,6 +29,7 @@ constexpr int WAVES = BLOCK / WAVE;
 constexpr int PUBLISH_OWNERS = 1024;
 constexpr int RATE_OWNERS = 256;
 constexpr int CHAIN_OWNERS = 256;
+constexpr int CHAIN_CHUNK = 4*NSEG;     // queue positions a chain wavefront takes per counter atomic
 // events one owner may take in one bath: the reference allows COL_EVENT_CAP events per launch and
 // 1e6 continuation launches (inc/swarm/_col_chain.cuh:381-384); exceeding this bound is status 7
 constexpr int EVENT_LIMIT = 1 << 22;
@@ -81,6 +82,66 @@ __device__ __forceinline__ double wave_max(double v)
     v = fmax(v, dpp_move<0x142, 0xa, false>(v));
     v = fmax(v, dpp_move<0x143, 0xc, false>(v));
     return lane63(v);
+}
+
+// one K-lane segment per owner whose lane t holds row position t (slot bitrev(t)): the reduction
+// levels pair positions differing in bit 0, 1, 2, 3 (quad exchanges, row half-mirror, row mirror),
+// then whole rows; in slots that is the reference's upper-half fold (slot j with j + K/2 first).
+// Segments are whole 16-lane rows, and every lane of a segment that reduces must be active.
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+template <bool MAX>
+__device__ __forceinline__ double seg_fold(double v)
+{
+    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+    v = op(v, dpp_move<0xb1, 0xf, true>(v));       // quad_perm [1,0,3,2]
+    v = op(v, dpp_move<0x4e, 0xf, true>(v));       // quad_perm [2,3,0,1]
+    v = op(v, dpp_move<0x141, 0xf, true>(v));      // row_half_mirror
+    v = op(v, dpp_move<0x140, 0xf, true>(v));      // row_mirror
+    if constexpr (SEG == 16)
+    {
+        return v;
+    }
+    else if constexpr (SEG == 32)
+    {
+        v = op(v, dpp_move<0x142, 0xa, false>(v)); // rows 1 and 3 take rows 0 and 2
+        double a = lane_value(v, 31), b = lane_value(v, 63);
+        return (threadIdx.x % WAVE) < 32 ? a : b;
+    }
+    else
+    {
+        v = op(v, dpp_move<0x142, 0xa, false>(v));
+        v = op(v, dpp_move<0x143, 0xc, false>(v)); // rows 2 and 3 take lane 31
+        return lane_value(v, 63);
+    }
+}
+// segment maximum of a non-negative integer (0 fills disabled sources)
+__device__ __forceinline__ int seg_max_u(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, true));
+    if constexpr (SEG == 16)
+    {
+        return v;
+    }
+    else if constexpr (SEG == 32)
+    {
+        v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+        int a = __builtin_amdgcn_readlane(v, 31), b = __builtin_amdgcn_readlane(v, 63);
+        return (threadIdx.x % WAVE) < 32 ? a : b;
+    }
+    else
+    {
+        v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+        v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+        return __builtin_amdgcn_readlane(v, 63);
+    }
 }
 
 __global__ void k_reset_scratch(unsigned long long *scr_min, unsigned long long *scr_max)
@@ -484,6 +545,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                                                 unsigned long long *counters, int *work)
 {
     __shared__ int prefix[WAVES][65];
+    __shared__ double walk[WAVES][NSEG][N_K];
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     // prefix of the due groups' queue lengths, one lane per due group
     int inc = lane < due.n ? qcount[due.gid[lane]] : 0;
@@ -498,11 +560,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
     const int total = __shfl(inc, WAVE - 1, WAVE);
     __builtin_amdgcn_wave_barrier();
 
-    const int seg = lane / SEG, kl = lane % SEG, segbase = seg*SEG;
+    // lane kl of a segment holds row position kl, i.e. slot bitrev(kl)
+    const int seg = lane / SEG, kl = lane % SEG, segbase = seg*SEG, slot_k = bitrev_slot(kl);
+    const unsigned long long seg_mask = (SEG == 64) ? ~0ull : ((1ull << SEG) - 1ull) << segbase;
     const unsigned long long seg_lead = (N_K == 64) ? 1ull : (N_K == 32) ? 0x0000000100000001ull
                                                                          : 0x0001000100010001ull;
     unsigned long long n_events = 0, n_chain = 0, n_pairs = 0;
-    int err_code = 0;
+    int err_code = 0, longest = 0;
     bool more = true, active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
     double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
@@ -511,54 +575,66 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
     owner_terms t{};
     cat_moments cm{0.0, 0.0, 0.0};
     partner q{0.0, 0.0, 0.0, false};
+    // the wavefront's current run of queue positions [cb, ce), taken CHAIN_CHUNK at a time from the
+    // global counter so that the counter sees one atomic per CHAIN_CHUNK owners
+    int cb = 0, ce = 0;
     while (true)
     {
-        // segments without an owner take the next queued owners, one counter atomic per wavefront
         unsigned long long need = __ballot(!active) & seg_lead;
-        if (more && need)
+        if (need && cb >= ce && more)
         {
-            int n = __popcll(need);
             int base = 0;
-            if (lane == 0) base = atomicAdd(work, n);
+            if (lane == 0) base = atomicAdd(work, CHAIN_CHUNK);
             base = __builtin_amdgcn_readfirstlane(base);
-            if (base + n >= total) more = false;
-            if (!active)
-            {
-                int v = base + __popcll(need & ((1ull << segbase) - 1ull));
-                if (v < total)
-                {
-                    gi = 0;
-                    while (prefix[wave][gi + 1] <= v) ++gi;
-                    pos = due.beg[gi] + (v - prefix[wave][gi]);
-                    s = queue[pos];
-                    end = due.dt[gi];
-                    double2 cur = st[s];
-                    numr_i = cur.y;
-                    ctr = rng[s];
-                    pidx = uint32_t(slot_pidx[s]);
-                    double hg_i = 0.0;
-                    gdx_env e{};
-                    if constexpr (COAG_KERNEL == 3) e = env[s];
-                    if constexpr (PARTNER_HG) hg_i = hgs[s];
-                    t = make_owner(e, cur.x, measure[s], hg_i);
-                    cm = catastrophic_moments(cur.x);
-                    // the partners are the published states: fixed for the whole chain
-                    q = load_partner(nbr[(long long)s*N_K + bitrev_slot(kl)], res, hgs);
-                    time = hazard = j1 = j2 = jmax = 0.0;
-                    events = 0;
-                    active = true;
-                }
-            }
+            if (base >= total) { more = false; base = total; }
+            cb = base;
+            ce = min(base + CHAIN_CHUNK, total);
         }
-        if (!__ballot(active)) break;
+        if (need && cb < ce)
+        {
+            int n = min(__popcll(need), ce - cb);
+            int rank = __popcll(need & ((1ull << segbase) - 1ull));
+            if (!active && rank < n)
+            {
+                int v = cb + rank;
+                gi = 0;
+                while (prefix[wave][gi + 1] <= v) ++gi;
+                pos = due.beg[gi] + (v - prefix[wave][gi]);
+                s = queue[pos];
+                end = due.dt[gi];
+                double2 cur = st[s];
+                numr_i = cur.y;
+                ctr = rng[s];
+                pidx = uint32_t(slot_pidx[s]);
+                double hg_i = 0.0;
+                gdx_env e{};
+                if constexpr (COAG_KERNEL == 3) e = env[s];
+                if constexpr (PARTNER_HG) hg_i = hgs[s];
+                t = make_owner(e, cur.x, measure[s], hg_i);
+                cm = catastrophic_moments(cur.x);
+                // the partners are the published states: fixed for the whole chain
+                q = load_partner(nbr[(long long)s*N_K + kl], res, hgs);
+                time = hazard = j1 = j2 = jmax = 0.0;
+                events = 0;
+                active = true;
+            }
+            cb += n;
+        }
+        if (!__ballot(active))
+        {
+            if (cb >= ce && !more) break;
+            continue;
+        }
         if (!active) continue;
         pair_term p = eval_pair_terms(t, cm, q, lambda_0);
-        int last = (!isfinite(p.rate) || p.rate < 0.0) ? -2 : (p.rate > 0.0 ? kl : -1);
-        double R = seg_sum<SEG>(p.rate);
-        double J1 = seg_sum<SEG>(p.j1);
-        double J2 = seg_sum<SEG>(p.j2);
-        double JM = seg_max<SEG>(p.jmax);
-        int LAST = seg_last<SEG>(last);
+        const bool bad_rate = !isfinite(p.rate) || p.rate < 0.0;
+        double R = seg_fold<false>(p.rate);
+        double J1 = seg_fold<false>(p.j1);
+        double J2 = seg_fold<false>(p.j2);
+        double JM = seg_fold<true>(p.jmax);
+        // last slot with a positive rate (-1: none), -2 when any rate of the segment is invalid
+        int LAST = seg_max_u(p.rate > 0.0 ? slot_k + 1 : 0) - 1;
+        if (__ballot(bad_rate) & seg_mask) LAST = -2;
         n_pairs += (kl == 0) ? N_K : 0;
         bool fin = false;
         int code = 0;
@@ -591,24 +667,35 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                     jmax = fmax(jmax, JM);
                     // partner by the cumulative walk in slot order, last positive slot on roundoff
                     double target = col_uniform(pidx, seed, ctr)*R;
+                    walk[wave][seg][slot_k] = p.rate;
+                    __builtin_amdgcn_wave_barrier();
+                    const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
                     double cumulative = 0.0;
                     int slot = -1;
-                    for (int k = 0; k < N_K; ++k)
+#pragma unroll 4
+                    for (int k = 0; k < N_K / 2; ++k)
                     {
-                        double rk = __shfl(p.rate, segbase + k, WAVE);
-                        if (rk > 0.0)
+                        double2 rk = wr[k];
+                        if (rk.x > 0.0)
                         {
-                            cumulative += rk;
-                            if (slot < 0 && cumulative > target) slot = k;
+                            cumulative += rk.x;
+                            if (slot < 0 && cumulative > target) slot = 2*k;
+                        }
+                        if (rk.y > 0.0)
+                        {
+                            cumulative += rk.y;
+                            if (slot < 0 && cumulative > target) slot = 2*k + 1;
                         }
                     }
+                    __builtin_amdgcn_wave_barrier();
                     if (slot < 0) slot = LAST;
                     if (slot < 0) { code = 5; fin = true; }
                     else
                     {
                         double size_i = t.size_i;
-                        double size_j = __shfl(p.size_j, segbase + slot, WAVE);
-                        double vrel = __shfl(p.vrel, segbase + slot, WAVE);
+                        // slot k sits in lane bitrev(k) of the segment
+                        double size_j = __shfl(p.size_j, segbase + bitrev_slot(slot), WAVE);
+                        double vrel = __shfl(p.vrel, segbase + bitrev_slot(slot), WAVE);
                         double mass_before = numr_i*size_i*size_i*size_i;
                         bool high_speed = vrel >= V_FRAG;
                         double sample = high_speed ? col_uniform(pidx, seed, ctr) : 0.0;
@@ -637,6 +724,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                 err_code = max(err_code, code);
                 qsum[pos] = make_double4(hazard, j1, j2, jmax);
                 qevents[pos] = code ? -1 : events;
+                longest = max(longest, events);
                 n_events += events;
                 n_chain += 1;
             }
@@ -650,9 +738,11 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
         n_events += __shfl_xor(n_events, o, WAVE);
         n_chain += __shfl_xor(n_chain, o, WAVE);
         n_pairs += __shfl_xor(n_pairs, o, WAVE);
+        longest = max(longest, __shfl_xor(longest, o, WAVE));
     }
     if (lane == 0)
     {
+        if (longest) atomicMax(&counters[3], (unsigned long long)longest);
         if (n_events) atomicAdd(&counters[0], n_events);
         if (n_chain) atomicAdd(&counters[1], n_chain);
         if (n_pairs) atomicAdd(&counters[2], n_pairs);

@@ -41,6 +41,10 @@ extern "C" {
 #define GDX_COL_DEFER_WRITEBACK 1      /* keep size/number resident between operator calls; gdx_col_store_state writes back */
 #define GDX_COL_PROFILE         2      /* time every operator kernel with HIP events on the launch stream */
 
+/* neighbour search of the geometry epoch (COLLISION_SEARCH=morton|kdtree) */
+#define GDX_COL_SEARCH_MORTON   0      /* adaptive Morton hierarchy, rows in (d^2, code) order */
+#define GDX_COL_SEARCH_KDTREE   1      /* left-balanced k-d tree, rows in heap order (_col_cache.cuh:15-71) */
+
 /* kernels timed under GDX_COL_PROFILE */
 #define GDX_COL_KERNEL_PUBLISH   0
 #define GDX_COL_KERNEL_RATE      1
@@ -65,6 +69,7 @@ typedef struct {
     double total_dust_mass;         /* lambda_0 = N / N_K / total_dust_mass (_col_chain.cuh:187) */
     unsigned long long seed;        /* random-number key; replaces rngstate_init (src/swarm/rngstate_init.cu:11-18) */
     int flags;                      /* GDX_COL_DEFER_WRITEBACK */
+    int search;                     /* GDX_COL_SEARCH_*: the build's COLLISION_SEARCH (Makefile:129-151) */
 } gdx_col_config;
 
 typedef struct {
@@ -77,6 +82,7 @@ typedef struct {
     long long pair_evals;           /* pair-rate evaluations */
     double dt_col;                  /* smallest bath duration of the last wave (_col_chain.cuh:481-482) */
     double clock_dyn;               /* operator clock at return: the horizon */
+    long long max_chain_events;     /* most events of one owner in one bath (the longest serial chain) */
 } gdx_col_stats;
 
 /* library and model */
@@ -120,6 +126,10 @@ int gdx_col_active_count(gdx_col_handle h, int *n_active);
 int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure);
 /* collision sites: float {x, y, z, search cap} x n, active flags, spatial group */
 int gdx_col_get_sites(gdx_col_handle h, float *pts4, unsigned char *active, unsigned char *group);
+/* k-d tree of the last geometry epoch (search GDX_COL_SEARCH_KDTREE): particle index and split
+ * dimension per node (n each) and the world box {lo xyz, hi xyz}; test access to
+ * kdtree::buildTree (inc/swarm/kdtree/builder_thrust.h:418-526) */
+int gdx_col_get_kdtree(gdx_col_handle h, int *node_idx, int *node_dim, float *world6);
 /* Morton keys and record order after the stable sort (n each) */
 int gdx_col_get_morton(gdx_col_handle h, unsigned long long *sorted_keys, int *sorted_idx);
 /* bath-start rate moments {rate (-1 invalid), first, second, maximum} per particle (0 for inactive) */

@@ -891,6 +891,81 @@ inline int thread_count(int threads)
 #endif
 }
 
+
+// ============================================================================================
+// left-balanced k-d tree (inc/swarm/kdtree/builder_thrust.h:418-526, helpers.h:95-300,
+// builder_common.h:205-240, box.h:111-125) and its closest-corner-tracking top-K query
+// (traverse-cct.h:26-100, index_heap.cuh:11-177, _col_cache.cuh:15-71)
+// ============================================================================================
+struct kd_box { float lo[3], hi[3]; };
+
+// widest extent; a later dimension wins ties (box.h:113-125)
+inline int kd_widest(const kd_box &b)
+{
+    int best = 0;
+    float w_best = 0.0f;
+    for (int d = 0; d < 3; d++)
+    {
+        float w = b.hi[d] - b.lo[d];
+        if (w < w_best) continue;
+        w_best = w;
+        best = d;
+    }
+    return best;
+}
+
+inline int kd_level_of(int node) { return 63 - __builtin_clzll(static_cast<unsigned long long>(node) + 1ull); }
+inline int kd_levels_for(int n) { return kd_level_of(n - 1) + 1; }
+inline int kd_full_nodes(int levels) { return (1 << levels) - 1; }
+
+// nodes in the subtree rooted at `root` of a left-balanced tree of n nodes (helpers.h:185-200)
+inline int kd_subtree_size(int n, int root)
+{
+    int levels_tree = kd_levels_for(n);
+    int levels_sub = levels_tree - kd_level_of(root);
+    int first = (root + 1) << (levels_sub - 1);
+    int last_on_last = first + (1 << (levels_sub - 1)) - 1;
+    int on_last = 1 << (levels_sub - 1);
+    int missing = std::min(std::max(last_on_last - n, 0), on_last);
+    return kd_full_nodes(levels_sub) - missing;
+}
+
+// position of a level-L subtree's pivot in the step-L array layout (helpers.h:214-280)
+inline int kd_pivot_pos(int L, int n, int subtree)
+{
+    int settled = kd_full_nodes(L);
+    int levels_total = kd_levels_for(n);
+    int remaining = levels_total - L;
+    int earlier = subtree - settled;
+    int left_if_full = earlier*kd_full_nodes(remaining);
+    int left_on_last_if_full = earlier*(1 << (remaining - 1));
+    int total_on_last = n - kd_full_nodes(levels_total - 1);
+    int missing = left_on_last_if_full - std::min(total_on_last, left_on_last_if_full);
+    int seg_begin = settled + left_if_full - missing;
+    int left_root = 2*subtree + 1;
+    int left_size = left_root >= n ? 0 : kd_subtree_size(n, left_root);
+    return seg_begin + left_size;
+}
+
+struct kd_rec { float p[3]; int idx; int dim; };
+
+// the subtree's domain: world bounds clipped by every ancestor's split plane (builder_common.h:205-240)
+inline kd_box kd_domain(int subtree, const kd_box &world, const std::vector<kd_rec> &nodes)
+{
+    kd_box b = world;
+    int cur = subtree;
+    while (cur > 0)
+    {
+        int parent = (cur + 1) / 2 - 1;
+        int d = nodes[parent].dim;
+        float split = nodes[parent].p[d];
+        if (cur & 1) b.hi[d] = std::fmin(split, b.hi[d]);
+        else b.lo[d] = std::fmax(split, b.lo[d]);
+        cur = parent;
+    }
+    return b;
+}
+
 }  // namespace
 
 // ================================================================================================
@@ -1128,6 +1203,163 @@ void orc_topk_tree(int n, const float *pts, const uint8_t *active, const int *so
         {
             out_codes[size_t(q)*k + s] = c[s] == INT_MAX ? -1 : c[s];
             out_d2[size_t(q)*k + s] = d[s];
+        }
+    }
+}
+
+// k-d tree build over all n sites (inactive ones included, as the reference's N_T nodes):
+// out_idx[node] = particle index, out_dim[node] = split dimension, world6 = {lo xyz, hi xyz}.
+// Each level stable-sorts (tag, coordinate along the point's dimension); the reference's sort is
+// not stable, so its layout for equal coordinates is unspecified and this one is the stable choice.
+void orc_kd_build(int n, const float *pts, int *out_idx, int *out_dim, float *world6)
+{
+    if (n < 1) return;
+    std::vector<kd_rec> nodes(n);
+    kd_box world;
+    for (int d = 0; d < 3; d++) { world.lo[d] = pts[d]; world.hi[d] = pts[d]; }
+    for (int i = 0; i < n; i++)
+    {
+        for (int d = 0; d < 3; d++)
+        {
+            nodes[i].p[d] = pts[3*size_t(i) + d];
+            world.lo[d] = std::fmin(world.lo[d], nodes[i].p[d]);
+            world.hi[d] = std::fmax(world.hi[d], nodes[i].p[d]);
+        }
+        nodes[i].idx = i;
+    }
+    int dim0 = kd_widest(world);
+    for (auto &r : nodes) r.dim = dim0;
+    std::vector<uint32_t> tag(n, 0u);
+    std::vector<int> order(n);
+    auto sort_level = [&]() {
+        for (int i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (tag[a] != tag[b]) return tag[a] < tag[b];
+            int d = nodes[a].dim;
+            return nodes[a].p[d] < nodes[b].p[d];
+        });
+        std::vector<kd_rec> nn(n);
+        std::vector<uint32_t> nt(n);
+        for (int i = 0; i < n; i++) { nn[i] = nodes[order[i]]; nt[i] = tag[order[i]]; }
+        nodes.swap(nn);
+        tag.swap(nt);
+    };
+    const int deepest = kd_levels_for(n) - 1;
+    for (int L = 0; L < deepest; L++)
+    {
+        sort_level();
+        const int settled = kd_full_nodes(L);
+        std::vector<int> new_dim(n);
+        for (int g = 0; g < n; g++) new_dim[g] = nodes[g].dim;
+        for (int g = settled; g < n; g++)
+        {
+            int subtree = int(tag[g]);
+            kd_box b = kd_domain(subtree, world, nodes);
+            int pivot = kd_pivot_pos(L, n, subtree);
+            int pd = nodes[pivot].dim;
+            float pc = nodes[pivot].p[pd];
+            if (g < pivot) { subtree = 2*subtree + 1; b.hi[pd] = pc; }
+            else if (g > pivot) { subtree = 2*subtree + 2; b.lo[pd] = pc; }
+            if (g != pivot) new_dim[g] = kd_widest(b);
+            tag[g] = uint32_t(subtree);
+        }
+        for (int g = 0; g < n; g++) nodes[g].dim = new_dim[g];
+    }
+    sort_level();
+    for (int i = 0; i < n; i++) { out_idx[i] = nodes[i].idx; out_dim[i] = nodes[i].dim; }
+    for (int d = 0; d < 3; d++) { world6[d] = world.lo[d]; world6[3 + d] = world.hi[d]; }
+}
+
+// top-K of the queries over the tree: a max-heap of (d^2 bits << 32 | code) keys initialised to
+// (q^2, 0xffffffff), culled at nextafter(root d^2), children visited near side first with the
+// far side's closest corner tracked on a stack. Output rows are in heap order; codes 3*j, -1 empty.
+void orc_kd_query(int n, const float *pts, const uint8_t *active, const int *node_idx, const int *node_dim,
+                  const float *world6, int nq, const int *query_idx, int k, const float *search_dist,
+                  int *out_codes, float *out_d2, int threads)
+{
+    kd_box world;
+    for (int d = 0; d < 3; d++) { world.lo[d] = world6[d]; world.hi[d] = world6[3 + d]; }
+    #pragma omp parallel for schedule(dynamic, 64) num_threads(thread_count(threads))
+    for (int qi = 0; qi < nq; qi++)
+    {
+        const int i = query_idx[qi];
+        const float *qp = pts + 3*size_t(i);
+        const float qd = search_dist[i];
+        auto enc = [](float d2, uint32_t code) {
+            uint32_t b;
+            std::memcpy(&b, &d2, 4);
+            return (uint64_t(b) << 32) | code;
+        };
+        auto dec = [](uint64_t key) {
+            uint32_t b = uint32_t(key >> 32);
+            float f;
+            std::memcpy(&f, &b, 4);
+            return f;
+        };
+        std::vector<uint64_t> heap(k, enc(qd*qd, 0xffffffffu));
+        auto cull = [&]() { return std::nextafter(dec(heap[0]), std::numeric_limits<float>::infinity()); };
+        auto process = [&](int node, float d2) {
+            int j = node_idx[node];
+            if (!active[j]) return;
+            uint64_t cand = enc(d2, uint32_t(3*j));
+            if (cand >= heap[0]) return;
+            int slot = 0;
+            while (true)
+            {
+                int child = 2*slot + 1;
+                if (child >= k) { heap[slot] = cand; break; }
+                uint64_t cv = heap[child];
+                if (child + 1 < k && heap[child + 1] > cv) { cv = heap[child + 1]; ++child; }
+                if (cv < cand) { heap[slot] = cand; break; }
+                heap[slot] = cv;
+                slot = child;
+            }
+        };
+        float cull_d2 = cull();
+        float corner[3];
+        for (int d = 0; d < 3; d++) corner[d] = std::fmin(std::fmax(qp[d], world.lo[d]), world.hi[d]);
+        struct entry { int node; float c[3]; };
+        std::vector<entry> stack;
+        bool run = point_d2(corner, qp) <= cull_d2;
+        int node = 0;
+        while (run)
+        {
+            if (node >= n)
+            {
+                bool found = false;
+                while (!stack.empty())
+                {
+                    entry e = stack.back();
+                    stack.pop_back();
+                    for (int d = 0; d < 3; d++) corner[d] = e.c[d];
+                    if (point_d2(corner, qp) >= cull_d2) continue;
+                    node = e.node;
+                    found = true;
+                    break;
+                }
+                if (!found) break;
+            }
+            const float *np = pts + 3*size_t(node_idx[node]);
+            process(node, point_d2(np, qp));
+            cull_d2 = cull();
+            int dim = node_dim[node];
+            bool left_close = qp[dim] < np[dim];
+            int lc = 2*node + 1, rc = lc + 1;
+            int far = left_close ? rc : lc;
+            float fc[3] = {corner[0], corner[1], corner[2]};
+            fc[dim] = np[dim];
+            if (far < n && point_d2(fc, qp) < cull_d2)
+            {
+                entry e{far, {fc[0], fc[1], fc[2]}};
+                stack.push_back(e);
+            }
+            node = left_close ? lc : rc;
+        }
+        for (int s2 = 0; s2 < k; s2++)
+        {
+            uint32_t code = uint32_t(heap[s2]);
+            out_codes[size_t(qi)*k + s2] = code == 0xffffffffu ? -1 : int(code);
+            out_d2[size_t(qi)*k + s2] = dec(heap[s2]);
         }
     }
 }

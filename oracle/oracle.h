@@ -15,6 +15,10 @@
 //   orc_hierarchy_build     independent recursive linear-partition split (val/swarm/mod/test_knn/topology.cuh:70-110)
 //   orc_topk_brute          knn edge cases with hand-derived answers (knn_edge_tests.cu:95-210), numpy brute force
 //   orc_topk_tree           equality with orc_topk_brute
+//   orc_kd_build            left-balanced layout invariants (subtree sizes, every left-subtree
+//                           coordinate <= the split and every right one >= it) and split
+//                           dimensions equal to the widest extent of each subtree's domain
+//   orc_kd_query            neighbour sets equal to orc_topk_brute, rows a valid max-heap
 //   orc_ball_measure        closed-form interior balls, Monte-Carlo cap volumes
 //   orc_vrel_t / pair rate  Ormel & Cuzzi (2007) equations 17/18/26-29 evaluated independently in tests
 //   orc_outcome_moments     numerical expectation of orc_sample_outcome over u, closed-form sticking moments
@@ -70,6 +74,10 @@ void orc_topk_brute(int n, const float *pts, const uint8_t *active, int nq, cons
                     const float *query_pts, const float *query_q, int k, int *out_codes, float *out_d2,
                     int threads);
 // same contract through the Morton hierarchy (the CPU-baseline search)
+void orc_kd_build(int n, const float *pts, int *out_idx, int *out_dim, float *world6);
+void orc_kd_query(int n, const float *pts, const uint8_t *active, const int *node_idx, const int *node_dim,
+                  const float *world6, int nq, const int *query_idx, int k, const float *search_dist,
+                  int *out_codes, float *out_d2, int threads);
 void orc_topk_tree(int n, const float *pts, const uint8_t *active, const int *sorted_idx,
                    int node_count, const float *lower, const float *nwidth, const int *begin,
                    const int *count, const int *child, const int *child_count, int dim, int max_level,

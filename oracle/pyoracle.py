@@ -60,6 +60,8 @@ class Oracle:
         L.orc_topk_brute.argtypes = [i, P, P, i, P, P, P, i, P, P, i]
         L.orc_topk_tree.argtypes = [i, P, P, P, i, P, P, P, P, P, P, i, i, i, P, i, P, P, P, i]
         L.orc_space_bin.argtypes = [i, P, P, P, P]
+        L.orc_kd_build.argtypes = [i, P, P, P, P]
+        L.orc_kd_query.argtypes = [i, P, P, P, P, P, i, P, i, P, P, P, i]
         L.orc_change_limit.restype = i
         L.orc_change_limit.argtypes = [d, d, d, d, C.POINTER(d), C.POINTER(i)]
         L.orc_build_binmap.restype = i
@@ -177,6 +179,30 @@ class Oracle:
         self.lib.orc_topk_tree(len(pts), _p(pts), _p(act), _p(si), len(nd["width"]), _p(nd["lower"]), _p(nd["width"]),
                                _p(nd["begin"]), _p(nd["count"]), _p(nd["child"]), _p(nd["child_count"]), dim,
                                max_level, nq, _p(qi), k, _p(sd), _p(codes), _p(d2), threads)
+        return codes, d2
+
+    def kd_build(self, pts):
+        """Left-balanced k-d tree over all points: (particle index per node, split dim per node, world box)."""
+        pts = np.ascontiguousarray(pts, dtype=np.float32)
+        n = len(pts)
+        idx = np.empty(n, np.int32)
+        dim = np.empty(n, np.int32)
+        world = np.empty(6, np.float32)
+        self.lib.orc_kd_build(n, _p(pts), _p(idx), _p(dim), _p(world))
+        return idx, dim, world
+
+    def kd_query(self, pts, active, tree, query_idx, k, search_dist, threads=0):
+        """Heap-ordered top-K rows (codes 3*j, -1 empty) and d^2 of the queries over a kd_build tree."""
+        pts = np.ascontiguousarray(pts, dtype=np.float32)
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+        idx, dim, world = (np.ascontiguousarray(a) for a in tree)
+        qi = np.ascontiguousarray(query_idx, dtype=np.int32)
+        sd = np.ascontiguousarray(search_dist, dtype=np.float32)
+        nq = len(qi)
+        codes = np.empty((nq, k), np.int32)
+        d2 = np.empty((nq, k), np.float32)
+        self.lib.orc_kd_query(len(pts), _p(pts), _p(act), _p(idx), _p(dim), _p(world), nq, _p(qi), k, _p(sd),
+                              _p(codes), _p(d2), threads)
         return codes, d2
 
     def space_bin(self, x, y, z):
