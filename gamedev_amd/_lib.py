@@ -16,6 +16,7 @@ STATUS = {
 }
 DEFER_WRITEBACK = 1
 PROFILE = 2
+DIAGNOSTICS = 4
 SEARCH_MORTON, SEARCH_KDTREE = 0, 1
 KERNELS = ("publish", "bath_rate", "screen", "chain", "chain_audit")
 
@@ -66,6 +67,10 @@ _SIGS = {
     "gdx_col_operator": [P, D, C.POINTER(Stats)],
     "gdx_col_reset_controller": [P],
     "gdx_col_synchronize": [P],
+    "gdx_col_open_log": [P, C.c_char_p],
+    "gdx_col_set_clock": [P, C.c_double],
+    "gdx_col_save_controller": [P, C.c_char_p],
+    "gdx_col_event_totals": [P, P, P],
     "gdx_col_kernel_times": [P, P, P, P],
     "gdx_col_reset_kernel_times": [P],
     "gdx_col_active_count": [P, C.POINTER(I)],

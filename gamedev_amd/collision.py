@@ -36,12 +36,13 @@ class CollisionOperator:
     """One swarm's collision operator on the current HIP device (one library per model directory)."""
 
     def __init__(self, model: str, n: int, total_dust_mass: float, seed: int = 0, defer_writeback: bool = False,
-                 profile: bool = False, search: str = "morton"):
+                 profile: bool = False, search: str = "morton", diagnostics: bool = False):
         self.model = model
         self.lib = _lib.load(model)
         self.info = model_info(model)
         self.n = int(n)
-        flags = (_lib.DEFER_WRITEBACK if defer_writeback else 0) | (_lib.PROFILE if profile else 0)
+        flags = ((_lib.DEFER_WRITEBACK if defer_writeback else 0) | (_lib.PROFILE if profile else 0)
+                 | (_lib.DIAGNOSTICS if diagnostics else 0))
         modes = {"morton": _lib.SEARCH_MORTON, "kdtree": _lib.SEARCH_KDTREE}
         if search not in modes:
             raise ValueError(f"search must be one of {sorted(modes)}")
@@ -104,6 +105,25 @@ class CollisionOperator:
 
     def synchronize(self) -> None:
         self._call(self.lib.gdx_col_synchronize)
+
+    # ---- diagnostics (diagnostics=True) ----
+    def open_log(self, directory: str) -> None:
+        """Write one schema-1 JSON line per operator call to <directory>collision_local_<stamp>.jsonl."""
+        self._call(self.lib.gdx_col_open_log, directory.encode())
+
+    def set_clock(self, clock_sim: float) -> None:
+        self._call(self.lib.gdx_col_set_clock, float(clock_sim))
+
+    def save_controller(self, path: str) -> None:
+        """Controller archive (schema 2) of every bath since the last reset_controller()."""
+        self._call(self.lib.gdx_col_save_controller, str(path).encode())
+
+    def event_totals(self) -> tuple[np.ndarray, np.ndarray]:
+        """Event counts and summed log-mass change per outcome category of the last operator call."""
+        counts = np.zeros(7, dtype=np.uint64)
+        log_mass = np.zeros(7, dtype=np.float64)
+        self._call(self.lib.gdx_col_event_totals, _np_ptr(counts), _np_ptr(log_mass))
+        return counts, log_mass
 
     def rate_pass(self) -> None:
         self._call(self.lib.gdx_col_rate_pass)

@@ -9,17 +9,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "controller.h"
+#include "diagnostics.h"
 #include "gdx_internal.h"
 
 struct gdx_col_ctx
 {
     gdx::context c;
+    gdx::diag_state diag;
 };
 
 struct gdx_morton_ctx
@@ -139,17 +142,22 @@ struct kernel_timer
 
 }  // namespace
 
-// one operator call (inc/swarm/_col_chain.cuh:108-539)
-void run_operator(context &c, double duration, gdx_col_stats *out)
+// one operator call (inc/swarm/_col_chain.cuh:108-539); with diagnostics it also keeps the
+// reference's controller archive and per-operator record (_col_chain.cuh:148-538)
+void run_operator(context &c, diag_state *dg, double duration, gdx_col_stats *out)
 {
     require(c.state_loaded, GDX_COL_ERR_STATE, "collision state has not been loaded");
     if (!(duration > 0) || !std::isfinite(duration)) throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
+    using wall = std::chrono::steady_clock;
+    const auto begin = wall::now();
+    const bool diag = c.diag && dg;
     hipStream_t st = c.stream;
     const double lambda0 = c.n / static_cast<double>(N_K) / c.total_mass;
     kernel_timer timer(c);
     gdx_col_stats stats{};
     stats.active_owners = c.n_active;
-    hip_check(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long)*4, st), "zero counters");
+    hip_check(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long)*8, st), "zero counters");
+    if (diag) hip_check(hipMemsetAsync(c.evw.p, 0, sizeof(gdx_event_work)*std::max(1, c.n_active), st), "zero event work");
     launch_reset_scratch(c);
 
     std::vector<gdx_raw_rate> raw_rate(RAW_BINS);
@@ -184,16 +192,18 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
         return m;
     };
     auto bin_end = [&](int g) { return g + 1 < LOCAL_GROUPS ? binmap[(g + 1)*COL_BIN_S] : merged; };
-    auto requested_step = [&](int g) {
+    auto requested_step = [&](int g, int *binding = nullptr) {
         int first = binmap[g*COL_BIN_S];
-        return choose_bath(ratebin.data() + first, bin_end(g) - first, duration, c.state[g].limit_scale);
+        return choose_bath(ratebin.data() + first, bin_end(g) - first, duration, c.state[g].limit_scale, binding);
     };
 
     std::vector<int> all(LOCAL_GROUPS);
     for (int g = 0; g < LOCAL_GROUPS; ++g) all[g] = g;
     merged = rate_wave(make_due(c, all, {}));
     std::vector<double> requested(LOCAL_GROUPS);
-    for (int g = 0; g < LOCAL_GROUPS; ++g) requested[g] = requested_step(g);
+    std::vector<group_stats> gs(diag ? LOCAL_GROUPS : 0);
+    for (int g = 0; g < LOCAL_GROUPS; ++g)
+        requested[g] = requested_step(g, diag ? &gs[g].binding : nullptr);
     schedule sched(duration, requested, c.edges);
     std::vector<double> steps(LOCAL_GROUPS);
     for (int g = 0; g < LOCAL_GROUPS; ++g) steps[g] = sched.seconds(sched.step[g]);
@@ -201,12 +211,30 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
     std::vector<double> current = requested;
     double dt_col = duration;
 
+    operator_record rec;
+    std::vector<double> published(LOCAL_GROUPS, 0.0);
+    int operator_index = 0, bath_index = 0;
+    if (diag)
+    {
+        for (int g = 0; g < LOCAL_GROUPS; ++g)
+        {
+            gs[g].initial_level = sched.level[g];
+            gs[g].min_request = gs[g].max_request = gs[g].initial_request = requested[g];
+        }
+        operator_index = ++dg->summary.operator_count;
+    }
+
     while (sched.time() < sched.end)
     {
         const uint64_t tick = sched.time();
         std::vector<int> groups = sched.due(tick);
+        const double time = sched.seconds(tick);
         long long listed = 0;
-        for (int g : groups) listed += c.group_all[g];
+        for (int g : groups)
+        {
+            listed += c.group_all[g];
+            if (diag) published[g] = time;
+        }
         if (listed == 0)
         {
             sched.advance(groups);
@@ -215,22 +243,57 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
         if (tick != 0)
         {
             merged = rate_wave(make_due(c, groups, {}));
-            for (int g : groups) current[g] = requested_step(g);
+            for (int g : groups)
+            {
+                current[g] = requested_step(g);
+                if (diag)
+                {
+                    gs[g].min_request = std::min(gs[g].min_request, current[g]);
+                    gs[g].max_request = std::max(gs[g].max_request, current[g]);
+                }
+            }
+            std::vector<int> previous = sched.level;
             sched.adapt(tick, groups, current, passed);
-            for (int g : groups) steps[g] = sched.seconds(sched.step[g]);
+            for (int g : groups)
+            {
+                steps[g] = sched.seconds(sched.step[g]);
+                if (diag)
+                {
+                    gs[g].coarsened += sched.level[g] < previous[g];
+                    gs[g].refined += sched.level[g] > previous[g];
+                    gs[g].constrained += steps[g] > current[g];
+                }
+            }
         }
+        if (diag)
+        {
+            for (int g : groups)
+            {
+                if (c.group_all[g] == 0) continue;
+                for (int d = 0; d < LOCAL_GROUPS; ++d)
+                    if (c.edges[size_t(g)*LOCAL_WORDS + d / 32] & (1u << (d % 32)))
+                        gs[g].max_age = std::max(gs[g].max_age, time - published[d]);
+                gs[g].max_requested_ratio = std::max(gs[g].max_requested_ratio, steps[g] / requested_step(g));
+            }
+        }
+        const auto chain_begin = wall::now();
         gdx_due due = make_due(c, groups, steps);
         hip_check(hipMemsetAsync(c.raw_audit.p, 0, sizeof(gdx_raw_audit)*RAW_BINS, st), "zero audit bins");
         hip_check(hipMemsetAsync(c.err.p, 0, sizeof(int), st), "zero error");
+        if (diag) hip_check(hipMemsetAsync(c.counters.p + 4, 0, sizeof(unsigned long long)*3, st), "zero wave counters");
         timer.time(GDX_COL_KERNEL_SCREEN, due.total, [&] { launch_screen(c, due); });
         timer.time(GDX_COL_KERNEL_CHAIN, due.total, [&] { launch_chain(c, due, lambda0); });
         timer.time(GDX_COL_KERNEL_AUDIT, due.total, [&] { launch_chain_audit(c, due); });
         int err = 0;
+        unsigned long long wave_ctr[3] = {0, 0, 0};
         copy_d2h(&err, c.err.p, sizeof(int), st);
         copy_d2h(raw_audit.data(), c.raw_audit.p, sizeof(gdx_raw_audit)*RAW_BINS, st);
+        if (diag) copy_d2h(wave_ctr, c.counters.p + 4, sizeof(wave_ctr), st);
         hip_check(hipStreamSynchronize(st), "chain pass");
         timer.resolve();
         if (err) throw status_error(err, "local collision chain error " + std::to_string(err));
+        const auto audit_begin = wall::now();
+        if (diag) rec.chain_seconds += std::chrono::duration<double>(audit_begin - chain_begin).count();
 
         std::fill(audit.begin(), audit.begin() + merged, audit_bin{});
         for (int r = 0; r < RAW_BINS; ++r)
@@ -267,9 +330,39 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
                 mass += audit[b].mass;
             }
             if (!(mass > 0)) continue;
+            const double before = c.state[g].limit_scale;
             bath_result r = finish_bath(audit.data() + first, last - first, c.state[g]);
             passed[g] = !(r.activity_overshoot || r.distribution_overshoot || r.persistent_overshoot);
             stats.baths++;
+            if (diag)
+            {
+                group_stats &s = gs[g];
+                ++s.updates;
+                s.overshoots += r.activity_overshoot || r.distribution_overshoot;
+                s.persistent += r.persistent_overshoot;
+                s.max_growth = std::max(s.max_growth, r.max_g);
+                s.max_activity = std::max(s.max_activity, r.max_f);
+                bath_record b;
+                b.group_index = g;
+                b.operator_index = operator_index;
+                b.bath_index = ++bath_index;
+                b.merged_bins = last - first;
+                b.duration = steps[g];
+                b.limit_before = before;
+                b.limit_after = c.state[g].limit_scale;
+                b.result = r;
+                record_bath(dg->summary, b);
+            }
+        }
+        if (diag)
+        {
+            // launches the reference needs: one per COL_EVENT_CAP accepted events of the longest chain
+            const int continuations = wave_ctr[2] ? int(wave_ctr[0] / COL_EVENT_CAP + 1) : 0;
+            dg->summary.continuation_launches += continuations;
+            dg->summary.wave_count++;
+            rec.chain_launches += continuations;
+            rec.chain_blocks += wave_ctr[1];
+            rec.audit_seconds += std::chrono::duration<double>(wall::now() - audit_begin).count();
         }
         stats.waves++;
         stats.owner_updates += due.total;
@@ -279,6 +372,11 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
     }
     unsigned long long counters[4] = {0, 0, 0, 0};
     copy_d2h(counters, c.counters.p, sizeof(counters), st);
+    if (diag)
+    {
+        launch_event_sum(c);
+        copy_d2h(&dg->totals, c.ev_sum.p, sizeof(gdx_event_work), st);
+    }
     if (!(c.flags & GDX_COL_DEFER_WRITEBACK)) store_state(c);
     hip_check(hipStreamSynchronize(st), "operator");
     stats.events = (long long)counters[0];
@@ -287,6 +385,28 @@ void run_operator(context &c, double duration, gdx_col_stats *out)
     stats.max_chain_events = (long long)counters[3];
     stats.dt_col = dt_col;
     stats.clock_dyn = duration;
+    if (diag && dg->log.is_open())
+    {
+        rec.clock_sim = dg->clock_sim;
+        rec.duration = duration;
+        rec.waves = uint64_t(stats.waves);
+        rec.owner_updates = uint64_t(stats.owner_updates);
+        rec.finest_ticks = sched.end;
+        rec.owners.assign(c.group_all.begin(), c.group_all.end());
+        rec.level = sched.level;
+        rec.dt = steps;
+        rec.final_request = current;
+        rec.groups = gs;
+        for (int k = 0; k < GDX_EVENT_CATEGORIES; ++k)
+        {
+            rec.event_count[k] = dg->totals.count[k];
+            rec.event_log_mass[k] = dg->totals.log_mass[k];
+        }
+        rec.scheduler_wall_seconds = std::chrono::duration<double>(wall::now() - begin).count();
+        write_operator_line(dg->log, ++dg->operator_id, rec);
+        dg->log.flush();
+        if (!dg->log) throw status_error(GDX_COL_ERR_ARG, "cannot write local collision diagnostics");
+    }
     if (out) *out = stats;
 }
 
@@ -343,6 +463,8 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         c.seed = cfg->seed;
         c.flags = cfg->flags;
         c.search = cfg->search;
+        c.diag = (cfg->flags & GDX_COL_DIAGNOSTICS) != 0;
+        if (c.diag) c.ev_sum.alloc(1);
         c.state.assign(LOCAL_GROUPS, context::bath_state{});
         c.rng_p.alloc(size_t(c.n));
         hip_check(hipMemset(c.rng_p.p, 0, sizeof(uint64_t)*c.n), "zero counters");
@@ -355,7 +477,7 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         c.raw_rate.alloc(RAW_BINS);
         c.raw_audit.alloc(RAW_BINS);
         c.err.alloc(1);
-        c.counters.alloc(4);
+        c.counters.alloc(8);
     });
     if (rc != GDX_COL_OK)
     {
@@ -450,14 +572,61 @@ int gdx_col_store_state(gdx_col_handle h)
 int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out)
 {
     if (!h) return GDX_COL_ERR_ARG;
-    return guard(&h->c, [&] { run_operator(h->c, duration, out); });
+    return guard(&h->c, [&] { run_operator(h->c, &h->diag, duration, out); });
 }
 
 int gdx_col_reset_controller(gdx_col_handle h)
 {
     if (!h) return GDX_COL_ERR_ARG;
     h->c.state.assign(LOCAL_GROUPS, context::bath_state{});
+    h->diag.summary = controller_summary{};
     return GDX_COL_OK;
+}
+
+int gdx_col_open_log(gdx_col_handle h, const char *dir)
+{
+    if (!h || !dir) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(h->c.diag, GDX_COL_ERR_STATE, "the handle was created without GDX_COL_DIAGNOSTICS");
+        auto stamp = std::chrono::duration_cast<std::chrono::microseconds>(
+            std::chrono::system_clock::now().time_since_epoch()).count();
+        std::string path = std::string(dir) + "collision_local_" + std::to_string(stamp) + ".jsonl";
+        if (h->diag.log.is_open()) h->diag.log.close();
+        h->diag.log.clear();
+        h->diag.log.open(path);
+        if (!h->diag.log) throw status_error(GDX_COL_ERR_ARG, "cannot open local collision diagnostics " + path);
+        h->diag.log << std::setprecision(17);
+    });
+}
+
+int gdx_col_set_clock(gdx_col_handle h, double clock_sim)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    h->diag.clock_sim = clock_sim;
+    return GDX_COL_OK;
+}
+
+int gdx_col_save_controller(gdx_col_handle h, const char *file_name)
+{
+    if (!h || !file_name) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(h->c.diag, GDX_COL_ERR_STATE, "the handle was created without GDX_COL_DIAGNOSTICS");
+        if (!save_controller(file_name, h->diag.summary))
+            throw status_error(GDX_COL_ERR_ARG, std::string("cannot save controller archive ") + file_name);
+    });
+}
+
+int gdx_col_event_totals(gdx_col_handle h, unsigned long long *counts, double *log_mass)
+{
+    if (!h || !counts || !log_mass) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(h->c.diag, GDX_COL_ERR_STATE, "the handle was created without GDX_COL_DIAGNOSTICS");
+        for (int k = 0; k < GDX_EVENT_CATEGORIES; ++k)
+        {
+            counts[k] = h->diag.totals.count[k];
+            log_mass[k] = h->diag.totals.log_mass[k];
+        }
+    });
 }
 
 int gdx_col_synchronize(gdx_col_handle h)

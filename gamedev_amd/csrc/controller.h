@@ -83,18 +83,26 @@ inline int build_binmap(const std::vector<int> &raw_count, std::vector<int> &map
     return merged;
 }
 
-inline double choose_bath(const rate_bin *bins, int count, double remaining, double limit_scale)
+// binding: 0 horizon or COL_BATH_MAX, 1 mean-change bound, 2 variance bound (the last bin that shortened it)
+inline double choose_bath(const rate_bin *bins, int count, double remaining, double limit_scale,
+                          int *binding = nullptr)
 {
     double duration = std::min(remaining, COL_BATH_MAX);
     const double tolerance = COL_BATH_EPS*limit_scale;
+    int reason = 0;
     for (int b = 0; b < count; ++b)
     {
         const rate_bin &v = bins[b];
         if (v.invalid_count) throw status_error(8, "invalid change-based rate moments");
         if (!(v.mass > 0)) continue;
         auto bound = change_limit(v.weighted_change / v.mass, v.weighted_second / v.mass, tolerance, duration);
-        if (bound.first < duration) duration = bound.first;
+        if (bound.first < duration)
+        {
+            duration = bound.first;
+            reason = bound.second;
+        }
     }
+    if (binding) *binding = reason;
     return duration;
 }
 

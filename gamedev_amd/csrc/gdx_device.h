@@ -436,6 +436,32 @@ GDX_FN double outcome_moments(double si, double sj, bool high_speed,
     return 1.0;
 }
 
+// outcome category and log mass change of one sampled event (_col_event.cuh:79-109): sticking in four
+// mass-ratio bins (0-3), catastrophic fragmentation (4), remnant erosion (5), debris (6)
+GDX_FN int outcome_category(double si, double sj, bool high_speed, double u, double size_new, double &log_mass)
+{
+    double q = mass_ratio(si, sj);
+    double Gp = sticking_packet(q);
+    if (!high_speed)
+    {
+        log_mass = log1p(Gp*q);
+        return q <= 1.0e-06 ? 0 : q <= 1.0e-04 ? 1 : q <= 1.0e-02 ? 2 : 3;
+    }
+    if (q <= 0.1)
+    {
+        double factor = (1.0 - q) / Gp + q;
+        if (u < q / factor)
+        {
+            log_mass = log(q);
+            return 6;
+        }
+        log_mass = log1p(-Gp*q);
+        return 5;
+    }
+    log_mass = 3.0*log(size_new / si);
+    return 4;
+}
+
 GDX_FN double sample_outcome(double si, double sj, bool high_speed, double u)
 {
     double q = mass_ratio(si, sj);

@@ -159,6 +159,9 @@ struct context
     dbuf<double4> qsum;        // chain sums (hazard, j1, j2, jmax) per queue position
     dbuf<int> qevents;         // chain events per queue position, -1 after an error
     int chain_blocks = 0;      // resident blocks of the persistent chain kernel
+    bool diag = false;         // GDX_COL_DIAGNOSTICS
+    dbuf<gdx_event_work> evw;  // per-slot event statistics of the current call
+    dbuf<gdx_event_work> ev_sum;
     dbuf<unsigned long long> counters;   // events, chain owners, pair evaluations
     std::vector<double> h_lower, h_upper;
 
@@ -196,6 +199,7 @@ void kd_query(context &c);
 void launch_publish(context &c, const gdx_due &due);
 void launch_bath_rate(context &c, const gdx_due &due, double lambda_0);
 void launch_screen(context &c, const gdx_due &due);
+void launch_event_sum(context &c);
 void launch_chain(context &c, const gdx_due &due, double lambda_0);
 void launch_chain_audit(context &c, const gdx_due &due);
 void launch_gather_cached(context &c, double *out_particle_order);

@@ -71,6 +71,15 @@ struct gdx_raw_audit
     int owner_count, invalid_count;
 };
 
+// per-slot event statistics of one operator call (_col_types.cuh:32-33): four sticking mass-ratio
+// bins, catastrophic fragmentation, remnant erosion, debris
+constexpr int GDX_EVENT_CATEGORIES = 7;
+struct gdx_event_work
+{
+    unsigned long long count[GDX_EVENT_CATEGORIES];
+    double log_mass[GDX_EVENT_CATEGORIES];
+};
+
 // the due groups of one refresh wave and their contiguous owner slot ranges
 struct gdx_due
 {

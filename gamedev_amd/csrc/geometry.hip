@@ -899,6 +899,7 @@ void load_state(context &c)
     c.queue.alloc(na);
     c.qcount.alloc(LOCAL_GROUPS + 1);
     c.qsum.alloc(na);
+    if (c.diag) c.evw.alloc(na);
     c.qevents.alloc(na);
     if (COAG_KERNEL == 3) c.env.alloc(na);
     if (PARTNER_HG) c.hgs.alloc(na);

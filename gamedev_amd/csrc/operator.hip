@@ -542,7 +542,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                                                 const double *measure, const gdx_env *env, const double *hgs,
                                                 const int *slot_pidx, uint64_t *rng, uint64_t seed, double lambda_0,
                                                 double4 *qsum, int *qevents, int *err,
-                                                unsigned long long *counters, int *work)
+                                                unsigned long long *counters, int *work, gdx_event_work *evw)
 {
     __shared__ int prefix[WAVES][65];
     __shared__ double walk[WAVES][NSEG][N_K];
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
     const unsigned long long seg_mask = (SEG == 64) ? ~0ull : ((1ull << SEG) - 1ull) << segbase;
     const unsigned long long seg_lead = (N_K == 64) ? 1ull : (N_K == 32) ? 0x0000000100000001ull
                                                                          : 0x0001000100010001ull;
-    unsigned long long n_events = 0, n_chain = 0, n_pairs = 0;
+    unsigned long long n_events = 0, n_chain = 0, n_pairs = 0, n_blocks = 0;
     int err_code = 0, longest = 0;
     bool more = true, active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
@@ -700,6 +700,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                         bool high_speed = vrel >= V_FRAG;
                         double sample = high_speed ? col_uniform(pidx, seed, ctr) : 0.0;
                         double size_new = sample_outcome(size_i, size_j, high_speed, sample);
+                        if (evw && kl == 0)
+                        {
+                            double log_mass;
+                            int cat = outcome_category(size_i, size_j, high_speed, sample, size_new, log_mass);
+                            evw[s].count[cat] += 1ull;
+                            evw[s].log_mass[cat] += log_mass;
+                        }
                         numr_i = mass_before / (size_new*size_new*size_new);
                         time = event_time;
                         ++events;
@@ -725,6 +732,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                 qsum[pos] = make_double4(hazard, j1, j2, jmax);
                 qevents[pos] = code ? -1 : events;
                 longest = max(longest, events);
+                // the reference relaunches an owner after every COL_EVENT_CAP accepted events
+                n_blocks += (unsigned long long)(events / COL_EVENT_CAP + 1);
                 n_events += events;
                 n_chain += 1;
             }
@@ -738,11 +747,15 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
         n_events += __shfl_xor(n_events, o, WAVE);
         n_chain += __shfl_xor(n_chain, o, WAVE);
         n_pairs += __shfl_xor(n_pairs, o, WAVE);
+        n_blocks += __shfl_xor(n_blocks, o, WAVE);
         longest = max(longest, __shfl_xor(longest, o, WAVE));
     }
     if (lane == 0)
     {
         if (longest) atomicMax(&counters[3], (unsigned long long)longest);
+        if (longest) atomicMax(&counters[4], (unsigned long long)longest);
+        if (n_blocks) atomicAdd(&counters[5], n_blocks);
+        if (n_chain) atomicAdd(&counters[6], n_chain);
         if (n_events) atomicAdd(&counters[0], n_events);
         if (n_chain) atomicAdd(&counters[1], n_chain);
         if (n_pairs) atomicAdd(&counters[2], n_pairs);
@@ -845,6 +858,35 @@ void launch_bath_rate(context &c, const gdx_due &due_in, double lambda_0)
     hip_check(hipGetLastError(), "launch bath rate");
 }
 
+__global__ __launch_bounds__(256) void k_event_sum(const gdx_event_work *w, int n, gdx_event_work *sum)
+{
+    __shared__ unsigned long long cnt[256];
+    __shared__ double lm[256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    unsigned long long a = 0;
+    double b = 0.0;
+    for (int i = t; i < n; i += 256)
+    {
+        a += w[i].count[k];
+        b += w[i].log_mass[k];
+    }
+    cnt[t] = a;
+    lm[t] = b;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1)
+    {
+        if (t < stride) { cnt[t] += cnt[t + stride]; lm[t] += lm[t + stride]; }
+        __syncthreads();
+    }
+    if (t == 0) { sum->count[k] = cnt[0]; sum->log_mass[k] = lm[0]; }
+}
+
+void launch_event_sum(context &c)
+{
+    k_event_sum<<<GDX_EVENT_CATEGORIES, 256, 0, c.stream>>>(c.evw.p, c.n_active, c.ev_sum.p);
+    hip_check(hipGetLastError(), "launch event sum");
+}
+
 void launch_screen(context &c, const gdx_due &due_in)
 {
     gdx_due due = due_in;
@@ -872,7 +914,8 @@ void launch_chain(context &c, const gdx_due &due, double lambda_0)
     int grid = std::max(1, std::min(c.chain_blocks, (due.total + 4*NSEG - 1) / (4*NSEG)));
     k_chain<<<grid, BLOCK, 0, c.stream>>>(due, c.qcount.p, c.queue.p, c.nbr.p, c.st.p, c.res.p, c.cached.p,
                                          c.measure.p, c.env.p, c.hgs.p, c.slot_pidx.p, c.rng.p, c.seed, lambda_0,
-                                         c.qsum.p, c.qevents.p, c.err.p, c.counters.p, c.qcount.p + LOCAL_GROUPS);
+                                         c.qsum.p, c.qevents.p, c.err.p, c.counters.p, c.qcount.p + LOCAL_GROUPS,
+                                         c.diag ? c.evw.p : nullptr);
     hip_check(hipGetLastError(), "launch chain");
 }
 

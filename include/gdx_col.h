@@ -40,6 +40,7 @@ extern "C" {
 /* ---- flags ---- */
 #define GDX_COL_DEFER_WRITEBACK 1      /* keep size/number resident between operator calls; gdx_col_store_state writes back */
 #define GDX_COL_PROFILE         2      /* time every operator kernel with HIP events on the launch stream */
+#define GDX_COL_DIAGNOSTICS     4      /* the reference's COL_DIAGNOSTICS: controller archive, per-operator log, event categories */
 
 /* neighbour search of the geometry epoch (COLLISION_SEARCH=morton|kdtree) */
 #define GDX_COL_SEARCH_MORTON   0      /* adaptive Morton hierarchy, rows in (d^2, code) order */
@@ -115,6 +116,20 @@ int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);
 /* controller memory reset at output boundaries (src/swarm/swarm_runtime.cu:491-496) */
 int gdx_col_reset_controller(gdx_col_handle h);
 int gdx_col_synchronize(gdx_col_handle h);
+
+/* ---- diagnostics (handles created with GDX_COL_DIAGNOSTICS) ---- */
+/* open <dir>collision_local_<microseconds>.jsonl, one schema-1 line per operator call
+ * (local_workspace, inc/swarm/_col_chain.cuh:70-75, 488-538) */
+int gdx_col_open_log(gdx_col_handle h, const char *dir);
+/* simulation clock written into the next log lines (clock_sim of evolve_local_collisions) */
+int gdx_col_set_clock(gdx_col_handle h, double clock_sim);
+/* controller archive since the last gdx_col_reset_controller, schema 2 (save_col_controller,
+ * inc/swarm/_col_bound.cuh:242-299; collision_chain_<frame>.json, swarm_runtime.cu:668-675) */
+int gdx_col_save_controller(gdx_col_handle h, const char *file_name);
+/* event counts and summed log mass change per outcome category over the last operator call
+ * (col_event_sum, src/swarm/col_event_sum.cu:12-44): sticking q <= 1e-6, <= 1e-4, <= 1e-2, larger;
+ * fragmentation; remnant erosion; debris */
+int gdx_col_event_totals(gdx_col_handle h, unsigned long long counts[7], double log_mass[7]);
 /* accumulated kernel time (ms), launches and owners per GDX_COL_KERNEL_* since the last reset; no
  * reference counterpart (the reference's COL_PERF_VAL hooks time host phases, _col_chain.cuh:215-241) */
 int gdx_col_kernel_times(gdx_col_handle h, double *ms, long long *launches, long long *owners);

@@ -781,7 +781,9 @@ struct owner_bath
 {
     double time = 0, hazard = 0, j1 = 0, j2 = 0, jmax = 0;
     int events = 0, error = 0;
-    bool complete = false;
+    bool complete = false, chained = false;
+    long long cat_count[7] = {0, 0, 0, 0, 0, 0, 0};
+    double cat_log[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 // col_skip_scan (col_skip_scan.cu:13-59) followed by col_chain_run (col_chain_run.cu:21-404) run to
@@ -818,6 +820,7 @@ void run_owner(int i, double end, const double *y, const double *z, double *size
             }
         }
     }
+    b.chained = true;
     while (keep_running)
     {
         owner_pairs(i, size_i, y, z, nbr, measure, size_old, numr_old, lambda_0, false, active, w);
@@ -865,6 +868,8 @@ void run_owner(int i, double end, const double *y, const double *z, double *size
         int category;
         double log_mass;
         double size_new = sample_outcome(size_i, size_j, high_speed, sample, category, log_mass);
+        b.cat_count[category]++;                                                 // _record_event_work
+        b.cat_log[category] += log_mass;
         numr_i = mass_before / (size_new*size_new*size_new);
         size_i = size_new;
         b.time = event_time;
@@ -964,6 +969,139 @@ inline kd_box kd_domain(int subtree, const kd_box &world, const std::vector<kd_r
         cur = parent;
     }
     return b;
+}
+
+
+// ============================================================================================
+// COL_DIAGNOSTICS: controller archive (_col_bound.cuh:219-299) and per-operator log line
+// (_col_chain.cuh:26-31, 262-538)
+// ============================================================================================
+struct diag_bath
+{
+    int group, op, bath, merged;
+    double duration, before, after;
+    bath_result r;
+};
+struct diag_group
+{
+    unsigned long long updates = 0, coarsened = 0, refined = 0, constrained = 0;
+    int overshoots = 0, persistent = 0, initial_level = 0, binding = 0, owners = 0, level = 0;
+    double max_age = 0, max_growth = 0, max_activity = 0, max_ratio = 0;
+    double min_request = 0, max_request = 0, initial_request = 0, final_request = 0, dt = 0;
+};
+struct diag_t
+{
+    // summary since the last reset
+    int operator_count = 0, wave_count = 0, continuation_launches = 0;
+    std::vector<diag_bath> baths;
+    // last operator call
+    unsigned long long waves = 0, owner_updates = 0, chain_blocks = 0, chain_launches = 0, finest_ticks = 0;
+    double duration = 0;
+    std::vector<diag_group> groups;
+    long long cat_count[7] = {0, 0, 0, 0, 0, 0, 0};
+    double cat_log[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+
+std::string diag_controller_json(const diag_t &d)
+{
+    int ao = 0, dov = 0, po = 0;
+    double dmin = INFINITY, dmax = 0, lmin = 1.0, mf = 0, me = 0, mt = 0, mev = 0, mg = 0, mgu = 0, mdb = 0;
+    for (const auto &b : d.baths)
+    {
+        ao += b.r.activity_overshoot ? 1 : 0;
+        dov += b.r.distribution_overshoot ? 1 : 0;
+        po += b.r.persistent_overshoot ? 1 : 0;
+        dmin = std::min(dmin, b.duration);
+        dmax = std::max(dmax, b.duration);
+        lmin = std::min(lmin, b.after);
+        mf = std::max(mf, b.r.max_f);
+        me = std::max(me, b.r.max_e);
+        mt = std::max(mt, b.r.max_touched);
+        mev = std::max(mev, b.r.max_events);
+        mg = std::max(mg, b.r.max_g);
+        mgu = std::max(mgu, b.r.max_g_upper);
+        mdb = std::max(mdb, b.r.d_bath);
+    }
+    char buf[512];
+    std::string o = "{\n  \"schema\": 2,\n";
+    auto num = [&](const char *k, double v, bool last = false) {
+        std::snprintf(buf, sizeof buf, "  \"%s\": %.17g%s\n", k, v, last ? "" : ",");
+        o += buf;
+    };
+    num("operator_count", d.operator_count);
+    num("bath_count", double(d.baths.size()));
+    num("wave_count", d.wave_count);
+    num("continuation_launches", d.continuation_launches);
+    num("activity_overshoots", ao);
+    num("distribution_overshoots", dov);
+    num("persistent_overshoots", po);
+    num("minimum_duration", std::isfinite(dmin) ? dmin : 0.0);
+    num("maximum_duration", dmax);
+    num("minimum_limit_scale", lmin);
+    num("maximum_f", mf);
+    num("maximum_e", me);
+    num("maximum_touched", mt);
+    num("maximum_events", mev);
+    num("maximum_g", mg);
+    num("maximum_g_upper", mgu);
+    num("maximum_d_bath", mdb);
+    o += "  \"baths\": [\n";
+    for (size_t i = 0; i < d.baths.size(); i++)
+    {
+        const diag_bath &b = d.baths[i];
+        std::snprintf(buf, sizeof buf,
+                      "    {\"operator\": %d, \"group\": %d, \"bath\": %d, \"merged_bins\": %d, \"continuation_launches\": 0, "
+                      "\"duration\": %.17g, \"limit_before\": %.17g, \"limit_after\": %.17g, \"max_f\": %.17g, "
+                      "\"max_e\": %.17g, \"max_touched\": %.17g, \"max_events\": %.17g, \"max_g\": %.17g, ",
+                      b.op, b.group, b.bath, b.merged, b.duration, b.before, b.after, b.r.max_f, b.r.max_e,
+                      b.r.max_touched, b.r.max_events, b.r.max_g);
+        o += buf;
+        std::snprintf(buf, sizeof buf,
+                      "\"max_g_upper\": %.17g, \"d_bath\": %.17g, \"activity_overshoot\": %s, "
+                      "\"distribution_overshoot\": %s, \"persistent_overshoot\": %s}%s\n",
+                      b.r.max_g_upper, b.r.d_bath, b.r.activity_overshoot ? "true" : "false",
+                      b.r.distribution_overshoot ? "true" : "false", b.r.persistent_overshoot ? "true" : "false",
+                      i + 1 < d.baths.size() ? "," : "");
+        o += buf;
+    }
+    o += "  ]\n}\n";
+    return o;
+}
+
+// the last operator's record without the wall-clock fields (chain, audit and scheduler seconds)
+std::string diag_operator_json(const diag_t &d)
+{
+    char buf[1024];
+    std::string o;
+    std::snprintf(buf, sizeof buf,
+                  "{\"duration\":%.17g,\"waves\":%llu,\"owner_updates\":%llu,\"chain_blocks\":%llu,"
+                  "\"chain_launches\":%llu,\"finest_ticks\":%llu,\"groups\":[",
+                  d.duration, d.waves, d.owner_updates, d.chain_blocks, d.chain_launches, d.finest_ticks);
+    o += buf;
+    for (size_t c = 0; c < d.groups.size(); c++)
+    {
+        const diag_group &g = d.groups[c];
+        std::snprintf(buf, sizeof buf,
+                      "%s{\"id\":%zu,\"owners\":%d,\"level\":%d,\"dt\":%.17g,\"initial_level\":%d,\"coarsened_updates\":%llu,"
+                      "\"refined_updates\":%llu,\"neighbor_constrained_updates\":%llu,\"minimum_requested_dt\":%.17g,"
+                      "\"maximum_requested_dt\":%.17g,\"final_requested_dt\":%.17g,\"initial_requested_dt\":%.17g,",
+                      c ? "," : "", c, g.owners, g.level, g.dt, g.initial_level, g.coarsened, g.refined, g.constrained,
+                      g.min_request, g.max_request, g.final_request, g.initial_request);
+        o += buf;
+        std::snprintf(buf, sizeof buf,
+                      "\"initial_binding_constraint\":%d,\"updates\":%llu,\"overshoots\":%d,\"persistent_overshoots\":%d,"
+                      "\"max_snapshot_age_at_start\":%.17g,\"max_requested_dt_ratio\":%.17g,\"max_growth\":%.17g,"
+                      "\"max_predicted_activity\":%.17g}",
+                      g.binding, g.updates, g.overshoots, g.persistent, g.max_age, g.max_ratio, g.max_growth,
+                      g.max_activity);
+        o += buf;
+    }
+    o += "],\"event_counts\":[";
+    for (int k = 0; k < 7; k++) { std::snprintf(buf, sizeof buf, "%s%lld", k ? "," : "", d.cat_count[k]); o += buf; }
+    o += "],\"event_log_mass_sums\":[";
+    for (int k = 0; k < 7; k++) { std::snprintf(buf, sizeof buf, "%s%.17g", k ? "," : "", d.cat_log[k]); o += buf; }
+    o += "]}";
+    return o;
 }
 
 }  // namespace
@@ -1448,6 +1586,28 @@ int orc_schedule_run(int n, double horizon, const double *requested, const uint3
     catch (const std::exception &) { return -1; }
 }
 
+void *orc_diag_new(void) { return new diag_t(); }
+void orc_diag_free(void *d) { delete static_cast<diag_t *>(d); }
+// the controller archive's reset at output boundaries (swarm_runtime.cu:491-496)
+void orc_diag_reset(void *d)
+{
+    diag_t &x = *static_cast<diag_t *>(d);
+    x.operator_count = x.wave_count = x.continuation_launches = 0;
+    x.baths.clear();
+}
+// JSON text: 0 = controller archive (schema 2), 1 = last operator record; returns the length
+int orc_diag_json(void *d, int which, char *buf, int len)
+{
+    std::string s = which == 0 ? diag_controller_json(*static_cast<diag_t *>(d))
+                               : diag_operator_json(*static_cast<diag_t *>(d));
+    if (buf && len > 0)
+    {
+        std::strncpy(buf, s.c_str(), size_t(len) - 1);
+        buf[len - 1] = 0;
+    }
+    return int(s.size());
+}
+
 int orc_ctl_size(void) { return int(sizeof(ctl_t)); }
 void orc_ctl_init(void *p)
 {
@@ -1481,9 +1641,10 @@ void orc_bath_rates(int n, const double *y, const double *z, const double *size,
 int orc_operator(int n, const double *y, const double *z, double *size, double *numr, const uint8_t *active,
                  const int *spatial, const int *nbr, const double *measure, uint64_t *rng, uint64_t seed,
                  double duration, double total_dust_mass, void *ctl_ptr, int threads, orc_stats *stats,
-                 char *err, int err_len)
+                 char *err, int err_len, void *diag_ptr)
 {
     ctl_t &ctl = *static_cast<ctl_t *>(ctl_ptr);
+    diag_t *dg = static_cast<diag_t *>(diag_ptr);
     std::memset(stats, 0, sizeof(*stats));
     const int nth = thread_count(threads);
     try
@@ -1571,10 +1732,10 @@ int orc_operator(int n, const double *y, const double *z, double *size, double *
             return merged;
         };
         auto bin_end = [&](int c, int merged) { return c + 1 < LOCAL_GROUPS ? binmap[(c + 1)*COL_BIN_S] : merged; };
-        auto requested_step = [&](int c, int merged) {
+        auto requested_step = [&](int c, int merged, int *binding = nullptr) {
             int first = binmap[c*COL_BIN_S], last = bin_end(c, merged);
             std::vector<rate_bin> slice(ratebin.begin() + first, ratebin.begin() + last);
-            return choose_bath(slice, duration, ctl.state[c].limit_scale, nullptr);
+            return choose_bath(slice, duration, ctl.state[c].limit_scale, binding);
         };
 
         std::vector<int> all(n);
@@ -1582,28 +1743,69 @@ int orc_operator(int n, const double *y, const double *z, double *size, double *
         initialize(all);
         int merged = rates_and_bins(all);
         std::vector<double> requested(LOCAL_GROUPS);
-        for (int c = 0; c < LOCAL_GROUPS; ++c) requested[c] = requested_step(c, merged);
+        std::vector<diag_group> dgs(LOCAL_GROUPS);
+        for (int c = 0; c < LOCAL_GROUPS; ++c) requested[c] = requested_step(c, merged, &dgs[c].binding);
         schedule sched(duration, requested, edges);
         std::vector<double> steps(LOCAL_GROUPS);
         for (int c = 0; c < LOCAL_GROUPS; ++c) steps[c] = sched.seconds(sched.step[c]);
         std::vector<bool> passed(LOCAL_GROUPS, true);
         std::vector<double> current = requested;
         double dt_col = duration;
+        std::vector<double> published(LOCAL_GROUPS, 0.0);
+        std::vector<long long> cat_count(dg ? size_t(n)*7 : 0, 0);
+        std::vector<double> cat_log(dg ? size_t(n)*7 : 0, 0.0);
+        int op_index = 0, bath_index = 0;
+        if (dg)
+        {
+            op_index = ++dg->operator_count;
+            dg->waves = dg->owner_updates = dg->chain_blocks = dg->chain_launches = 0;
+            dg->duration = duration;
+            for (int c = 0; c < LOCAL_GROUPS; ++c)
+            {
+                dgs[c].initial_level = sched.level[c];
+                dgs[c].min_request = dgs[c].max_request = dgs[c].initial_request = requested[c];
+            }
+        }
         while (sched.time() < sched.end)
         {
             uint64_t tick = sched.time();
             auto groups = sched.due(tick);
+            double time = sched.seconds(tick);
             ids.clear();
-            for (int c : groups) ids.insert(ids.end(), owners[c].begin(), owners[c].end());
+            for (int c : groups)
+            {
+                published[c] = time;
+                ids.insert(ids.end(), owners[c].begin(), owners[c].end());
+            }
             int count = int(ids.size());
             if (count == 0) { sched.advance(groups); continue; }
             if (tick != 0)
             {
                 initialize(ids);
                 merged = rates_and_bins(ids);
-                for (int c : groups) current[c] = requested_step(c, merged);
+                for (int c : groups)
+                {
+                    current[c] = requested_step(c, merged);
+                    dgs[c].min_request = std::min(dgs[c].min_request, current[c]);
+                    dgs[c].max_request = std::max(dgs[c].max_request, current[c]);
+                }
+                std::vector<int> previous = sched.level;
                 sched.adapt(tick, groups, current, passed);
-                for (int c : groups) steps[c] = sched.seconds(sched.step[c]);
+                for (int c : groups)
+                {
+                    steps[c] = sched.seconds(sched.step[c]);
+                    dgs[c].coarsened += sched.level[c] < previous[c];
+                    dgs[c].refined += sched.level[c] > previous[c];
+                    dgs[c].constrained += steps[c] > current[c];
+                }
+            }
+            for (int c : groups)
+            {
+                if (owners[c].empty()) continue;
+                for (int d = 0; d < LOCAL_GROUPS; ++d)
+                    if (edges[size_t(c)*LOCAL_WORDS + d / 32] & (1u << (d % 32)))
+                        dgs[c].max_age = std::max(dgs[c].max_age, time - published[d]);
+                dgs[c].max_ratio = std::max(dgs[c].max_ratio, steps[c] / requested_step(c, merged));
             }
             // screen and chain (col_skip_scan + col_chain_run)
             int error = 0;
@@ -1626,6 +1828,29 @@ int orc_operator(int n, const double *y, const double *z, double *size, double *
             stats->events += events;
             stats->pair_evals += evals;
             if (error) throw std::runtime_error("local collision chain error " + std::to_string(error));
+            if (dg)
+            {
+                // chain blocks and launches of the reference's COL_EVENT_CAP continuation scheme
+                long long chained = 0, longest = 0;
+                for (int i : ids)
+                {
+                    if (!bath[i].chained) continue;
+                    chained++;
+                    longest = std::max<long long>(longest, bath[i].events);
+                    dg->chain_blocks += (unsigned long long)(bath[i].events / COL_EVENT_CAP + 1);
+                    for (int k = 0; k < 7; k++)
+                    {
+                        cat_count[size_t(i)*7 + k] += bath[i].cat_count[k];
+                        cat_log[size_t(i)*7 + k] += bath[i].cat_log[k];
+                    }
+                }
+                int continuations = chained ? int(longest / COL_EVENT_CAP + 1) : 0;
+                dg->chain_launches += (unsigned long long)continuations;
+                dg->continuation_launches += continuations;
+                dg->wave_count++;
+                dg->waves++;
+                dg->owner_updates += (unsigned long long)count;
+            }
             // col_audit_bin (col_audit_bin.cu:13-60)
             std::fill(audit.begin(), audit.end(), audit_bin{});
             for (int i : ids)
@@ -1672,9 +1897,19 @@ int orc_operator(int n, const double *y, const double *z, double *size, double *
                 }
                 if (!(mass > 0)) continue;
                 std::vector<audit_bin> slice(audit.begin() + first, audit.begin() + last);
+                double before = ctl.state[c].limit_scale;
                 auto result = finish_bath(slice, ctl.state[c]);
                 passed[c] = !(result.activity_overshoot || result.distribution_overshoot || result.persistent_overshoot);
                 stats->baths++;
+                diag_group &g = dgs[c];
+                g.updates++;
+                g.overshoots += result.activity_overshoot || result.distribution_overshoot;
+                g.persistent += result.persistent_overshoot;
+                g.max_growth = std::max(g.max_growth, result.max_g);
+                g.max_activity = std::max(g.max_activity, result.max_f);
+                if (dg)
+                    dg->baths.push_back(diag_bath{c, op_index, ++bath_index, last - first, steps[c], before,
+                                                  ctl.state[c].limit_scale, result});
             }
             stats->waves++;
             stats->owner_updates += count;
@@ -1684,6 +1919,25 @@ int orc_operator(int n, const double *y, const double *z, double *size, double *
         }
         stats->dt_col = dt_col;
         stats->clock_dyn = duration;
+        if (dg)
+        {
+            dg->finest_ticks = sched.end;
+            for (int c = 0; c < LOCAL_GROUPS; ++c)
+            {
+                dgs[c].owners = int(owners[c].size());
+                dgs[c].level = sched.level[c];
+                dgs[c].dt = steps[c];
+                dgs[c].final_request = current[c];
+            }
+            dg->groups = dgs;
+            for (int k = 0; k < 7; k++) { dg->cat_count[k] = 0; dg->cat_log[k] = 0.0; }
+            for (int i = 0; i < n; i++)
+                for (int k = 0; k < 7; k++)
+                {
+                    dg->cat_count[k] += cat_count[size_t(i)*7 + k];
+                    dg->cat_log[k] += cat_log[size_t(i)*7 + k];
+                }
+        }
         return 0;
     }
     catch (const std::exception &e)

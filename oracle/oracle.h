@@ -111,7 +111,12 @@ void orc_ctl_init(void *ctl);
 int orc_operator(int n, const double *y, const double *z, double *size, double *numr,
                  const uint8_t *active, const int *spatial, const int *nbr, const double *measure,
                  uint64_t *rng, uint64_t seed, double duration, double total_dust_mass,
-                 void *ctl, int threads, orc_stats *stats, char *err, int err_len);
+                 void *ctl, int threads, orc_stats *stats, char *err, int err_len, void *diag);
+// COL_DIAGNOSTICS bookkeeping of orc_operator (pass the object as `diag`, or null)
+void *orc_diag_new(void);
+void orc_diag_free(void *d);
+void orc_diag_reset(void *d);
+int orc_diag_json(void *d, int which, char *buf, int len);
 // bath-start rate moments for every owner (col_bath_rate with size_old = size)
 void orc_bath_rates(int n, const double *y, const double *z, const double *size, const double *numr,
                     const uint8_t *active, const int *nbr, const double *measure, double lambda_0,

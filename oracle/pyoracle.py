@@ -75,7 +75,12 @@ class Oracle:
         L.orc_ctl_size.restype = i
         L.orc_ctl_init.argtypes = [P]
         L.orc_operator.restype = i
-        L.orc_operator.argtypes = [i, P, P, P, P, P, P, P, P, P, u64, d, d, P, i, C.POINTER(Stats), C.c_char_p, i]
+        L.orc_operator.argtypes = [i, P, P, P, P, P, P, P, P, P, u64, d, d, P, i, C.POINTER(Stats), C.c_char_p, i, P]
+        L.orc_diag_new.restype = P
+        L.orc_diag_free.argtypes = [P]
+        L.orc_diag_reset.argtypes = [P]
+        L.orc_diag_json.restype = i
+        L.orc_diag_json.argtypes = [P, i, C.c_char_p, i]
         L.orc_bath_rates.argtypes = [i, P, P, P, P, P, P, P, d, P, i]
         info = ModelInfo()
         L.orc_get_model_info(C.byref(info))
@@ -270,7 +275,7 @@ class Oracle:
         return out
 
     def operator(self, y, z, size, numr, active, spatial, nbr, measure, rng, seed, duration, total_mass, ctl,
-                 threads=0):
+                 threads=0, diag=None):
         """Advance size/numr/rng in place; returns the stats dict. Raises on an operator error."""
         ys = np.ascontiguousarray(y, dtype=np.float64)
         zs = np.ascontiguousarray(z, dtype=np.float64)
@@ -284,7 +289,39 @@ class Oracle:
         st = Stats()
         err = C.create_string_buffer(256)
         rc = self.lib.orc_operator(len(ys), _p(ys), _p(zs), _p(size), _p(numr), _p(act), _p(sp), _p(nb), _p(me),
-                                   _p(rng), seed, duration, total_mass, _p(ctl), threads, C.byref(st), err, 256)
+                                   _p(rng), seed, duration, total_mass, _p(ctl), threads, C.byref(st), err, 256,
+                                   diag.ptr if diag is not None else None)
         if rc:
             raise RuntimeError(err.value.decode())
         return {f: getattr(st, f) for f, _ in st._fields_}
+
+
+class Diagnostics:
+    """COL_DIAGNOSTICS bookkeeping of Oracle.operator: the controller archive (schema 2) since the last
+    reset and the last operator's record (the schema-1 log line without wall-clock fields)."""
+
+    def __init__(self, oracle: Oracle):
+        self.lib = oracle.lib
+        self.ptr = self.lib.orc_diag_new()
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.lib.orc_diag_free(self.ptr)
+            self.ptr = None
+
+    def reset(self) -> None:
+        self.lib.orc_diag_reset(self.ptr)
+
+    def _json(self, which: int) -> str:
+        n = self.lib.orc_diag_json(self.ptr, which, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.lib.orc_diag_json(self.ptr, which, buf, n + 1)
+        return buf.value.decode()
+
+    def controller(self) -> dict:
+        import json
+        return json.loads(self._json(0))
+
+    def last_operator(self) -> dict:
+        import json
+        return json.loads(self._json(1))
