@@ -195,6 +195,20 @@ class CollisionOperator:
         return out
 
 
+def evolve_batch(ops: list, duration: float) -> list[dict]:
+    """One operator call on each of several CollisionOperator instances of the same model (independent
+    swarms), sharing every kernel launch; returns one stats dict per operator."""
+    if not ops:
+        return []
+    lib = ops[0].lib
+    if any(op.lib is not lib for op in ops):
+        raise ValueError("all operators of a batch must use the same model library")
+    handles = (C.c_void_p*len(ops))(*[op.h for op in ops])
+    stats = (_lib.Stats*len(ops))()
+    _lib.check(lib, ops[0].h, lib.gdx_col_operator_batch(handles, len(ops), float(duration), stats))
+    return [s.as_dict() for s in stats]
+
+
 class MortonIndex:
     """Stand-alone adaptive Morton index (morton_index::build, inc/swarm/morton/morton_index.cuh:168-229)."""
 

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -23,6 +24,7 @@ struct gdx_col_ctx
 {
     gdx::context c;
     gdx::diag_state diag;
+    gdx::run_buffers rb;
 };
 
 struct gdx_morton_ctx
@@ -70,22 +72,80 @@ void require(bool ok, int code, const char *what)
     if (!ok) throw status_error(code, what);
 }
 
-gdx_due make_due(const context &c, const std::vector<int> &groups, const std::vector<double> &steps)
+// the group-table entry of group g of swarm `run` of a run whose table starts that swarm at tbase
+gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
 {
-    gdx_due due{};
-    due.n = 0;
-    due.total = 0;
-    for (int g : groups)
+    gdx_gref r{};
+    r.st = c.st.p;
+    r.res = c.res.p;
+    r.nbr = c.nbr.p;
+    r.measure = c.measure.p;
+    r.env = c.env.p;
+    r.hgs = c.hgs.p;
+    r.cached = c.cached.p;
+    r.rng = c.rng.p;
+    r.slot_pidx = c.slot_pidx.p;
+    r.queue = c.queue.p;
+    r.qsum = c.qsum.p;
+    r.qevents = c.qevents.p;
+    r.evw = c.diag ? c.evw.p : nullptr;
+    r.raw_rate = rb.raw_rate.p + size_t(tbase + g)*COL_BIN_S;
+    r.raw_audit = rb.raw_audit.p + size_t(tbase + g)*COL_BIN_S;
+    r.qcount = rb.qcount.p + tbase + g;
+    r.scr_min = c.scr_min.p + g;
+    r.scr_max = c.scr_max.p + g;
+    r.lower = c.lower.p + g;
+    r.upper = c.upper.p + g;
+    r.counters = rb.counters.p + size_t(run)*4;
+    r.wave_ctr = rb.wave_ctr.p + size_t(run)*3;
+    r.err = rb.err.p + run;
+    r.lambda_0 = c.n / static_cast<double>(N_K) / c.total_mass;
+    r.seed = c.seed;
+    return r;
+}
+
+struct phase_entry
+{
+    int ref, beg, cnt;
+    double dt;
+};
+
+// due entries of one phase in device memory; the launch shapes for 1024 and 256 owners per block
+struct phase_wave
+{
+    gdx_wave w1024, w256;
+    int blocks1024 = 0, blocks256 = 0;
+};
+
+phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipStream_t st)
+{
+    const int n = int(e.size());
+    rb.reserve(n);
+    int *ref = rb.h_int, *beg = ref + n, *cnt = beg + n, *b1024 = cnt + n, *b256 = b1024 + n + 1;
+    int total = 0, x = 0, y = 0;
+    for (int i = 0; i < n; ++i)
     {
-        if (c.group_cnt[g] <= 0) continue;
-        due.gid[due.n] = g;
-        due.beg[due.n] = c.group_beg[g];
-        due.cnt[due.n] = c.group_cnt[g];
-        due.dt[due.n] = steps.empty() ? 0.0 : steps[g];
-        due.total += c.group_cnt[g];
-        due.n++;
+        ref[i] = e[i].ref;
+        beg[i] = e[i].beg;
+        cnt[i] = e[i].cnt;
+        rb.h_dt[i] = e[i].dt;
+        b1024[i] = x;
+        b256[i] = y;
+        x += (e[i].cnt + 1023) / 1024;
+        y += (e[i].cnt + 255) / 256;
+        total += e[i].cnt;
     }
-    return due;
+    b1024[n] = x;
+    b256[n] = y;
+    hip_check(hipMemcpyAsync(rb.d_int.p, rb.h_int, sizeof(int)*(5*size_t(n) + 2), hipMemcpyHostToDevice, st), "due list");
+    hip_check(hipMemcpyAsync(rb.d_dt.p, rb.h_dt, sizeof(double)*std::max(1, n), hipMemcpyHostToDevice, st), "due list");
+    phase_wave pw;
+    const int *d = rb.d_int.p;
+    pw.w1024 = gdx_wave{n, total, rb.table.p, d, d + n, d + 2*n, d + 3*n, rb.d_dt.p, rb.qprefix.p};
+    pw.w256 = gdx_wave{n, total, rb.table.p, d, d + n, d + 2*n, d + 4*n + 1, rb.d_dt.p, rb.qprefix.p};
+    pw.blocks1024 = x;
+    pw.blocks256 = y;
+    return pw;
 }
 
 void copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t st)
@@ -142,43 +202,122 @@ struct kernel_timer
 
 }  // namespace
 
-// one operator call (inc/swarm/_col_chain.cuh:108-539); with diagnostics it also keeps the
-// reference's controller archive and per-operator record (_col_chain.cuh:148-538)
-void run_operator(context &c, diag_state *dg, double duration, gdx_col_stats *out)
+// One operator call of one swarm (inc/swarm/_col_chain.cuh:108-539) as a sequence of phases: a rate
+// phase (publish, bath-start rates, binning; host: bath durations and scheduler) and a chain phase
+// (screen, chains, audit; host: bath audit and scheduler advance) per refresh wave. With
+// diagnostics it also keeps the reference's controller archive and per-operator record
+// (_col_chain.cuh:148-538). run_swarms drives several of these through shared launches.
+struct op_run
 {
-    require(c.state_loaded, GDX_COL_ERR_STATE, "collision state has not been loaded");
-    if (!(duration > 0) || !std::isfinite(duration)) throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
     using wall = std::chrono::steady_clock;
-    const auto begin = wall::now();
-    const bool diag = c.diag && dg;
-    hipStream_t st = c.stream;
-    const double lambda0 = c.n / static_cast<double>(N_K) / c.total_mass;
-    kernel_timer timer(c);
+    enum phase_t { RATE, CHAIN, DONE };
+
+    context &c;
+    diag_state *dg;
+    double duration;
+    bool diag;
+    int tbase = 0, index = 0;                 // first table entry and swarm index within the run
+    phase_t phase = RATE;
+    bool initial = true;
     gdx_col_stats stats{};
-    stats.active_owners = c.n_active;
-    hip_check(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned long long)*8, st), "zero counters");
-    if (diag) hip_check(hipMemsetAsync(c.evw.p, 0, sizeof(gdx_event_work)*std::max(1, c.n_active), st), "zero event work");
-    launch_reset_scratch(c);
-
-    std::vector<gdx_raw_rate> raw_rate(RAW_BINS);
-    std::vector<gdx_raw_audit> raw_audit(RAW_BINS);
-    std::vector<int> counts(RAW_BINS), binmap(RAW_BINS);
-    std::vector<rate_bin> ratebin(RAW_BINS);
-    std::vector<audit_bin> audit(RAW_BINS);
+    const gdx_raw_rate *raw_rate = nullptr;   // this swarm's raw bins in the run's host copies
+    const gdx_raw_audit *raw_audit = nullptr;
+    std::vector<int> counts, binmap, groups;
+    std::vector<rate_bin> ratebin;
+    std::vector<audit_bin> audit;
     int merged = 0;
+    std::unique_ptr<schedule> sched;
+    std::vector<double> requested, steps, current;
+    std::vector<bool> passed;
+    double dt_col = 0.0;
+    uint64_t tick = 0;
+    const int *err = nullptr;
+    const unsigned long long *wave_ctr = nullptr;
+    std::vector<group_stats> gs;
+    operator_record rec;
+    std::vector<double> published;
+    int operator_index = 0, bath_index = 0;
+    wall::time_point begin, chain_begin;
 
-    // publish, rate and bin the due owners; returns the merged-bin count (_col_chain.cuh:190-243)
-    auto rate_wave = [&](const gdx_due &due) {
-        hip_check(hipMemsetAsync(c.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, st), "zero rate bins");
-        timer.time(GDX_COL_KERNEL_PUBLISH, due.total, [&] { launch_publish(c, due); });
-        timer.time(GDX_COL_KERNEL_RATE, due.total, [&] { launch_bath_rate(c, due, lambda0); });
-        copy_d2h(raw_rate.data(), c.raw_rate.p, sizeof(gdx_raw_rate)*RAW_BINS, st);
-        hip_check(hipStreamSynchronize(st), "rating pass");
-        timer.resolve();
-        stats.pair_evals += (long long)due.total*N_K;
+    op_run(context &ctx, diag_state *d, double h)
+        : c(ctx), dg(d), duration(h), diag(ctx.diag && d),
+          counts(RAW_BINS), binmap(RAW_BINS), ratebin(RAW_BINS), audit(RAW_BINS), steps(LOCAL_GROUPS),
+          passed(LOCAL_GROUPS, true), published(LOCAL_GROUPS, 0.0)
+    {
+        require(c.state_loaded, GDX_COL_ERR_STATE, "collision state has not been loaded");
+        if (!(duration > 0) || !std::isfinite(duration))
+            throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
+        begin = wall::now();
+        stats.active_owners = c.n_active;
+        dt_col = duration;
+        groups.resize(LOCAL_GROUPS);
+        for (int g = 0; g < LOCAL_GROUPS; ++g) groups[g] = g;
+        if (diag) gs.assign(LOCAL_GROUPS, group_stats{});
+    }
+
+    void start(hipStream_t st, run_buffers &rb)
+    {
+        raw_rate = rb.h_rate.data() + size_t(tbase)*COL_BIN_S;
+        raw_audit = rb.h_audit.data() + size_t(tbase)*COL_BIN_S;
+        err = rb.h_err.data() + index;
+        wave_ctr = rb.h_wave.data() + size_t(index)*3;
+        if (diag)
+            hip_check(hipMemsetAsync(c.evw.p, 0, sizeof(gdx_event_work)*std::max(1, c.n_active), st), "zero event work");
+    }
+
+    void entries(std::vector<phase_entry> &e, bool chain) const
+    {
+        for (int g : groups)
+            if (c.group_cnt[g] > 0)
+                e.push_back(phase_entry{tbase + g, c.group_beg[g], c.group_cnt[g], chain ? steps[g] : 0.0});
+    }
+    long long due_owners() const
+    {
+        long long n = 0;
+        for (int g : groups) n += std::max(0, c.group_cnt[g]);
+        return n;
+    }
+
+    int bin_end(int g) const { return g + 1 < LOCAL_GROUPS ? binmap[(g + 1)*COL_BIN_S] : merged; }
+    double requested_step(int g, int *binding = nullptr) const
+    {
+        int first = binmap[g*COL_BIN_S];
+        return choose_bath(ratebin.data() + first, bin_end(g) - first, duration, c.state[g].limit_scale, binding);
+    }
+
+    // the next due groups with listed owners (_col_chain.cuh:291-313), or DONE at the horizon
+    void next_groups()
+    {
+        while (sched->time() < sched->end)
+        {
+            tick = sched->time();
+            groups = sched->due(tick);
+            const double time = sched->seconds(tick);
+            long long listed = 0;
+            for (int g : groups)
+            {
+                listed += c.group_all[g];
+                published[g] = time;
+            }
+            if (listed == 0)
+            {
+                sched->advance(groups);
+                continue;
+            }
+            phase = tick != 0 ? RATE : CHAIN;
+            if (phase == CHAIN) before_chain();
+            return;
+        }
+        phase = DONE;
+    }
+
+    // merged rate bins of the rated groups (_col_chain.cuh:205-243)
+    void after_rate()
+    {
+        stats.pair_evals += due_owners()*N_K;
         for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count;
-        int m = build_binmap(counts, binmap);
-        std::fill(ratebin.begin(), ratebin.begin() + m, rate_bin{});
+        merged = build_binmap(counts, binmap);
+        std::fill(ratebin.begin(), ratebin.begin() + merged, rate_bin{});
         for (int r = 0; r < RAW_BINS; ++r)
         {
             rate_bin &b = ratebin[binmap[r]];
@@ -189,84 +328,56 @@ void run_operator(context &c, diag_state *dg, double duration, gdx_col_stats *ou
             b.owner_count += raw_rate[r].owner_count;
             b.invalid_count += raw_rate[r].invalid_count;
         }
-        return m;
-    };
-    auto bin_end = [&](int g) { return g + 1 < LOCAL_GROUPS ? binmap[(g + 1)*COL_BIN_S] : merged; };
-    auto requested_step = [&](int g, int *binding = nullptr) {
-        int first = binmap[g*COL_BIN_S];
-        return choose_bath(ratebin.data() + first, bin_end(g) - first, duration, c.state[g].limit_scale, binding);
-    };
-
-    std::vector<int> all(LOCAL_GROUPS);
-    for (int g = 0; g < LOCAL_GROUPS; ++g) all[g] = g;
-    merged = rate_wave(make_due(c, all, {}));
-    std::vector<double> requested(LOCAL_GROUPS);
-    std::vector<group_stats> gs(diag ? LOCAL_GROUPS : 0);
-    for (int g = 0; g < LOCAL_GROUPS; ++g)
-        requested[g] = requested_step(g, diag ? &gs[g].binding : nullptr);
-    schedule sched(duration, requested, c.edges);
-    std::vector<double> steps(LOCAL_GROUPS);
-    for (int g = 0; g < LOCAL_GROUPS; ++g) steps[g] = sched.seconds(sched.step[g]);
-    std::vector<bool> passed(LOCAL_GROUPS, true);
-    std::vector<double> current = requested;
-    double dt_col = duration;
-
-    operator_record rec;
-    std::vector<double> published(LOCAL_GROUPS, 0.0);
-    int operator_index = 0, bath_index = 0;
-    if (diag)
-    {
-        for (int g = 0; g < LOCAL_GROUPS; ++g)
+        if (initial)
         {
-            gs[g].initial_level = sched.level[g];
-            gs[g].min_request = gs[g].max_request = gs[g].initial_request = requested[g];
+            initial = false;
+            requested.assign(LOCAL_GROUPS, 0.0);
+            for (int g = 0; g < LOCAL_GROUPS; ++g) requested[g] = requested_step(g, diag ? &gs[g].binding : nullptr);
+            sched = std::make_unique<schedule>(duration, requested, c.edges);
+            for (int g = 0; g < LOCAL_GROUPS; ++g) steps[g] = sched->seconds(sched->step[g]);
+            current = requested;
+            if (diag)
+            {
+                for (int g = 0; g < LOCAL_GROUPS; ++g)
+                {
+                    gs[g].initial_level = sched->level[g];
+                    gs[g].min_request = gs[g].max_request = gs[g].initial_request = requested[g];
+                }
+                operator_index = ++dg->summary.operator_count;
+            }
+            next_groups();
+            return;
         }
-        operator_index = ++dg->summary.operator_count;
-    }
-
-    while (sched.time() < sched.end)
-    {
-        const uint64_t tick = sched.time();
-        std::vector<int> groups = sched.due(tick);
-        const double time = sched.seconds(tick);
-        long long listed = 0;
         for (int g : groups)
         {
-            listed += c.group_all[g];
-            if (diag) published[g] = time;
-        }
-        if (listed == 0)
-        {
-            sched.advance(groups);
-            continue;
-        }
-        if (tick != 0)
-        {
-            merged = rate_wave(make_due(c, groups, {}));
-            for (int g : groups)
+            current[g] = requested_step(g);
+            if (diag)
             {
-                current[g] = requested_step(g);
-                if (diag)
-                {
-                    gs[g].min_request = std::min(gs[g].min_request, current[g]);
-                    gs[g].max_request = std::max(gs[g].max_request, current[g]);
-                }
-            }
-            std::vector<int> previous = sched.level;
-            sched.adapt(tick, groups, current, passed);
-            for (int g : groups)
-            {
-                steps[g] = sched.seconds(sched.step[g]);
-                if (diag)
-                {
-                    gs[g].coarsened += sched.level[g] < previous[g];
-                    gs[g].refined += sched.level[g] > previous[g];
-                    gs[g].constrained += steps[g] > current[g];
-                }
+                gs[g].min_request = std::min(gs[g].min_request, current[g]);
+                gs[g].max_request = std::max(gs[g].max_request, current[g]);
             }
         }
+        std::vector<int> previous = sched->level;
+        sched->adapt(tick, groups, current, passed);
+        for (int g : groups)
+        {
+            steps[g] = sched->seconds(sched->step[g]);
+            if (diag)
+            {
+                gs[g].coarsened += sched->level[g] < previous[g];
+                gs[g].refined += sched->level[g] > previous[g];
+                gs[g].constrained += steps[g] > current[g];
+            }
+        }
+        phase = CHAIN;
+        before_chain();
+    }
+
+    void before_chain()
+    {
         if (diag)
         {
+            const double time = sched->seconds(tick);
             for (int g : groups)
             {
                 if (c.group_all[g] == 0) continue;
@@ -276,25 +387,15 @@ void run_operator(context &c, diag_state *dg, double duration, gdx_col_stats *ou
                 gs[g].max_requested_ratio = std::max(gs[g].max_requested_ratio, steps[g] / requested_step(g));
             }
         }
-        const auto chain_begin = wall::now();
-        gdx_due due = make_due(c, groups, steps);
-        hip_check(hipMemsetAsync(c.raw_audit.p, 0, sizeof(gdx_raw_audit)*RAW_BINS, st), "zero audit bins");
-        hip_check(hipMemsetAsync(c.err.p, 0, sizeof(int), st), "zero error");
-        if (diag) hip_check(hipMemsetAsync(c.counters.p + 4, 0, sizeof(unsigned long long)*3, st), "zero wave counters");
-        timer.time(GDX_COL_KERNEL_SCREEN, due.total, [&] { launch_screen(c, due); });
-        timer.time(GDX_COL_KERNEL_CHAIN, due.total, [&] { launch_chain(c, due, lambda0); });
-        timer.time(GDX_COL_KERNEL_AUDIT, due.total, [&] { launch_chain_audit(c, due); });
-        int err = 0;
-        unsigned long long wave_ctr[3] = {0, 0, 0};
-        copy_d2h(&err, c.err.p, sizeof(int), st);
-        copy_d2h(raw_audit.data(), c.raw_audit.p, sizeof(gdx_raw_audit)*RAW_BINS, st);
-        if (diag) copy_d2h(wave_ctr, c.counters.p + 4, sizeof(wave_ctr), st);
-        hip_check(hipStreamSynchronize(st), "chain pass");
-        timer.resolve();
-        if (err) throw status_error(err, "local collision chain error " + std::to_string(err));
+        chain_begin = wall::now();
+    }
+
+    // bath audits (_col_chain.cuh:386-470), then the scheduler advance
+    void after_chain()
+    {
+        if (*err) throw status_error(*err, "local collision chain error " + std::to_string(*err));
         const auto audit_begin = wall::now();
         if (diag) rec.chain_seconds += std::chrono::duration<double>(audit_begin - chain_begin).count();
-
         std::fill(audit.begin(), audit.begin() + merged, audit_bin{});
         for (int r = 0; r < RAW_BINS; ++r)
         {
@@ -365,49 +466,141 @@ void run_operator(context &c, diag_state *dg, double duration, gdx_col_stats *ou
             rec.audit_seconds += std::chrono::duration<double>(wall::now() - audit_begin).count();
         }
         stats.waves++;
-        stats.owner_updates += due.total;
+        stats.owner_updates += due_owners();
         dt_col = duration;
         for (int g : groups) dt_col = std::min(dt_col, steps[g]);
-        sched.advance(groups);
+        sched->advance(groups);
+        next_groups();
     }
-    unsigned long long counters[4] = {0, 0, 0, 0};
-    copy_d2h(counters, c.counters.p, sizeof(counters), st);
-    if (diag)
+
+    // counters: this swarm's four run counters, read back after the run
+    void finish(hipStream_t st, const unsigned long long *counters, gdx_col_stats *out)
     {
-        launch_event_sum(c);
-        copy_d2h(&dg->totals, c.ev_sum.p, sizeof(gdx_event_work), st);
-    }
-    if (!(c.flags & GDX_COL_DEFER_WRITEBACK)) store_state(c);
-    hip_check(hipStreamSynchronize(st), "operator");
-    stats.events = (long long)counters[0];
-    stats.chain_owners = (long long)counters[1];
-    stats.pair_evals += (long long)counters[2];
-    stats.max_chain_events = (long long)counters[3];
-    stats.dt_col = dt_col;
-    stats.clock_dyn = duration;
-    if (diag && dg->log.is_open())
-    {
-        rec.clock_sim = dg->clock_sim;
-        rec.duration = duration;
-        rec.waves = uint64_t(stats.waves);
-        rec.owner_updates = uint64_t(stats.owner_updates);
-        rec.finest_ticks = sched.end;
-        rec.owners.assign(c.group_all.begin(), c.group_all.end());
-        rec.level = sched.level;
-        rec.dt = steps;
-        rec.final_request = current;
-        rec.groups = gs;
-        for (int k = 0; k < GDX_EVENT_CATEGORIES; ++k)
+        if (diag)
         {
-            rec.event_count[k] = dg->totals.count[k];
-            rec.event_log_mass[k] = dg->totals.log_mass[k];
+            launch_event_sum(c, st);
+            copy_d2h(&dg->totals, c.ev_sum.p, sizeof(gdx_event_work), st);
         }
-        rec.scheduler_wall_seconds = std::chrono::duration<double>(wall::now() - begin).count();
-        write_operator_line(dg->log, ++dg->operator_id, rec);
-        dg->log.flush();
-        if (!dg->log) throw status_error(GDX_COL_ERR_ARG, "cannot write local collision diagnostics");
+        hip_check(hipStreamSynchronize(st), "operator");
+        if (!(c.flags & GDX_COL_DEFER_WRITEBACK))
+        {
+            store_state(c);
+            hip_check(hipStreamSynchronize(c.stream), "write-back");
+        }
+        stats.events = (long long)counters[0];
+        stats.chain_owners = (long long)counters[1];
+        stats.pair_evals += (long long)counters[2];
+        stats.max_chain_events = (long long)counters[3];
+        stats.dt_col = dt_col;
+        stats.clock_dyn = duration;
+        if (diag && dg->log.is_open())
+        {
+            rec.clock_sim = dg->clock_sim;
+            rec.duration = duration;
+            rec.waves = uint64_t(stats.waves);
+            rec.owner_updates = uint64_t(stats.owner_updates);
+            rec.finest_ticks = sched ? sched->end : 0;
+            rec.owners.assign(c.group_all.begin(), c.group_all.end());
+            rec.level = sched ? sched->level : std::vector<int>(LOCAL_GROUPS, 0);
+            rec.dt = steps;
+            rec.final_request = current.empty() ? std::vector<double>(LOCAL_GROUPS, 0.0) : current;
+            rec.groups = gs;
+            for (int k = 0; k < GDX_EVENT_CATEGORIES; ++k)
+            {
+                rec.event_count[k] = dg->totals.count[k];
+                rec.event_log_mass[k] = dg->totals.log_mass[k];
+            }
+            rec.scheduler_wall_seconds = std::chrono::duration<double>(wall::now() - begin).count();
+            write_operator_line(dg->log, ++dg->operator_id, rec);
+            dg->log.flush();
+            if (!dg->log) throw status_error(GDX_COL_ERR_ARG, "cannot write local collision diagnostics");
+        }
+        if (out) *out = stats;
     }
-    if (out) *out = stats;
+};
+
+// Advance several swarms through one operator horizon each: every loop runs one rate phase over the
+// due groups of all swarms that need rates and one chain phase over those whose baths are set, with
+// one launch per kernel and one host synchronisation per phase for all of them.
+void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, kernel_timer &timer)
+{
+    const int nr = int(runs.size());
+    const int ng = nr*LOCAL_GROUPS;
+    rb.size_run(ng, nr);
+    rb.h_table.clear();
+    for (int i = 0; i < nr; ++i)
+    {
+        op_run *r = runs[i];
+        r->index = i;
+        r->tbase = i*LOCAL_GROUPS;
+        for (int g = 0; g < LOCAL_GROUPS; ++g) rb.h_table.push_back(make_gref(r->c, g, rb, i, r->tbase));
+    }
+    rb.table.alloc(ng);
+    rb.reserve(ng);
+    hip_check(hipMemcpyAsync(rb.table.p, rb.h_table.data(), sizeof(gdx_gref)*ng, hipMemcpyHostToDevice, st),
+              "group table");
+    hip_check(hipMemsetAsync(rb.counters.p, 0, sizeof(unsigned long long)*4*nr, st), "zero counters");
+    if (!rb.resident) rb.resident = chain_resident_blocks();
+    for (op_run *r : runs) r->start(st, rb);
+    launch_reset_scratch(st, rb.table.p, ng);
+    std::vector<phase_entry> e;
+    std::vector<op_run *> now;
+    while (true)
+    {
+        bool work = false;
+        // ---- rate phase ----
+        e.clear();
+        now.clear();
+        for (op_run *r : runs)
+            if (r->phase == op_run::RATE) { now.push_back(r); r->entries(e, false); }
+        if (!now.empty())
+        {
+            work = true;
+            hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st), "zero rate bins");
+            phase_wave pw = upload_phase(rb, e, st);
+            timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024); });
+            timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.w256, pw.blocks256); });
+            copy_d2h(rb.h_rate.data(), rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st);
+            hip_check(hipStreamSynchronize(st), "rating pass");
+            timer.resolve();
+            for (op_run *r : now) r->after_rate();
+        }
+        // ---- chain phase ----
+        e.clear();
+        now.clear();
+        for (op_run *r : runs)
+            if (r->phase == op_run::CHAIN) { now.push_back(r); r->entries(e, true); }
+        if (!now.empty())
+        {
+            work = true;
+            hip_check(hipMemsetAsync(rb.raw_audit.p, 0, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st), "zero audit bins");
+            hip_check(hipMemsetAsync(rb.err.p, 0, sizeof(int)*nr, st), "zero errors");
+            hip_check(hipMemsetAsync(rb.qcount.p, 0, sizeof(int)*ng, st), "zero queue counters");
+            hip_check(hipMemsetAsync(rb.wave_ctr.p, 0, sizeof(unsigned long long)*3*nr, st), "zero wave counters");
+            hip_check(hipMemsetAsync(rb.work.p, 0, sizeof(int), st), "zero chain counter");
+            phase_wave pw = upload_phase(rb, e, st);
+            timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256, rb.qprefix.p); });
+            timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.work.p, rb.resident); });
+            timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w256, pw.blocks256); });
+            copy_d2h(rb.h_err.data(), rb.err.p, sizeof(int)*nr, st);
+            copy_d2h(rb.h_audit.data(), rb.raw_audit.p, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st);
+            copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*3*nr, st);
+            hip_check(hipStreamSynchronize(st), "chain pass");
+            timer.resolve();
+            for (op_run *r : now) r->after_chain();
+        }
+        if (!work) break;
+    }
+    copy_d2h(rb.h_counters.data(), rb.counters.p, sizeof(unsigned long long)*4*nr, st);
+}
+
+void run_operator(context &c, diag_state *dg, run_buffers &rb, double duration, gdx_col_stats *out)
+{
+    op_run r(c, dg, duration);
+    std::vector<op_run *> runs{&r};
+    kernel_timer timer(c);
+    run_swarms(runs, rb, c.stream, timer);
+    r.finish(c.stream, rb.h_counters.data(), out);
 }
 
 }  // namespace gdx
@@ -474,10 +667,6 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         c.upper.alloc(LOCAL_GROUPS);
         hip_check(hipMemset(c.lower.p, 0, sizeof(double)*LOCAL_GROUPS), "zero bounds");
         hip_check(hipMemset(c.upper.p, 0, sizeof(double)*LOCAL_GROUPS), "zero bounds");
-        c.raw_rate.alloc(RAW_BINS);
-        c.raw_audit.alloc(RAW_BINS);
-        c.err.alloc(1);
-        c.counters.alloc(8);
     });
     if (rc != GDX_COL_OK)
     {
@@ -572,7 +761,33 @@ int gdx_col_store_state(gdx_col_handle h)
 int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out)
 {
     if (!h) return GDX_COL_ERR_ARG;
-    return guard(&h->c, [&] { run_operator(h->c, &h->diag, duration, out); });
+    return guard(&h->c, [&] { run_operator(h->c, &h->diag, h->rb, duration, out); });
+}
+
+int gdx_col_operator_batch(const gdx_col_handle *handles, int count, double duration, gdx_col_stats *out)
+{
+    if (!handles || count <= 0 || !handles[0]) return GDX_COL_ERR_ARG;
+    for (int i = 0; i < count; ++i)
+    {
+        if (!handles[i]) return GDX_COL_ERR_ARG;
+        for (int j = 0; j < i; ++j)
+            if (handles[j] == handles[i]) return GDX_COL_ERR_ARG;
+    }
+    gdx_col_ctx *h0 = handles[0];
+    return guard(&h0->c, [&] {
+        std::vector<std::unique_ptr<op_run>> owned;
+        std::vector<op_run *> runs;
+        for (int i = 0; i < count; ++i)
+        {
+            owned.push_back(std::make_unique<op_run>(handles[i]->c, &handles[i]->diag, duration));
+            runs.push_back(owned.back().get());
+        }
+        // every swarm's device work goes to the first handle's stream
+        for (int i = 1; i < count; ++i) hip_check(hipStreamSynchronize(handles[i]->c.stream), "batch stream");
+        kernel_timer timer(h0->c);
+        run_swarms(runs, h0->rb, h0->c.stream, timer);
+        for (int i = 0; i < count; ++i) runs[i]->finish(h0->c.stream, h0->rb.h_counters.data() + 4*i, out ? out + i : nullptr);
+    });
 }
 
 int gdx_col_reset_controller(gdx_col_handle h)
@@ -819,13 +1034,22 @@ int gdx_col_rate_pass(gdx_col_handle h)
     return guard(&h->c, [&] {
         context &c = h->c;
         require(c.state_loaded, GDX_COL_ERR_STATE, "collision state has not been loaded");
-        std::vector<int> all(LOCAL_GROUPS);
-        for (int g = 0; g < LOCAL_GROUPS; ++g) all[g] = g;
-        gdx_due due = make_due(c, all, {});
-        launch_reset_scratch(c);
-        hip_check(hipMemsetAsync(c.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, c.stream), "zero rate bins");
-        launch_publish(c, due);
-        launch_bath_rate(c, due, c.n / static_cast<double>(N_K) / c.total_mass);
+        run_buffers &rb = h->rb;
+        rb.size_run(LOCAL_GROUPS, 1);
+        rb.reserve(LOCAL_GROUPS);
+        rb.h_table.clear();
+        for (int g = 0; g < LOCAL_GROUPS; ++g) rb.h_table.push_back(make_gref(c, g, rb, 0, 0));
+        rb.table.alloc(LOCAL_GROUPS);
+        hip_check(hipMemcpyAsync(rb.table.p, rb.h_table.data(), sizeof(gdx_gref)*LOCAL_GROUPS, hipMemcpyHostToDevice,
+                                 c.stream), "group table");
+        std::vector<phase_entry> e;
+        for (int g = 0; g < LOCAL_GROUPS; ++g)
+            if (c.group_cnt[g] > 0) e.push_back(phase_entry{g, c.group_beg[g], c.group_cnt[g], 0.0});
+        launch_reset_scratch(c.stream, rb.table.p, LOCAL_GROUPS);
+        hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, c.stream), "zero rate bins");
+        phase_wave pw = upload_phase(rb, e, c.stream);
+        launch_publish(c.stream, pw.w1024, pw.blocks1024);
+        launch_bath_rate(c.stream, pw.w256, pw.blocks256);
         hip_check(hipStreamSynchronize(c.stream), "rating pass");
     });
 }

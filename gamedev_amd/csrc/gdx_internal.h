@@ -87,6 +87,72 @@ struct kd_tree
     dbuf<float> world;         // data bounds {lo xyz, hi xyz}
 };
 
+// buffers of operator runs: the group table, pinned staging and device copies of the due lists,
+// the chain's queue prefix and work counter
+struct run_buffers
+{
+    dbuf<gdx_gref> table;
+    std::vector<gdx_gref> h_table;
+    int *h_int = nullptr;
+    double *h_dt = nullptr;
+    size_t cap = 0;
+    dbuf<int> d_int;
+    dbuf<double> d_dt;
+    dbuf<int> qprefix;
+    dbuf<int> work;
+    int resident = 0;                 // resident blocks of the persistent chain kernel
+    // controller sums, queue lengths, counters and errors of every group and swarm of the run, so that
+    // each phase clears and reads them back with one call each
+    dbuf<gdx_raw_rate> raw_rate;
+    dbuf<gdx_raw_audit> raw_audit;
+    dbuf<int> qcount;
+    dbuf<int> err;
+    dbuf<unsigned long long> counters, wave_ctr;
+    std::vector<gdx_raw_rate> h_rate;
+    std::vector<gdx_raw_audit> h_audit;
+    std::vector<int> h_err;
+    std::vector<unsigned long long> h_counters, h_wave;
+    void size_run(int groups, int swarms)
+    {
+        raw_rate.alloc(size_t(groups)*COL_BIN_S);
+        raw_audit.alloc(size_t(groups)*COL_BIN_S);
+        qcount.alloc(groups);
+        err.alloc(swarms);
+        counters.alloc(size_t(swarms)*4);
+        wave_ctr.alloc(size_t(swarms)*3);
+        h_rate.resize(size_t(groups)*COL_BIN_S);
+        h_audit.resize(size_t(groups)*COL_BIN_S);
+        h_err.resize(swarms);
+        h_counters.resize(size_t(swarms)*4);
+        h_wave.resize(size_t(swarms)*3);
+    }
+    run_buffers() = default;
+    run_buffers(const run_buffers &) = delete;
+    run_buffers &operator=(const run_buffers &) = delete;
+    ~run_buffers()
+    {
+        if (h_int) (void)hipHostFree(h_int);
+        if (h_dt) (void)hipHostFree(h_dt);
+    }
+    // room for n due entries
+    void reserve(size_t n)
+    {
+        if (n <= cap && h_int) return;
+        size_t m = std::max<size_t>(n, 64);
+        if (h_int) (void)hipHostFree(h_int);
+        if (h_dt) (void)hipHostFree(h_dt);
+        h_int = nullptr;
+        h_dt = nullptr;
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_int), sizeof(int)*(5*m + 2)), "pinned due list");
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_dt), sizeof(double)*m), "pinned due list");
+        d_int.alloc(5*m + 2);
+        d_dt.alloc(m);
+        qprefix.alloc(m + 1);
+        work.alloc(1);
+        cap = m;
+    }
+};
+
 // reusable scratch for radix sorts and scans
 struct sort_scratch
 {
@@ -151,18 +217,12 @@ struct context
     // controller scratch
     dbuf<unsigned long long> scr_min, scr_max;
     dbuf<double> lower, upper;
-    dbuf<gdx_raw_rate> raw_rate;
-    dbuf<gdx_raw_audit> raw_audit;
-    dbuf<int> err;
     dbuf<int> queue;           // event owners of one wave, in their group's slot range
-    dbuf<int> qcount;          // per-group queue lengths + the chain work counter
     dbuf<double4> qsum;        // chain sums (hazard, j1, j2, jmax) per queue position
     dbuf<int> qevents;         // chain events per queue position, -1 after an error
-    int chain_blocks = 0;      // resident blocks of the persistent chain kernel
     bool diag = false;         // GDX_COL_DIAGNOSTICS
     dbuf<gdx_event_work> evw;  // per-slot event statistics of the current call
     dbuf<gdx_event_work> ev_sum;
-    dbuf<unsigned long long> counters;   // events, chain owners, pair evaluations
     std::vector<double> h_lower, h_upper;
 
     // host controller memory (inc/swarm/_col_chain.cuh:46; persists across operator calls)
@@ -193,15 +253,16 @@ void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int
                unsigned int *overflow, hipStream_t st);
 
 // ---- operator.hip ---------------------------------------------------------------------------
-void launch_reset_scratch(context &c);
 void kd_build(context &c);
 void kd_query(context &c);
-void launch_publish(context &c, const gdx_due &due);
-void launch_bath_rate(context &c, const gdx_due &due, double lambda_0);
-void launch_screen(context &c, const gdx_due &due);
-void launch_event_sum(context &c);
-void launch_chain(context &c, const gdx_due &due, double lambda_0);
-void launch_chain_audit(context &c, const gdx_due &due);
+void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng);
+void launch_publish(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix);
+int chain_resident_blocks();
+void launch_chain(hipStream_t st, const gdx_wave &w, int *work, int resident);
+void launch_chain_audit(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_event_sum(context &c, hipStream_t st);
 void launch_gather_cached(context &c, double *out_particle_order);
 
 }  // namespace gdx

@@ -80,16 +80,49 @@ struct gdx_event_work
     double log_mass[GDX_EVENT_CATEGORIES];
 };
 
-// the due groups of one refresh wave and their contiguous owner slot ranges
-struct gdx_due
+// one controller group of one swarm among the swarms an operator run advances together: where the
+// group's owner arrays live and its per-group scalars (pointers already offset to the group)
+struct gdx_gref
+{
+    double2 *st, *res;
+    const int *nbr;
+    const double *measure;
+    const gdx_env *env;
+    const double *hgs;
+    double4 *cached;
+    uint64_t *rng;
+    const int *slot_pidx;
+    int *queue;
+    double4 *qsum;
+    int *qevents;
+    gdx_event_work *evw;                      // null without diagnostics
+    gdx_raw_rate *raw_rate;                   // the group's COL_BIN_S raw bins
+    gdx_raw_audit *raw_audit;
+    int *qcount;                              // the group's event-queue length
+    unsigned long long *scr_min, *scr_max;    // the group's size extrema scratch
+    double *lower, *upper;                    // the group's moving size bounds
+    unsigned long long *counters;             // the swarm's operator counters: events, chained owners, pair
+                                              // terms, longest chain
+    unsigned long long *wave_ctr;             // the swarm's per-wave counters: longest chain, chain blocks of
+                                              // the reference's continuation scheme, chained owners
+    int *err;                                 // the swarm's chain error
+    double lambda_0;
+    uint64_t seed;
+};
+
+// the due groups of one phase of a run, in device memory: group-table index, owner slot range in the
+// group's swarm, first block of each group for the launching kernel's owners per block, bath end
+struct gdx_wave
 {
     int n;                       // due groups with active owners
-    int total;                   // total owners
-    int gid[64];
-    int beg[64];
-    int cnt[64];
-    int bstart[65];              // first block of each due group for the launching kernel
-    double dt[64];               // bath end per due group
+    int total;                   // their owners
+    const gdx_gref *table;
+    const int *ref;
+    const int *beg;
+    const int *cnt;
+    const int *bstart;           // n + 1 entries
+    const double *dt;
+    const int *qprefix;          // chain: exclusive prefix of the groups' queue lengths (n + 1)
 };
 
 #endif  // GDX_MODEL_H

@@ -897,7 +897,6 @@ void load_state(context &c)
     c.cached.alloc(na);
     c.rng.alloc(na);
     c.queue.alloc(na);
-    c.qcount.alloc(LOCAL_GROUPS + 1);
     c.qsum.alloc(na);
     if (c.diag) c.evw.alloc(na);
     c.qevents.alloc(na);

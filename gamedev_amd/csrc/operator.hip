@@ -1,17 +1,20 @@
 // Frozen-bath collision operator kernels on gfx950.
 //
-// One refresh wave of evolve_local_collisions (inc/swarm/_col_chain.cuh:291-485) runs three kernels
-// over the due groups' contiguous owner slots:
-//   k_publish    col_bath_init + col_comp_zero + col_size_scan (src/swarm/col_bath_init.cu,
-//                col_comp_zero.cu, col_size_scan.cu): freeze the reservoir, record size extrema
-//   k_bath_rate  col_size_bnds + col_bath_rate + col_count_bin + col_rate_bins (col_size_bnds.cu,
-//                col_bath_rate.cu, col_count_bin.cu, col_rate_bins.cu): bath-start rate moments and
-//                per-raw-bin controller sums
-//   k_chain      col_skip_scan + col_chain_run + col_audit_bin (col_skip_scan.cu, col_chain_run.cu,
-//                col_audit_bin.cu): screen, run every event chain to the bath end, audit sums
+// One refresh wave of evolve_local_collisions (inc/swarm/_col_chain.cuh:291-485) runs in two phases
+// over the due groups' contiguous owner slots. Rate phase:
+//   k_publish      col_bath_init + col_comp_zero + col_size_scan (src/swarm/col_bath_init.cu,
+//                  col_comp_zero.cu, col_size_scan.cu): freeze the reservoir, record size extrema
+//   k_bath_rate    col_size_bnds + col_bath_rate + col_count_bin + col_rate_bins (col_size_bnds.cu,
+//                  col_bath_rate.cu, col_count_bin.cu, col_rate_bins.cu): bath-start rate moments and
+//                  per-raw-bin controller sums
+// Chain phase:
+//   k_screen       col_skip_scan (col_skip_scan.cu): no-event owners finish and enter the audit sums,
+//                  the others are queued per group
+//   k_chain        col_chain_run (col_chain_run.cu): every queued owner's chain to its bath end
+//   k_chain_audit  col_audit_bin (col_audit_bin.cu) for the chained owners
 // Merged-bin sums are sums of raw-bin sums, so the host merges after the binmap is known.
-// A wavefront segment of N_K lanes holds one owner, lane k evaluating neighbour slot k; the
-// xor-butterfly sums equal the reference's generic upper-half fold (col_bath_rate.cu:105-118).
+// Every kernel reads its groups through a table of per-group array pointers, so one launch serves
+// the due groups of several independent swarms (realizations) as well as those of one.
 #include "gdx_internal.h"
 #include "gdx_device.h"
 
@@ -34,15 +37,22 @@ constexpr int CHAIN_CHUNK = 4*NSEG;     // queue positions a chain wavefront tak
 // 1e6 continuation launches (inc/swarm/_col_chain.cuh:381-384); exceeding this bound is status 7
 constexpr int EVENT_LIMIT = 1 << 22;
 
-__device__ __forceinline__ bool due_block(const gdx_due &due, int per_block, int &gi, int &s0, int &s1)
+// the due group of this block: last entry with bstart <= block (binary search, block-uniform)
+__device__ __forceinline__ bool due_block(const gdx_wave &w, int per_block, int &gi, int &s0, int &s1)
 {
-    int b = xcd_block();
-    gi = 0;
-    while (gi < due.n && b >= due.bstart[gi + 1]) ++gi;
-    if (gi >= due.n) return false;
-    int local = b - due.bstart[gi];
-    s0 = due.beg[gi] + local*per_block;
-    s1 = min(due.beg[gi] + due.cnt[gi], s0 + per_block);
+    const int b = xcd_block();
+    if (b >= w.bstart[w.n]) return false;
+    int lo = 0, hi = w.n - 1;
+    while (lo < hi)
+    {
+        int mid = (lo + hi + 1) >> 1;
+        if (w.bstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    gi = lo;
+    int local = b - w.bstart[gi];
+    s0 = w.beg[gi] + local*per_block;
+    s1 = min(w.beg[gi] + w.cnt[gi], s0 + per_block);
     return s0 < s1;
 }
 
@@ -144,24 +154,25 @@ __device__ __forceinline__ int seg_max_u(int v)
     }
 }
 
-__global__ void k_reset_scratch(unsigned long long *scr_min, unsigned long long *scr_max)
+__global__ void k_reset_scratch(const gdx_gref *table, int ng)
 {
-    int g = threadIdx.x;
-    if (g < LOCAL_GROUPS)
+    int g = blockIdx.x*blockDim.x + threadIdx.x;
+    if (g < ng)
     {
-        scr_min[g] = dbits(__builtin_inf());
-        scr_max[g] = 0ull;
+        *table[g].scr_min = dbits(__builtin_inf());
+        *table[g].scr_max = 0ull;
     }
 }
 
 // col_bath_init + col_comp_zero + col_size_scan (col_bath_init.cu:11-23, col_size_scan.cu:13-25)
-__global__ __launch_bounds__(BLOCK) void k_publish(gdx_due due, const double2 *st, double2 *res,
-                                                  unsigned long long *scr_min, unsigned long long *scr_max)
+__global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w)
 {
     __shared__ unsigned long long red[2][WAVES];
     int gi, s0, s1;
-    if (!due_block(due, PUBLISH_OWNERS, gi, s0, s1)) return;
-    const int g = due.gid[gi];
+    if (!due_block(w, PUBLISH_OWNERS, gi, s0, s1)) return;
+    const gdx_gref G = w.table[w.ref[gi]];
+    const double2 *st = G.st;
+    double2 *res = G.res;
     unsigned long long lo = ~0ull, hi = 0ull;
     for (int s = s0 + threadIdx.x; s < s1; s += BLOCK)
     {
@@ -184,8 +195,8 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_due due, const double2 *s
         for (int w = 1; w < WAVES; ++w) { lo = min(lo, red[0][w]); hi = max(hi, red[1][w]); }
         if (hi != 0ull)
         {
-            atomicMin(scr_min + g, lo);
-            atomicMax(scr_max + g, hi);
+            atomicMin(G.scr_min, lo);
+            atomicMax(G.scr_max, hi);
         }
     }
 }
@@ -203,18 +214,20 @@ struct fold3
 };
 __device__ __forceinline__ fold3 add3(const fold3 &x, const fold3 &y) { return {x.r + y.r, x.a + y.a, x.b + y.b}; }
 
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_due due, const int *nbr, const double2 *st,
-                                                    const double2 *res, const double *measure, const gdx_env *env,
-                                                    const double *hgs, double lambda_0,
-                                                    const unsigned long long *scr_min,
-                                                    const unsigned long long *scr_max, double *lower, double *upper,
-                                                    double4 *cached, gdx_raw_rate *raw)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_wave w)
 {
     __shared__ double accd[COL_BIN_S][4];     // mass, weighted rate, change, second
     __shared__ int acci[COL_BIN_S][3];        // count, owner count, invalid count
     int gi, s0, s1;
-    if (!due_block(due, RATE_OWNERS, gi, s0, s1)) return;
-    const int g = due.gid[gi];
+    if (!due_block(w, RATE_OWNERS, gi, s0, s1)) return;
+    const gdx_gref *GP = w.table + w.ref[gi];
+    const int *nbr = GP->nbr;
+    const double2 *st = GP->st;
+    const double2 *res = GP->res;
+    const double *measure = GP->measure;
+    const gdx_env *env = GP->env;
+    const double *hgs = GP->hgs;
+    const double lambda_0 = GP->lambda_0;
     if (threadIdx.x < COL_BIN_S*4) (&accd[0][0])[threadIdx.x] = 0.0;
     if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
     __syncthreads();
@@ -283,17 +296,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         jj = jn;
     }
     const double R = total.r, J1 = total.a, J2 = total.b;
-    if (own) cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
+    if (own) GP->cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
 
     // moving bounds from this wave's extrema (col_size_bnds.cu:12-21); read after the pair loop so
     // they hold no registers during it
-    double lo = lower[g], up = upper[g];
-    unsigned long long mx = scr_max[g];
+    const gdx_gref &G = *GP;
+    double lo = *G.lower, up = *G.upper;
+    unsigned long long mx = *G.scr_max;
     if (mx != 0ull)
     {
-        lo = 0.5*bitsd(scr_min[g]);
+        lo = 0.5*bitsd(*G.scr_min);
         up = 8.0*bitsd(mx);
-        if (threadIdx.x == 0 && xcd_block() == due.bstart[gi]) { lower[g] = lo; upper[g] = up; }
+        if (threadIdx.x == 0 && xcd_block() == w.bstart[gi]) { *G.lower = lo; *G.upper = up; }
     }
 
     // controller-bin sums, reduced over each distinct bin of the wavefront
@@ -329,13 +343,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     {
         int b = threadIdx.x / 4, f = threadIdx.x % 4;
         double v = accd[b][f];
-        if (v != 0.0) atomicAdd(&(&raw[g*COL_BIN_S + b].mass)[f], v);
+        if (v != 0.0) atomicAdd(&(&G.raw_rate[b].mass)[f], v);
     }
     else if (threadIdx.x >= 64 && threadIdx.x < 64 + COL_BIN_S*3)
     {
         int b = (threadIdx.x - 64) / 3, f = (threadIdx.x - 64) % 3;
         int v = acci[b][f];
-        if (v != 0) atomicAdd(&(&raw[g*COL_BIN_S + b].count)[f], v);
+        if (v != 0) atomicAdd(&(&G.raw_rate[b].count)[f], v);
     }
 }
 
@@ -452,21 +466,23 @@ __device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int 
 
 // screen (col_skip_scan.cu:13-59): one lane per owner. Owners with no event in the bath finish here
 // and enter the audit sums; the rest are appended to their group's queue region [beg, beg + cnt).
-__global__ __launch_bounds__(BLOCK) void k_screen(gdx_due due, const double4 *cached, const double *measure,
-                                                 const double2 *res, const int *slot_pidx, uint64_t *rng,
-                                                 uint64_t seed, const double *lower, const double *upper,
-                                                 unsigned long long *scr_min, unsigned long long *scr_max,
-                                                 gdx_raw_audit *raw, int *queue, int *qcount)
+__global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 {
     __shared__ audit_lds a;
     int gi, s0, s1;
-    if (!due_block(due, CHAIN_OWNERS, gi, s0, s1)) return;
-    const int g = due.gid[gi];
-    const double end = due.dt[gi];
-    const double lo = lower[g], up = upper[g];
+    if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
+    const gdx_gref &G = w.table[w.ref[gi]];
+    const double4 *cached = G.cached;
+    const double *measure = G.measure;
+    const double2 *res = G.res;
+    const int *slot_pidx = G.slot_pidx;
+    uint64_t *rng = G.rng;
+    const uint64_t seed = G.seed;
+    const double end = w.dt[gi];
+    const double lo = *G.lower, up = *G.upper;
     clear_audit(a);
     // this group's extrema scratch was consumed by k_bath_rate; clear it for the group's next refresh
-    if (threadIdx.x == 0 && xcd_block() == due.bstart[gi]) { scr_min[g] = dbits(__builtin_inf()); scr_max[g] = 0ull; }
+    if (threadIdx.x == 0 && xcd_block() == w.bstart[gi]) { *G.scr_min = dbits(__builtin_inf()); *G.scr_max = 0ull; }
     __syncthreads();
 
     const int lane = threadIdx.x % WAVE;
@@ -526,39 +542,56 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_due due, const double4 *ca
     {
         int leader = __builtin_ctzll(ball);
         int base = 0;
-        if (lane == leader) base = atomicAdd(qcount + g, __popcll(ball));
+        if (lane == leader) base = atomicAdd(G.qcount, __popcll(ball));
         base = __builtin_amdgcn_readlane(base, leader);
-        if (chain) queue[due.beg[gi] + base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
+        if (chain) G.queue[w.beg[gi] + base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
     }
     __syncthreads();
-    flush_audit(a, raw + g*COL_BIN_S);
+    flush_audit(a, G.raw_audit);
+}
+
+// exclusive prefix of the due groups' queue lengths (one block)
+__global__ __launch_bounds__(256) void k_queue_prefix(gdx_wave w, int *qprefix)
+{
+    __shared__ int part[256];
+    const int t = threadIdx.x;
+    const int per = (w.n + 255) / 256;
+    int sum = 0;
+    for (int i = t*per; i < min(w.n, (t + 1)*per); ++i) sum += *w.table[w.ref[i]].qcount;
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0)
+    {
+        int acc = 0;
+        for (int k = 0; k < 256; ++k) { int v = part[k]; part[k] = acc; acc += v; }
+    }
+    __syncthreads();
+    int acc = part[t];
+    for (int i = t*per; i < min(w.n, (t + 1)*per); ++i)
+    {
+        qprefix[i] = acc;
+        acc += *w.table[w.ref[i]].qcount;
+    }
+    if (t == 255) qprefix[w.n] = acc;
 }
 
 // event chains (col_chain_run.cu:127-382): a persistent grid pulls the queued owners from one global
 // counter. Each K-lane segment of a wavefront runs one owner to its bath end, then takes the next;
 // the chain sums go to the owner's queue position for the audit pass.
-__global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount, const int *queue, const int *nbr,
-                                                double2 *st, const double2 *res, const double4 *cached,
-                                                const double *measure, const gdx_env *env, const double *hgs,
-                                                const int *slot_pidx, uint64_t *rng, uint64_t seed, double lambda_0,
-                                                double4 *qsum, int *qevents, int *err,
-                                                unsigned long long *counters, int *work, gdx_event_work *evw)
+constexpr int CHAIN_PREFIX_LDS = 4096;     // due groups whose queue prefix a chain block keeps in LDS
+
+__global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
 {
-    __shared__ int prefix[WAVES][65];
     __shared__ double walk[WAVES][NSEG][N_K];
+    __shared__ int qp[CHAIN_PREFIX_LDS + 1];
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    // prefix of the due groups' queue lengths, one lane per due group
-    int inc = lane < due.n ? qcount[due.gid[lane]] : 0;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1)
-    {
-        int t = __shfl_up(inc, o, WAVE);
-        if (lane >= o) inc += t;
-    }
-    prefix[wave][lane + 1] = inc;
-    if (lane == 0) prefix[wave][0] = 0;
-    const int total = __shfl(inc, WAVE - 1, WAVE);
-    __builtin_amdgcn_wave_barrier();
+    // the queue prefix of the due groups, searched at every refill: LDS when it fits
+    const bool in_lds = w.n <= CHAIN_PREFIX_LDS;
+    if (in_lds)
+        for (int i = threadIdx.x; i <= w.n; i += BLOCK) qp[i] = w.qprefix[i];
+    __syncthreads();
+    const int *qprefix = in_lds ? qp : w.qprefix;
+    const int total = qprefix[w.n];
 
     // lane kl of a segment holds row position kl, i.e. slot bitrev(kl)
     const int seg = lane / SEG, kl = lane % SEG, segbase = seg*SEG, slot_k = bitrev_slot(kl);
@@ -566,15 +599,34 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
     const unsigned long long seg_lead = (N_K == 64) ? 1ull : (N_K == 32) ? 0x0000000100000001ull
                                                                          : 0x0001000100010001ull;
     unsigned long long n_events = 0, n_chain = 0, n_pairs = 0, n_blocks = 0;
-    int err_code = 0, longest = 0;
+    int longest = 0;
     bool more = true, active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
     double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
-    uint64_t ctr = 0;
+    uint64_t ctr = 0, seed = 0;
     uint32_t pidx = 0;
+    double lambda_0 = 0.0;
+    const gdx_gref *G = nullptr;              // the current owner's group
+    unsigned long long *ctr_of = nullptr;     // the swarm whose counters the accumulators hold
+    unsigned long long *wave_of = nullptr;
     owner_terms t{};
     cat_moments cm{0.0, 0.0, 0.0};
     partner q{0.0, 0.0, 0.0, false};
+    // counters are flushed when a segment moves to an owner of another swarm, and at the end
+    auto flush = [&]() {
+        if (kl == 0 && ctr_of)
+        {
+            if (n_events) atomicAdd(ctr_of + 0, n_events);
+            if (n_chain) atomicAdd(ctr_of + 1, n_chain);
+            if (n_pairs) atomicAdd(ctr_of + 2, n_pairs);
+            if (longest) atomicMax(ctr_of + 3, (unsigned long long)longest);
+            if (longest) atomicMax(wave_of + 0, (unsigned long long)longest);
+            if (n_blocks) atomicAdd(wave_of + 1, n_blocks);
+            if (n_chain) atomicAdd(wave_of + 2, n_chain);
+        }
+        n_events = n_chain = n_pairs = n_blocks = 0;
+        longest = 0;
+    };
     // the wavefront's current run of queue positions [cb, ce), taken CHAIN_CHUNK at a time from the
     // global counter so that the counter sees one atomic per CHAIN_CHUNK owners
     int cb = 0, ce = 0;
@@ -597,23 +649,34 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
             if (!active && rank < n)
             {
                 int v = cb + rank;
-                gi = 0;
-                while (prefix[wave][gi + 1] <= v) ++gi;
-                pos = due.beg[gi] + (v - prefix[wave][gi]);
-                s = queue[pos];
-                end = due.dt[gi];
-                double2 cur = st[s];
+                // due group of queue position v: last entry with qprefix <= v
+                int lo = 0, hi = w.n - 1;
+                while (lo < hi)
+                {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (qprefix[mid] <= v) lo = mid;
+                    else hi = mid - 1;
+                }
+                gi = lo;
+                G = w.table + w.ref[gi];
+                if (G->counters != ctr_of) { flush(); ctr_of = G->counters; wave_of = G->wave_ctr; }
+                pos = w.beg[gi] + (v - qprefix[gi]);
+                s = G->queue[pos];
+                end = w.dt[gi];
+                seed = G->seed;
+                lambda_0 = G->lambda_0;
+                double2 cur = G->st[s];
                 numr_i = cur.y;
-                ctr = rng[s];
-                pidx = uint32_t(slot_pidx[s]);
+                ctr = G->rng[s];
+                pidx = uint32_t(G->slot_pidx[s]);
                 double hg_i = 0.0;
                 gdx_env e{};
-                if constexpr (COAG_KERNEL == 3) e = env[s];
-                if constexpr (PARTNER_HG) hg_i = hgs[s];
-                t = make_owner(e, cur.x, measure[s], hg_i);
+                if constexpr (COAG_KERNEL == 3) e = G->env[s];
+                if constexpr (PARTNER_HG) hg_i = G->hgs[s];
+                t = make_owner(e, cur.x, G->measure[s], hg_i);
                 cm = catastrophic_moments(cur.x);
                 // the partners are the published states: fixed for the whole chain
-                q = load_partner(nbr[(long long)s*N_K + kl], res, hgs);
+                q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
                 time = hazard = j1 = j2 = jmax = 0.0;
                 events = 0;
                 active = true;
@@ -700,12 +763,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
                         bool high_speed = vrel >= V_FRAG;
                         double sample = high_speed ? col_uniform(pidx, seed, ctr) : 0.0;
                         double size_new = sample_outcome(size_i, size_j, high_speed, sample);
-                        if (evw && kl == 0)
+                        if (kl == 0 && G->evw)
                         {
                             double log_mass;
                             int cat = outcome_category(size_i, size_j, high_speed, sample, size_new, log_mass);
-                            evw[s].count[cat] += 1ull;
-                            evw[s].log_mass[cat] += log_mass;
+                            G->evw[s].count[cat] += 1ull;
+                            G->evw[s].log_mass[cat] += log_mass;
                         }
                         numr_i = mass_before / (size_new*size_new*size_new);
                         time = event_time;
@@ -726,11 +789,11 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
         {
             if (kl == 0)
             {
-                if (events > 0) st[s] = make_double2(t.size_i, numr_i);
-                rng[s] = ctr;
-                err_code = max(err_code, code);
-                qsum[pos] = make_double4(hazard, j1, j2, jmax);
-                qevents[pos] = code ? -1 : events;
+                if (events > 0) G->st[s] = make_double2(t.size_i, numr_i);
+                G->rng[s] = ctr;
+                if (code) atomicMax(G->err, code);
+                G->qsum[pos] = make_double4(hazard, j1, j2, jmax);
+                G->qevents[pos] = code ? -1 : events;
                 longest = max(longest, events);
                 // the reference relaunches an owner after every COL_EVENT_CAP accepted events
                 n_blocks += (unsigned long long)(events / COL_EVENT_CAP + 1);
@@ -740,42 +803,24 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_due due, const int *qcount,
             active = false;
         }
     }
-    if (err_code) atomicMax(err, err_code);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-    {
-        n_events += __shfl_xor(n_events, o, WAVE);
-        n_chain += __shfl_xor(n_chain, o, WAVE);
-        n_pairs += __shfl_xor(n_pairs, o, WAVE);
-        n_blocks += __shfl_xor(n_blocks, o, WAVE);
-        longest = max(longest, __shfl_xor(longest, o, WAVE));
-    }
-    if (lane == 0)
-    {
-        if (longest) atomicMax(&counters[3], (unsigned long long)longest);
-        if (longest) atomicMax(&counters[4], (unsigned long long)longest);
-        if (n_blocks) atomicAdd(&counters[5], n_blocks);
-        if (n_chain) atomicAdd(&counters[6], n_chain);
-        if (n_events) atomicAdd(&counters[0], n_events);
-        if (n_chain) atomicAdd(&counters[1], n_chain);
-        if (n_pairs) atomicAdd(&counters[2], n_pairs);
-    }
+    flush();
 }
 
 // audit of the chained owners, one lane per queue position of each due group (col_audit_bin.cu:13-60)
-__global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_due due, const int *qcount, const int *queue,
-                                                      const double2 *st, const double2 *res, const double4 *cached,
-                                                      const double4 *qsum, const int *qevents, const double *lower,
-                                                      const double *upper, gdx_raw_audit *raw)
+__global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_wave w)
 {
     __shared__ audit_lds a;
     int gi, s0, s1;
-    if (!due_block(due, CHAIN_OWNERS, gi, s0, s1)) return;
-    const int g = due.gid[gi];
-    const int q1 = due.beg[gi] + qcount[g];
+    if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
+    const gdx_gref &G = w.table[w.ref[gi]];
+    const int q1 = w.beg[gi] + *G.qcount;
     if (s0 >= q1) return;
-    const double end = due.dt[gi];
-    const double lo = lower[g], up = upper[g];
+    const double end = w.dt[gi];
+    const double lo = *G.lower, up = *G.upper;
+    const int *queue = G.queue;
+    const double2 *st = G.st, *res = G.res;
+    const double4 *cached = G.cached, *qsum = G.qsum;
+    const int *qevents = G.qevents;
     clear_audit(a);
     __syncthreads();
     const int pos = s0 + int(threadIdx.x);
@@ -805,7 +850,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_due due, const int *q
     double growth = fabs(log(size_end / old.x));
     audit_wave<true>(a, take, bs, be, weight, probability, sum.x, sum.y, sum.z, sum.w, ev, growth, bad);
     __syncthreads();
-    flush_audit(a, raw + g*COL_BIN_S);
+    flush_audit(a, G.raw_audit);
 }
 
 __global__ __launch_bounds__(256) void k_cached_to_particles(const double4 *cached, const int *slot_pidx, int n_active,
@@ -818,43 +863,26 @@ __global__ __launch_bounds__(256) void k_cached_to_particles(const double4 *cach
     o[0] = c.x; o[1] = c.y; o[2] = c.z; o[3] = c.w;
 }
 
-inline int blocks_for_due(gdx_due &due, int per_block)
-{
-    int b = 0;
-    for (int i = 0; i < due.n; ++i)
-    {
-        due.bstart[i] = b;
-        b += (due.cnt[i] + per_block - 1) / per_block;
-    }
-    due.bstart[due.n] = b;
-    return b;
-}
-
 }  // namespace
 
-void launch_reset_scratch(context &c)
+void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng)
 {
-    k_reset_scratch<<<1, 64, 0, c.stream>>>(c.scr_min.p, c.scr_max.p);
+    if (ng <= 0) return;
+    k_reset_scratch<<<(ng + 255) / 256, 256, 0, st>>>(table, ng);
     hip_check(hipGetLastError(), "launch reset scratch");
 }
 
-void launch_publish(context &c, const gdx_due &due_in)
+void launch_publish(hipStream_t st, const gdx_wave &w, int blocks)
 {
-    gdx_due due = due_in;
-    int nb = blocks_for_due(due, PUBLISH_OWNERS);
-    if (!nb) return;
-    k_publish<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.st.p, c.res.p, c.scr_min.p, c.scr_max.p);
+    if (!blocks) return;
+    k_publish<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch publish");
 }
 
-void launch_bath_rate(context &c, const gdx_due &due_in, double lambda_0)
+void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks)
 {
-    gdx_due due = due_in;
-    int nb = blocks_for_due(due, RATE_OWNERS);
-    if (!nb) return;
-    k_bath_rate<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.nbr.p, c.st.p, c.res.p, c.measure.p, c.env.p, c.hgs.p, lambda_0,
-                                           c.scr_min.p, c.scr_max.p, c.lower.p, c.upper.p, c.cached.p,
-                                           c.raw_rate.p);
+    if (!blocks) return;
+    k_bath_rate<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch bath rate");
 }
 
@@ -881,51 +909,43 @@ __global__ __launch_bounds__(256) void k_event_sum(const gdx_event_work *w, int 
     if (t == 0) { sum->count[k] = cnt[0]; sum->log_mass[k] = lm[0]; }
 }
 
-void launch_event_sum(context &c)
+void launch_event_sum(context &c, hipStream_t st)
 {
-    k_event_sum<<<GDX_EVENT_CATEGORIES, 256, 0, c.stream>>>(c.evw.p, c.n_active, c.ev_sum.p);
+    k_event_sum<<<GDX_EVENT_CATEGORIES, 256, 0, st>>>(c.evw.p, c.n_active, c.ev_sum.p);
     hip_check(hipGetLastError(), "launch event sum");
 }
 
-void launch_screen(context &c, const gdx_due &due_in)
+void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix)
 {
-    gdx_due due = due_in;
-    int nb = blocks_for_due(due, CHAIN_OWNERS);
-    hip_check(hipMemsetAsync(c.qcount.p, 0, sizeof(int)*(LOCAL_GROUPS + 1), c.stream), "zero queue counters");
-    if (!nb) return;
-    k_screen<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.cached.p, c.measure.p, c.res.p, c.slot_pidx.p, c.rng.p,
-                                                  c.seed, c.lower.p, c.upper.p, c.scr_min.p, c.scr_max.p,
-                                                  c.raw_audit.p, c.queue.p, c.qcount.p);
+    if (!blocks) return;
+    k_screen<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch screen");
+    k_queue_prefix<<<1, 256, 0, st>>>(w, qprefix);
+    hip_check(hipGetLastError(), "launch queue prefix");
 }
 
-void launch_chain(context &c, const gdx_due &due, double lambda_0)
+int chain_resident_blocks()
 {
-    if (!due.total) return;
-    // a persistent grid of as many blocks as stay resident together, fewer for small waves
-    if (!c.chain_blocks)
-    {
-        int per_cu = 0, cus = 0, dev = 0;
-        hip_check(hipGetDevice(&dev), "get device");
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain, BLOCK, 0), "chain occupancy");
-        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
-        c.chain_blocks = std::max(1, per_cu*cus);
-    }
-    int grid = std::max(1, std::min(c.chain_blocks, (due.total + 4*NSEG - 1) / (4*NSEG)));
-    k_chain<<<grid, BLOCK, 0, c.stream>>>(due, c.qcount.p, c.queue.p, c.nbr.p, c.st.p, c.res.p, c.cached.p,
-                                         c.measure.p, c.env.p, c.hgs.p, c.slot_pidx.p, c.rng.p, c.seed, lambda_0,
-                                         c.qsum.p, c.qevents.p, c.err.p, c.counters.p, c.qcount.p + LOCAL_GROUPS,
-                                         c.diag ? c.evw.p : nullptr);
+    int per_cu = 0, cus = 0, dev = 0;
+    hip_check(hipGetDevice(&dev), "get device");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain, BLOCK, 0), "chain occupancy");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    return std::max(1, per_cu*cus);
+}
+
+// a persistent grid of at most `resident` blocks, fewer for small waves
+void launch_chain(hipStream_t st, const gdx_wave &w, int *work, int resident)
+{
+    if (!w.total) return;
+    int grid = std::max(1, std::min(resident, (w.total + 4*NSEG - 1) / (4*NSEG)));
+    k_chain<<<grid, BLOCK, 0, st>>>(w, work);
     hip_check(hipGetLastError(), "launch chain");
 }
 
-void launch_chain_audit(context &c, const gdx_due &due_in)
+void launch_chain_audit(hipStream_t st, const gdx_wave &w, int blocks)
 {
-    gdx_due due = due_in;
-    int nb = blocks_for_due(due, CHAIN_OWNERS);
-    if (!nb) return;
-    k_chain_audit<<<xcd_grid(nb), BLOCK, 0, c.stream>>>(due, c.qcount.p, c.queue.p, c.st.p, c.res.p, c.cached.p,
-                                                       c.qsum.p, c.qevents.p, c.lower.p, c.upper.p, c.raw_audit.p);
+    if (!blocks) return;
+    k_chain_audit<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch chain audit");
 }
 

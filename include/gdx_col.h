@@ -113,6 +113,12 @@ int gdx_col_store_state(gdx_col_handle h);
 
 /* one operator call over `duration`: evolve_local_collisions (inc/swarm/_col_chain.cuh:108-539) */
 int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);
+/* one operator call on each of `count` independent swarms (realizations, each with its own handle,
+ * geometry, state, controller and random key): the swarms' due groups share every kernel launch and
+ * host synchronisation; results equal separate gdx_col_operator calls. No reference counterpart (the
+ * reference runs one swarm per process); for ensembles of small realizations. Launches go to the
+ * first handle's stream; out holds `count` entries. */
+int gdx_col_operator_batch(const gdx_col_handle *handles, int count, double duration, gdx_col_stats *out);
 /* controller memory reset at output boundaries (src/swarm/swarm_runtime.cu:491-496) */
 int gdx_col_reset_controller(gdx_col_handle h);
 int gdx_col_synchronize(gdx_col_handle h);
