@@ -18,7 +18,9 @@ DEFER_WRITEBACK = 1
 PROFILE = 2
 DIAGNOSTICS = 4
 SEARCH_MORTON, SEARCH_KDTREE = 0, 1
-KERNELS = ("publish", "bath_rate", "screen", "chain", "chain_audit")
+KERNELS = ("publish", "bath_rate", "screen", "chain", "chain_audit", "majorant")
+MAJORANT_FEATURES = 8
+MAJORANT_PARAMS = 4801
 
 
 class GdxError(RuntimeError):
@@ -40,11 +42,17 @@ class Config(C.Structure):
                 ("search", C.c_int)]
 
 
+class TrainConfig(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double), ("beta1", C.c_double),
+                ("beta2", C.c_double), ("eps", C.c_double), ("seed", C.c_ulonglong)]
+
+
 class Stats(C.Structure):
     _fields_ = [("waves", C.c_int), ("baths", C.c_int), ("owner_updates", C.c_longlong),
                 ("active_owners", C.c_longlong), ("events", C.c_longlong), ("chain_owners", C.c_longlong),
                 ("pair_evals", C.c_longlong), ("dt_col", C.c_double), ("clock_dyn", C.c_double),
-                ("max_chain_events", C.c_longlong)]
+                ("max_chain_events", C.c_longlong), ("thinned", C.c_longlong), ("candidates", C.c_longlong),
+                ("accepted", C.c_longlong), ("exact_checks", C.c_longlong), ("violations", C.c_longlong)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -93,8 +101,17 @@ _SIGS = {
     "gdx_morton_node_count": [P, C.POINTER(I)],
     "gdx_morton_get": [P, P, P, P, P, P, P, P, P],
     "gdx_morton_knn": [P, P, I, I, P, P],
+    "gdx_col_majorant_create": [C.c_ulonglong, C.POINTER(P)],
+    "gdx_col_majorant_destroy": [P],
+    "gdx_col_majorant_get_params": [P, P, P],
+    "gdx_col_majorant_set_params": [P, P, P],
+    "gdx_col_majorant_infer": [P, P, C.c_longlong, P],
+    "gdx_col_majorant_grad": [P, P, P, I, P, C.POINTER(D)],
+    "gdx_col_majorant_train": [P, P, P, C.c_longlong, C.POINTER(TrainConfig), P],
+    "gdx_col_majorant_sample": [P, P, P, P],
+    "gdx_col_use_majorant": [P, P, D],
 }
-EXPORTS = tuple(_SIGS) + ("gdx_col_last_error",)
+EXPORTS = tuple(_SIGS) + ("gdx_col_last_error", "gdx_col_majorant_last_error")
 
 _cache: dict[str, C.CDLL] = {}
 
@@ -117,6 +134,8 @@ def load(model: str) -> C.CDLL:
         fn.restype = C.c_int
     lib.gdx_col_last_error.argtypes = [P]
     lib.gdx_col_last_error.restype = C.c_char_p
+    lib.gdx_col_majorant_last_error.argtypes = [P]
+    lib.gdx_col_majorant_last_error.restype = C.c_char_p
     _cache[model] = lib
     return lib
 

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 MOD = PKG / "mod"
 LIB = PKG / "lib"
 BUILD = PKG / "_build"
-SOURCES = ["geometry.hip", "kdtree.hip", "operator.hip", "capi.cpp"]
+SOURCES = ["geometry.hip", "kdtree.hip", "operator.hip", "majorant.hip", "capi.cpp"]
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = "gfx950"
 COMMON = [

@@ -194,6 +194,23 @@ class CollisionOperator:
         self._call(self.lib.gdx_col_get_graph, _np_ptr(out))
         return out
 
+    # ---- learned majorant ----
+    def majorant_sample(self):
+        """Features (n x 8, float32), partner scale S and exact bath-start rate R per particle at the
+        current state (zero rows for inactive particles), as device tensors."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        feats = torch.empty((self.n, _lib.MAJORANT_FEATURES), dtype=torch.float32, device=dev)
+        scale = torch.empty(self.n, dtype=torch.float64, device=dev)
+        rate = torch.empty(self.n, dtype=torch.float64, device=dev)
+        self._call(self.lib.gdx_col_majorant_sample, _ptr(feats), _ptr(scale), _ptr(rate))
+        return feats, scale, rate
+
+    def use_majorant(self, majorant, safety: float = 1.2) -> None:
+        """Draw event times by thinning against the majorant (None: the plain method)."""
+        self._call(self.lib.gdx_col_use_majorant, majorant.h if majorant is not None else None, float(safety))
+        self._majorant = majorant
+
 
 def evolve_batch(ops: list, duration: float) -> list[dict]:
     """One operator call on each of several CollisionOperator instances of the same model (independent
@@ -207,6 +224,113 @@ def evolve_batch(ops: list, duration: float) -> list[dict]:
     stats = (_lib.Stats*len(ops))()
     _lib.check(lib, ops[0].h, lib.gdx_col_operator_batch(handles, len(ops), float(duration), stats))
     return [s.as_dict() for s in stats]
+
+
+class Majorant:
+    """Learned rate majorant (no reference counterpart): an 8-64-64-1 BF16 perceptron trained and run by
+    the library's MFMA kernels. Feature rows and targets are device tensors (torch supplies memory)."""
+
+    def __init__(self, model: str, seed: int = 0):
+        self.model = model
+        self.lib = _lib.load(model)
+        h = C.c_void_p()
+        rc = self.lib.gdx_col_majorant_create(int(seed) & (2**64 - 1), C.byref(h))
+        self._check(rc, None)
+        self.h = h
+
+    def _check(self, rc: int, h) -> None:
+        if rc != 0:
+            msg = self.lib.gdx_col_majorant_last_error(h)
+            raise _lib.GdxError(rc, msg.decode() if msg else "")
+
+    def _call(self, fn, *args):
+        self._check(fn(self.h, *args), self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.gdx_col_majorant_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> tuple[np.ndarray, np.ndarray]:
+        p = np.empty(_lib.MAJORANT_PARAMS, np.float32)
+        norm = np.empty(2*_lib.MAJORANT_FEATURES, np.float32)
+        self._call(self.lib.gdx_col_majorant_get_params, _np_ptr(p), _np_ptr(norm))
+        return p, norm
+
+    def set_params(self, params=None, norm=None) -> None:
+        pa = None if params is None else np.ascontiguousarray(params, np.float32)
+        na = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        self._call(self.lib.gdx_col_majorant_set_params, None if pa is None else _np_ptr(pa),
+                   None if na is None else _np_ptr(na))
+
+    def fit_normalization(self, features) -> np.ndarray:
+        """Set the input normalization to the per-feature mean and standard deviation (1 where constant)."""
+        x = features.detach().double().cpu().numpy() if hasattr(features, "detach") else np.asarray(features)
+        mu = x.mean(axis=0)
+        sd = x.std(axis=0)
+        sd = np.where(sd > 1e-12, sd, 1.0)
+        norm = np.concatenate([mu, sd]).astype(np.float32)
+        self.set_params(None, norm)
+        return norm
+
+    def fit_output(self, log_target) -> float:
+        """Start the output bias at the mean log target, softplus(b3) = exp(mean), so that training
+        begins at the right scale; returns b3."""
+        import math
+        m = float(log_target.double().mean()) if hasattr(log_target, "double") else float(np.mean(log_target))
+        b3 = m if m < -20.0 else (math.log(math.expm1(math.exp(m))) if m < 20.0 else math.exp(m))
+        p, _ = self.params()
+        p[-1] = b3
+        self.set_params(p, None)
+        return b3
+
+    def calibrate(self, features, log_target, quantile: float = 0.999, floor: float = 1.2) -> float:
+        """Safety factor for the bound: the `quantile` of exact / predicted over the samples, at least
+        `floor` (the factor the bound is used with in gdx_col_use_majorant)."""
+        import torch
+        y = self.infer(features).double()
+        r = torch.exp(log_target.double())/y
+        r = r[torch.isfinite(r)]
+        if r.numel() == 0:
+            return floor
+        q = float(torch.sort(r).values[min(r.numel() - 1, int(quantile*r.numel()))])
+        return max(floor, q)
+
+    def fit(self, features, log_target, epochs: int = 20, batch: int = 8192, lr: float = 1e-3, seed: int = 0,
+            quantile: float = 0.999, floor: float = 1.2) -> tuple[np.ndarray, float]:
+        """Normalization, output bias, Adam training and the calibrated safety factor in one call."""
+        self.fit_normalization(features)
+        self.fit_output(log_target)
+        losses = self.train(features, log_target, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        return losses, self.calibrate(features, log_target, quantile, floor)
+
+    def infer(self, features):
+        import torch
+        n = features.shape[0]
+        out = torch.empty(n, dtype=torch.float32, device=features.device)
+        self._call(self.lib.gdx_col_majorant_infer, _ptr(features.contiguous()), n, _ptr(out))
+        return out
+
+    def grad(self, features, log_target) -> tuple[np.ndarray, float]:
+        g = np.empty(_lib.MAJORANT_PARAMS, np.float32)
+        loss = C.c_double()
+        self._call(self.lib.gdx_col_majorant_grad, _ptr(features.contiguous()), _ptr(log_target.contiguous()),
+                   int(features.shape[0]), _np_ptr(g), C.byref(loss))
+        return g, loss.value
+
+    def train(self, features, log_target, epochs: int = 20, batch: int = 8192, lr: float = 1e-3,
+              seed: int = 0) -> np.ndarray:
+        cfg = _lib.TrainConfig(int(epochs), int(batch), float(lr), 0.0, 0.0, 0.0, int(seed) & (2**64 - 1))
+        losses = np.zeros(max(1, epochs), np.float32)
+        self._call(self.lib.gdx_col_majorant_train, _ptr(features.contiguous()), _ptr(log_target.contiguous()),
+                   int(features.shape[0]), C.byref(cfg), _np_ptr(losses))
+        return losses[:epochs]
 
 
 class MortonIndex:

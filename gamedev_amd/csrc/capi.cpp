@@ -96,11 +96,15 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.scr_max = c.scr_max.p + g;
     r.lower = c.lower.p + g;
     r.upper = c.upper.p + g;
-    r.counters = rb.counters.p + size_t(run)*4;
-    r.wave_ctr = rb.wave_ctr.p + size_t(run)*3;
+    r.counters = rb.counters.p + size_t(run)*GDX_CTRS;
+    r.wave_ctr = rb.wave_ctr.p + size_t(run)*GDX_WAVE_CTRS;
     r.err = rb.err.p + run;
     r.lambda_0 = c.n / static_cast<double>(N_K) / c.total_mass;
     r.seed = c.seed;
+    r.mlp = c.mlp;
+    r.safety = c.safety;
+    r.lam = c.lam.p;
+    r.qstart = c.qstart.p;
     return r;
 }
 
@@ -108,43 +112,64 @@ struct phase_entry
 {
     int ref, beg, cnt;
     double dt;
+    int sub;              // learned swarm: exact-subsample phase of this bath; -1 plain
 };
 
-// due entries of one phase in device memory; the launch shapes for 1024 and 256 owners per block
+// due entries of one phase in device memory; launch shapes: 1024 and 256 owners per block, the
+// exactly rated owners (all, or a learned subsample) and the learned groups' owners, 256 per block
 struct phase_wave
 {
-    gdx_wave w1024, w256;
-    int blocks1024 = 0, blocks256 = 0;
+    gdx_wave w1024, w256, wsub, wmaj;
+    int blocks1024 = 0, blocks256 = 0, blocks_sub = 0, blocks_maj = 0;
 };
 
 phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipStream_t st)
 {
     const int n = int(e.size());
     rb.reserve(n);
-    int *ref = rb.h_int, *beg = ref + n, *cnt = beg + n, *b1024 = cnt + n, *b256 = b1024 + n + 1;
-    int total = 0, x = 0, y = 0;
+    int *ref = rb.h_int, *beg = ref + n, *cnt = beg + n, *sub = cnt + n, *b1024 = sub + n, *b256 = b1024 + n + 1;
+    int *bsub = b256 + n + 1, *bmaj = bsub + n + 1;
+    int total = 0, x = 0, y = 0, z = 0, u = 0, learned = 0;
     for (int i = 0; i < n; ++i)
     {
         ref[i] = e[i].ref;
         beg[i] = e[i].beg;
         cnt[i] = e[i].cnt;
+        sub[i] = e[i].sub;
         rb.h_dt[i] = e[i].dt;
         b1024[i] = x;
         b256[i] = y;
+        bsub[i] = z;
+        bmaj[i] = u;
         x += (e[i].cnt + 1023) / 1024;
         y += (e[i].cnt + 255) / 256;
+        // the same count as rated_count on the device
+        int rated = e[i].sub < 0 ? e[i].cnt : std::max(0, (e[i].cnt - e[i].sub + THIN_STRIDE - 1) / THIN_STRIDE);
+        z += (rated + 255) / 256;
+        u += e[i].sub < 0 ? 0 : (e[i].cnt + 255) / 256;
+        learned |= e[i].sub >= 0;
         total += e[i].cnt;
     }
     b1024[n] = x;
     b256[n] = y;
-    hip_check(hipMemcpyAsync(rb.d_int.p, rb.h_int, sizeof(int)*(5*size_t(n) + 2), hipMemcpyHostToDevice, st), "due list");
+    bsub[n] = z;
+    bmaj[n] = u;
+    hip_check(hipMemcpyAsync(rb.d_int.p, rb.h_int, sizeof(int)*(run_buffers::DUE_INTS*size_t(n) + run_buffers::DUE_EXTRA),
+                             hipMemcpyHostToDevice, st), "due list");
     hip_check(hipMemcpyAsync(rb.d_dt.p, rb.h_dt, sizeof(double)*std::max(1, n), hipMemcpyHostToDevice, st), "due list");
     phase_wave pw;
     const int *d = rb.d_int.p;
-    pw.w1024 = gdx_wave{n, total, rb.table.p, d, d + n, d + 2*n, d + 3*n, rb.d_dt.p, rb.qprefix.p};
-    pw.w256 = gdx_wave{n, total, rb.table.p, d, d + n, d + 2*n, d + 4*n + 1, rb.d_dt.p, rb.qprefix.p};
+    auto wave = [&](const int *bstart) {
+        return gdx_wave{n, total, rb.table.p, d, d + n, d + 2*n, bstart, rb.d_dt.p, rb.qprefix.p, d + 3*n, learned};
+    };
+    pw.w1024 = wave(d + (b1024 - ref));
+    pw.w256 = wave(d + (b256 - ref));
+    pw.wsub = wave(d + (bsub - ref));
+    pw.wmaj = wave(d + (bmaj - ref));
     pw.blocks1024 = x;
     pw.blocks256 = y;
+    pw.blocks_sub = z;
+    pw.blocks_maj = u;
     return pw;
 }
 
@@ -238,6 +263,10 @@ struct op_run
     std::vector<double> published;
     int operator_index = 0, bath_index = 0;
     wall::time_point begin, chain_begin;
+    // learned majorant: in use for this call (switched off for the rest of the call when the exact
+    // checks of a wave find the bound too often below the exact rate); subsample phase per group
+    bool learned = false, rate_learned = false;
+    std::vector<int> sub_phase;
 
     op_run(context &ctx, diag_state *d, double h)
         : c(ctx), dg(d), duration(h), diag(ctx.diag && d),
@@ -248,6 +277,8 @@ struct op_run
         if (!(duration > 0) || !std::isfinite(duration))
             throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
         begin = wall::now();
+        learned = c.mlp != nullptr;
+        sub_phase.assign(LOCAL_GROUPS, 0);
         stats.active_owners = c.n_active;
         dt_col = duration;
         groups.resize(LOCAL_GROUPS);
@@ -260,16 +291,18 @@ struct op_run
         raw_rate = rb.h_rate.data() + size_t(tbase)*COL_BIN_S;
         raw_audit = rb.h_audit.data() + size_t(tbase)*COL_BIN_S;
         err = rb.h_err.data() + index;
-        wave_ctr = rb.h_wave.data() + size_t(index)*3;
+        wave_ctr = rb.h_wave.data() + size_t(index)*GDX_WAVE_CTRS;
         if (diag)
             hip_check(hipMemsetAsync(c.evw.p, 0, sizeof(gdx_event_work)*std::max(1, c.n_active), st), "zero event work");
     }
 
-    void entries(std::vector<phase_entry> &e, bool chain) const
+    void entries(std::vector<phase_entry> &e, bool chain)
     {
+        if (!chain) rate_learned = learned;
         for (int g : groups)
             if (c.group_cnt[g] > 0)
-                e.push_back(phase_entry{tbase + g, c.group_beg[g], c.group_cnt[g], chain ? steps[g] : 0.0});
+                e.push_back(phase_entry{tbase + g, c.group_beg[g], c.group_cnt[g], chain ? steps[g] : 0.0,
+                                        learned ? sub_phase[g] % THIN_STRIDE : -1});
     }
     long long due_owners() const
     {
@@ -314,8 +347,16 @@ struct op_run
     // merged rate bins of the rated groups (_col_chain.cuh:205-243)
     void after_rate()
     {
-        stats.pair_evals += due_owners()*N_K;
-        for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count;
+        for (int r = 0; r < RAW_BINS; ++r) stats.pair_evals += (long long)raw_rate[r].count*N_K;
+        // the bound failed on more than 1% of the exact subsample: the swarm reverts to the plain
+        // method for the rest of the call, starting with a full rate pass of the same groups
+        if (rate_learned && wave_ctr[GDX_WAVE_RATE_VIOLATIONS]*100 > wave_ctr[GDX_WAVE_RATE_CHECKS])
+        {
+            learned = false;
+            return;
+        }
+        // a learned rate pass rated one owner in THIN_STRIDE: bins merge on the population estimate
+        for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count*(rate_learned ? THIN_STRIDE : 1);
         merged = build_binmap(counts, binmap);
         std::fill(ratebin.begin(), ratebin.begin() + merged, rate_bin{});
         for (int r = 0; r < RAW_BINS; ++r)
@@ -469,6 +510,13 @@ struct op_run
         stats.owner_updates += due_owners();
         dt_col = duration;
         for (int g : groups) dt_col = std::min(dt_col, steps[g]);
+        if (learned)
+        {
+            for (int g : groups) sub_phase[g]++;
+            // more than 1% of at least 256 exact checks above the bound: the plain method takes over
+            const unsigned long long checks = wave_ctr[GDX_WAVE_CHECKS], viol = wave_ctr[GDX_WAVE_VIOLATIONS];
+            if (checks >= 256 && viol*100 > checks) learned = false;
+        }
         sched->advance(groups);
         next_groups();
     }
@@ -487,10 +535,15 @@ struct op_run
             store_state(c);
             hip_check(hipStreamSynchronize(c.stream), "write-back");
         }
-        stats.events = (long long)counters[0];
-        stats.chain_owners = (long long)counters[1];
-        stats.pair_evals += (long long)counters[2];
-        stats.max_chain_events = (long long)counters[3];
+        stats.events = (long long)counters[GDX_CTR_EVENTS];
+        stats.chain_owners = (long long)counters[GDX_CTR_CHAINED];
+        stats.pair_evals += (long long)counters[GDX_CTR_PAIRS];
+        stats.max_chain_events = (long long)counters[GDX_CTR_LONGEST];
+        stats.thinned = (long long)counters[GDX_CTR_THINNED];
+        stats.candidates = (long long)counters[GDX_CTR_CANDIDATES];
+        stats.accepted = (long long)counters[GDX_CTR_ACCEPTED];
+        stats.exact_checks = (long long)counters[GDX_CTR_CHECKS];
+        stats.violations = (long long)counters[GDX_CTR_VIOLATIONS];
         stats.dt_col = dt_col;
         stats.clock_dyn = duration;
         if (diag && dg->log.is_open())
@@ -539,7 +592,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
     rb.reserve(ng);
     hip_check(hipMemcpyAsync(rb.table.p, rb.h_table.data(), sizeof(gdx_gref)*ng, hipMemcpyHostToDevice, st),
               "group table");
-    hip_check(hipMemsetAsync(rb.counters.p, 0, sizeof(unsigned long long)*4*nr, st), "zero counters");
+    hip_check(hipMemsetAsync(rb.counters.p, 0, sizeof(unsigned long long)*GDX_CTRS*nr, st), "zero counters");
     if (!rb.resident) rb.resident = chain_resident_blocks();
     for (op_run *r : runs) r->start(st, rb);
     launch_reset_scratch(st, rb.table.p, ng);
@@ -557,10 +610,15 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
         {
             work = true;
             hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st), "zero rate bins");
+            hip_check(hipMemsetAsync(rb.wave_ctr.p, 0, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st),
+                      "zero wave counters");
             phase_wave pw = upload_phase(rb, e, st);
             timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024); });
-            timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.w256, pw.blocks256); });
+            timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
+            timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.wsub, pw.blocks_sub); });
             copy_d2h(rb.h_rate.data(), rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st);
+            if (pw.wsub.learned)
+                copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st);
             hip_check(hipStreamSynchronize(st), "rating pass");
             timer.resolve();
             for (op_run *r : now) r->after_rate();
@@ -576,7 +634,8 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             hip_check(hipMemsetAsync(rb.raw_audit.p, 0, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st), "zero audit bins");
             hip_check(hipMemsetAsync(rb.err.p, 0, sizeof(int)*nr, st), "zero errors");
             hip_check(hipMemsetAsync(rb.qcount.p, 0, sizeof(int)*ng, st), "zero queue counters");
-            hip_check(hipMemsetAsync(rb.wave_ctr.p, 0, sizeof(unsigned long long)*3*nr, st), "zero wave counters");
+            hip_check(hipMemsetAsync(rb.wave_ctr.p, 0, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st),
+                      "zero wave counters");
             hip_check(hipMemsetAsync(rb.work.p, 0, sizeof(int), st), "zero chain counter");
             phase_wave pw = upload_phase(rb, e, st);
             timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256, rb.qprefix.p); });
@@ -584,14 +643,14 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w256, pw.blocks256); });
             copy_d2h(rb.h_err.data(), rb.err.p, sizeof(int)*nr, st);
             copy_d2h(rb.h_audit.data(), rb.raw_audit.p, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st);
-            copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*3*nr, st);
+            copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st);
             hip_check(hipStreamSynchronize(st), "chain pass");
             timer.resolve();
             for (op_run *r : now) r->after_chain();
         }
         if (!work) break;
     }
-    copy_d2h(rb.h_counters.data(), rb.counters.p, sizeof(unsigned long long)*4*nr, st);
+    copy_d2h(rb.h_counters.data(), rb.counters.p, sizeof(unsigned long long)*GDX_CTRS*nr, st);
 }
 
 void run_operator(context &c, diag_state *dg, run_buffers &rb, double duration, gdx_col_stats *out)
@@ -786,7 +845,7 @@ int gdx_col_operator_batch(const gdx_col_handle *handles, int count, double dura
         for (int i = 1; i < count; ++i) hip_check(hipStreamSynchronize(handles[i]->c.stream), "batch stream");
         kernel_timer timer(h0->c);
         run_swarms(runs, h0->rb, h0->c.stream, timer);
-        for (int i = 0; i < count; ++i) runs[i]->finish(h0->c.stream, h0->rb.h_counters.data() + 4*i, out ? out + i : nullptr);
+        for (int i = 0; i < count; ++i) runs[i]->finish(h0->c.stream, h0->rb.h_counters.data() + size_t(GDX_CTRS)*i, out ? out + i : nullptr);
     });
 }
 
@@ -1044,13 +1103,39 @@ int gdx_col_rate_pass(gdx_col_handle h)
                                  c.stream), "group table");
         std::vector<phase_entry> e;
         for (int g = 0; g < LOCAL_GROUPS; ++g)
-            if (c.group_cnt[g] > 0) e.push_back(phase_entry{g, c.group_beg[g], c.group_cnt[g], 0.0});
+            if (c.group_cnt[g] > 0) e.push_back(phase_entry{g, c.group_beg[g], c.group_cnt[g], 0.0, -1});
         launch_reset_scratch(c.stream, rb.table.p, LOCAL_GROUPS);
         hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, c.stream), "zero rate bins");
         phase_wave pw = upload_phase(rb, e, c.stream);
         launch_publish(c.stream, pw.w1024, pw.blocks1024);
-        launch_bath_rate(c.stream, pw.w256, pw.blocks256);
+        launch_bath_rate(c.stream, pw.wsub, pw.blocks_sub);
         hip_check(hipStreamSynchronize(c.stream), "rating pass");
+    });
+}
+
+int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, double *rate)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    int rc = gdx_col_rate_pass(h);
+    if (rc != GDX_COL_OK) return rc;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(features && scale && rate, GDX_COL_ERR_ARG, "null sample output");
+        hip_check(hipMemsetAsync(features, 0, sizeof(float)*GDX_COL_MAJORANT_FEATURES*size_t(c.n), c.stream), "samples");
+        hip_check(hipMemsetAsync(scale, 0, sizeof(double)*size_t(c.n), c.stream), "samples");
+        hip_check(hipMemsetAsync(rate, 0, sizeof(double)*size_t(c.n), c.stream), "samples");
+        launch_sample(c, make_gref(c, 0, h->rb, 0, 0), features, scale, rate);
+        hip_check(hipStreamSynchronize(c.stream), "majorant samples");
+    });
+}
+
+int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(!m || (std::isfinite(safety) && safety >= 1.0), GDX_COL_ERR_ARG, "majorant safety factor must be >= 1");
+        h->c.mlp = majorant_pack(m);
+        h->c.safety = m ? safety : 1.0;
     });
 }
 

@@ -91,6 +91,8 @@ struct kd_tree
 // the chain's queue prefix and work counter
 struct run_buffers
 {
+    // due-list layout per phase: ref, beg, cnt, sub (n each), then four block-start arrays (n + 1 each)
+    static constexpr int DUE_INTS = 8, DUE_EXTRA = 4;
     dbuf<gdx_gref> table;
     std::vector<gdx_gref> h_table;
     int *h_int = nullptr;
@@ -118,13 +120,13 @@ struct run_buffers
         raw_audit.alloc(size_t(groups)*COL_BIN_S);
         qcount.alloc(groups);
         err.alloc(swarms);
-        counters.alloc(size_t(swarms)*4);
-        wave_ctr.alloc(size_t(swarms)*3);
+        counters.alloc(size_t(swarms)*GDX_CTRS);
+        wave_ctr.alloc(size_t(swarms)*GDX_WAVE_CTRS);
         h_rate.resize(size_t(groups)*COL_BIN_S);
         h_audit.resize(size_t(groups)*COL_BIN_S);
         h_err.resize(swarms);
-        h_counters.resize(size_t(swarms)*4);
-        h_wave.resize(size_t(swarms)*3);
+        h_counters.resize(size_t(swarms)*GDX_CTRS);
+        h_wave.resize(size_t(swarms)*GDX_WAVE_CTRS);
     }
     run_buffers() = default;
     run_buffers(const run_buffers &) = delete;
@@ -143,9 +145,9 @@ struct run_buffers
         if (h_dt) (void)hipHostFree(h_dt);
         h_int = nullptr;
         h_dt = nullptr;
-        hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_int), sizeof(int)*(5*m + 2)), "pinned due list");
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_int), sizeof(int)*(DUE_INTS*m + DUE_EXTRA)), "pinned due list");
         hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_dt), sizeof(double)*m), "pinned due list");
-        d_int.alloc(5*m + 2);
+        d_int.alloc(DUE_INTS*m + DUE_EXTRA);
         d_dt.alloc(m);
         qprefix.alloc(m + 1);
         work.alloc(1);
@@ -220,6 +222,10 @@ struct context
     dbuf<int> queue;           // event owners of one wave, in their group's slot range
     dbuf<double4> qsum;        // chain sums (hazard, j1, j2, jmax) per queue position
     dbuf<int> qevents;         // chain events per queue position, -1 after an error
+    dbuf<double2> qstart;      // learned: {candidate time, bound} per queue position
+    dbuf<double> lam;          // learned: per-slot bound of the current bath
+    const void *mlp = nullptr; // learned: packed network of the attached majorant (null: plain)
+    double safety = 1.2;
     bool diag = false;         // GDX_COL_DIAGNOSTICS
     dbuf<gdx_event_work> evw;  // per-slot event statistics of the current call
     dbuf<gdx_event_work> ev_sum;
@@ -259,6 +265,10 @@ void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng);
 void launch_publish(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix);
+void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, double *rate);
+// ---- majorant.hip ---------------------------------------------------------------------------
+const void *majorant_pack(gdx_col_majorant m);
 int chain_resident_blocks();
 void launch_chain(hipStream_t st, const gdx_wave &w, int *work, int resident);
 void launch_chain_audit(hipStream_t st, const gdx_wave &w, int blocks);

@@ -101,14 +101,32 @@ struct gdx_gref
     int *qcount;                              // the group's event-queue length
     unsigned long long *scr_min, *scr_max;    // the group's size extrema scratch
     double *lower, *upper;                    // the group's moving size bounds
-    unsigned long long *counters;             // the swarm's operator counters: events, chained owners, pair
-                                              // terms, longest chain
-    unsigned long long *wave_ctr;             // the swarm's per-wave counters: longest chain, chain blocks of
-                                              // the reference's continuation scheme, chained owners
+    unsigned long long *counters;             // the swarm's operator counters (GDX_CTR_*)
+    unsigned long long *wave_ctr;             // the swarm's per-wave counters (GDX_WAVE_*)
     int *err;                                 // the swarm's chain error
     double lambda_0;
     uint64_t seed;
+    // learned majorant (mlp null: the plain method)
+    const void *mlp;                          // packed network of the swarm's majorant
+    double safety;                            // bound = safety * S * softplus(net)
+    double *lam;                              // per slot: the bound of the current bath
+    double2 *qstart;                          // per queue position: {candidate time, bound}; bound 0 = plain start
 };
+
+// operator counters per swarm
+enum
+{
+    GDX_CTR_EVENTS, GDX_CTR_CHAINED, GDX_CTR_PAIRS, GDX_CTR_LONGEST, GDX_CTR_THINNED, GDX_CTR_CANDIDATES,
+    GDX_CTR_ACCEPTED, GDX_CTR_CHECKS, GDX_CTR_VIOLATIONS, GDX_CTRS
+};
+// per-wave counters per swarm
+enum
+{
+    GDX_WAVE_LONGEST, GDX_WAVE_BLOCKS, GDX_WAVE_CHAINED, GDX_WAVE_CHECKS, GDX_WAVE_VIOLATIONS,
+    GDX_WAVE_RATE_CHECKS, GDX_WAVE_RATE_VIOLATIONS, GDX_WAVE_CTRS
+};
+// learned swarms rate one owner in THIN_STRIDE exactly per bath (the controller and audit subsample)
+constexpr int THIN_STRIDE = 8;
 
 // the due groups of one phase of a run, in device memory: group-table index, owner slot range in the
 // group's swarm, first block of each group for the launching kernel's owners per block, bath end
@@ -123,6 +141,8 @@ struct gdx_wave
     const int *bstart;           // n + 1 entries
     const double *dt;
     const int *qprefix;          // chain: exclusive prefix of the groups' queue lengths (n + 1)
+    const int *sub;              // per group: exact-rate subsample phase of a learned swarm, -1 plain
+    int learned;                 // some group belongs to a learned swarm
 };
 
 #endif  // GDX_MODEL_H

@@ -897,6 +897,8 @@ void load_state(context &c)
     c.cached.alloc(na);
     c.rng.alloc(na);
     c.queue.alloc(na);
+    c.qstart.alloc(na);
+    c.lam.alloc(na);
     c.qsum.alloc(na);
     if (c.diag) c.evw.alloc(na);
     c.qevents.alloc(na);

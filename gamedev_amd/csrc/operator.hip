@@ -17,6 +17,7 @@
 // the due groups of several independent swarms (realizations) as well as those of one.
 #include "gdx_internal.h"
 #include "gdx_device.h"
+#include "gdx_mlp.h"
 
 #include <algorithm>
 #include <cstdint>
@@ -54,6 +55,21 @@ __device__ __forceinline__ bool due_block(const gdx_wave &w, int per_block, int 
     s0 = w.beg[gi] + local*per_block;
     s1 = min(w.beg[gi] + w.cnt[gi], s0 + per_block);
     return s0 < s1;
+}
+
+// owners of due group gi rated with the exact K-sum in this bath: all of a plain swarm's, one in
+// THIN_STRIDE of a learned swarm's (slots beg + sub, beg + sub + THIN_STRIDE, ...)
+__device__ __forceinline__ int rated_count(const gdx_wave &w, int gi)
+{
+    const int sub = w.learned ? w.sub[gi] : -1;
+    const int cnt = w.cnt[gi];
+    return sub < 0 ? cnt : max(0, (cnt - sub + THIN_STRIDE - 1) / THIN_STRIDE);
+}
+__device__ __forceinline__ bool exact_owner(const gdx_wave &w, int gi, int s)
+{
+    if (!w.learned) return true;
+    const int sub = w.sub[gi];
+    return sub < 0 || (s - w.beg[gi]) % THIN_STRIDE == sub;
 }
 
 // wavefront reductions of a double through DPP lane moves (no LDS): quad exchanges, row shifts by 4
@@ -214,12 +230,27 @@ struct fold3
 };
 __device__ __forceinline__ fold3 add3(const fold3 &x, const fold3 &y) { return {x.r + y.r, x.a + y.a, x.b + y.b}; }
 
+template <bool SUB>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_wave w)
 {
     __shared__ double accd[COL_BIN_S][4];     // mass, weighted rate, change, second
     __shared__ int acci[COL_BIN_S][3];        // count, owner count, invalid count
-    int gi, s0, s1;
-    if (!due_block(w, RATE_OWNERS, gi, s0, s1)) return;
+    // blocks cover the group's rated owners (every owner, or a learned swarm's subsample)
+    const int b = xcd_block();
+    if (b >= w.bstart[w.n]) return;
+    int gi = 0;
+    {
+        int lo = 0, hi = w.n - 1;
+        while (lo < hi)
+        {
+            int mid = (lo + hi + 1) >> 1;
+            if (w.bstart[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        gi = lo;
+    }
+    const int i = (b - w.bstart[gi])*RATE_OWNERS + int(threadIdx.x);
+    const int sub = SUB ? w.sub[gi] : -1;
     const gdx_gref *GP = w.table + w.ref[gi];
     const int *nbr = GP->nbr;
     const double2 *st = GP->st;
@@ -232,8 +263,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
     __syncthreads();
     const int lane = threadIdx.x % WAVE;
-    const int s = s0 + int(threadIdx.x);
-    const bool own = s < s1;
+    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : sub + THIN_STRIDE*i);
+    const bool own = s < w.beg[gi] + w.cnt[gi];
     double2 me = own ? st[s] : make_double2(1.0, 0.0);
     const double m = own ? measure[s] : 0.0;
     gdx_env e{};
@@ -251,7 +282,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     }
     const cat_moments_lds cm{&cats[threadIdx.x][0]};
     const bool live = own && m > 0.0;
-    const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : s0)*N_K);
+    const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : w.beg[gi])*N_K);
 
     fold3 stack[LOG2K];
     fold3 total{0.0, 0.0, 0.0};
@@ -297,6 +328,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     }
     const double R = total.r, J1 = total.a, J2 = total.b;
     if (own) GP->cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
+    if constexpr (SUB)
+    {
+        // the exact subsample of a learned swarm checks the bound of the same bath
+        if (sub >= 0)
+        {
+            const bool check = own && live && V;
+            const bool viol = check && !(R <= GP->lam[s]);
+            const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
+            if (lane == 0 && nc)
+            {
+                atomicAdd(GP->counters + GDX_CTR_CHECKS, (unsigned long long)nc);
+                atomicAdd(GP->wave_ctr + GDX_WAVE_RATE_CHECKS, (unsigned long long)nc);
+                if (nv)
+                {
+                    atomicAdd(GP->counters + GDX_CTR_VIOLATIONS, (unsigned long long)nv);
+                    atomicAdd(GP->wave_ctr + GDX_WAVE_RATE_VIOLATIONS, (unsigned long long)nv);
+                }
+            }
+        }
+    }
 
     // moving bounds from this wave's extrema (col_size_bnds.cu:12-21); read after the pair loop so
     // they hold no registers during it
@@ -351,6 +402,126 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         int v = acci[b][f];
         if (v != 0) atomicAdd(&(&G.raw_rate[b].count)[f], v);
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// learned majorant: features, bound per owner (no reference counterpart)
+// ---------------------------------------------------------------------------------------------
+// Kernel weight of one partner (the factors of _get_col_chain_rate that need no transcendental,
+// _col_rates.cuh:167-223): N_j for the constant kernel, N_j (m_i + m_j) and N_j m_i m_j for the sum
+// and product kernels, N_j (s_i + s_j)^2 for the physical one (over the vertical-overlap root in a
+// vertically integrated disk). The scale S = sum over partners / measure (times lambda_0 for the
+// synthetic kernels) leaves the network the relative speed and outcome factors to predict.
+__device__ __forceinline__ double partner_weight(double size_i, double mass_i, double2 q, double hg_i, double hg_j)
+{
+    if constexpr (COAG_KERNEL == 0) { (void)size_i; (void)mass_i; (void)hg_i; (void)hg_j; return q.y; }
+    else if constexpr (COAG_KERNEL == 1) { (void)hg_i; (void)hg_j; return q.y*(mass_i + grain_mass(q.x)); }
+    else if constexpr (COAG_KERNEL == 2) { (void)hg_i; (void)hg_j; return q.y*mass_i*grain_mass(q.x); }
+    else
+    {
+        (void)mass_i;
+        double a = size_i + q.x;
+        double w = q.y*a*a;
+        if constexpr (PARTNER_HG) w /= sqrt(hg_i*hg_i + hg_j*hg_j);
+        else { (void)hg_i; (void)hg_j; }
+        return w;
+    }
+}
+
+// features of owner s against its published partners (FP32):
+//   0 ln s_i, 1 ln St_i (physical kernel), 2-5 partner-weighted mean, spread, min and max of
+//   ln(s_j / s_i), 6 ln of the owner's radial distance and 7 its height over the gas scale height
+//   (physical kernel); returns S (0 when no partner has weight)
+__device__ double owner_features(const gdx_gref &G, int s, float *f)
+{
+    const double2 me = G.st[s];
+    const double m = G.measure[s];
+    const double size_i = me.x, mass_i = grain_mass(size_i);
+    const float ls = logf(static_cast<float>(size_i));
+    double hg_i = 0.0;
+    if constexpr (PARTNER_HG) hg_i = G.hgs[s];
+    const int *row = G.nbr + (long long)s*N_K;
+    double W = 0.0, m1 = 0.0, m2 = 0.0;
+    float dmin = __builtin_inff(), dmax = -__builtin_inff();
+    for (int k = 0; k < N_K; ++k)
+    {
+        const int j = row[k];
+        if (j < 0) continue;
+        const double2 q = G.res[j];
+        double hg_j = 0.0;
+        if constexpr (PARTNER_HG) hg_j = G.hgs[j];
+        const double w = partner_weight(size_i, mass_i, q, hg_i, hg_j);
+        if (!(w > 0.0) || !isfinite(w)) continue;
+        const float d = logf(static_cast<float>(q.x)) - ls;
+        W += w;
+        m1 += w*d;
+        m2 += w*d*d;
+        dmin = fminf(dmin, d);
+        dmax = fmaxf(dmax, d);
+    }
+    for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
+    f[0] = ls;
+    if (W > 0.0)
+    {
+        double mean = m1 / W;
+        f[2] = static_cast<float>(mean);
+        f[3] = static_cast<float>(sqrt(fmax(0.0, m2 / W - mean*mean)));
+        f[4] = dmin;
+        f[5] = dmax;
+    }
+    if constexpr (COAG_KERNEL == 3)
+    {
+        const gdx_env e = G.env[s];
+        f[1] = logf(static_cast<float>(stokes_scaled(size_i, e.radial, e.strat)));
+        f[6] = logf(static_cast<float>(e.radial));
+        f[7] = static_cast<float>(e.Z*e.omega / e.cs);
+    }
+    if (!(m > 0.0) || !(W > 0.0)) return 0.0;
+    return (COAG_KERNEL == 3 ? W : G.lambda_0*W) / m;
+}
+
+// bound of every due owner of a learned swarm: features per lane, then the network over the
+// wavefront's 64 owners as four 16-owner tiles on the matrix cores
+__global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
+{
+    __shared__ float feat[BLOCK][mlp::F + 1];
+    __shared__ float ys[BLOCK];
+    int gi, s0, s1;
+    if (!due_block(w, RATE_OWNERS, gi, s0, s1)) return;
+    const gdx_gref &G = w.table[w.ref[gi]];
+    const int s = s0 + int(threadIdx.x);
+    const bool own = s < s1;
+    float f[mlp::F];
+    double S = 0.0;
+    if (own) S = owner_features(G, s, f);
+    else for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
+    for (int k = 0; k < mlp::F; ++k) feat[threadIdx.x][k] = f[k];
+    __syncthreads();
+    const mlp::pack *P = static_cast<const mlp::pack *>(G.mlp);
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    for (int tt = 0; tt < WAVE / 16; ++tt)
+    {
+        const int row = wv*WAVE + tt*16 + (lane & 15);
+        mlp::forward_state fs;
+        float z = mlp::tile_forward(P, mlp::input_fragment(P, &feat[row][0], true), fs);
+        if (lane < 16) ys[row] = mlp::softplus(z);
+    }
+    __syncthreads();
+    if (own) G.lam[s] = G.safety*S*static_cast<double>(ys[threadIdx.x]);
+}
+
+// training samples: features, scale and the cached exact rate of every active owner, scattered to
+// particle order
+__global__ __launch_bounds__(256) void k_sample(gdx_gref G, int n, float *feat, double *scale, double *rate)
+{
+    const int s = blockIdx.x*blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    float f[mlp::F];
+    double S = owner_features(G, s, f);
+    const long long p = G.slot_pidx[s];
+    for (int k = 0; k < mlp::F; ++k) feat[p*mlp::F + k] = f[k];
+    scale[p] = S;
+    rate[p] = G.cached[s].x;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -488,10 +659,36 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     const int lane = threadIdx.x % WAVE;
     const int s = s0 + int(threadIdx.x);
     const bool valid = s < s1;
+    const bool learned = w.learned && w.sub[gi] >= 0;
     bool chain = false, done = false;
+    // learned swarms: owners outside the exact subsample draw a candidate time from their bound
+    // (thinning); the subsample follows the plain screen (its bound was checked by the rate pass)
+    bool thin = false, candidate = false, check = false, violation = false;
+    double2 qs = make_double2(0.0, 0.0);
     double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
     double2 old = make_double2(1.0, 0.0);
-    if (valid)
+    if (valid && learned && !exact_owner(w, gi, s))
+    {
+        thin = measure[s] > 0.0;
+        const double L = thin ? G.lam[s] : 0.0;
+        if (thin && (!isfinite(L) || L < 0.0))
+        {
+            // unusable bound: the owner takes the plain chain from the bath start
+            check = violation = chain = true;
+        }
+        else if (thin && L > 0.0)
+        {
+            uint64_t trial = rng[s];
+            const double t1 = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / L;
+            rng[s] = trial;
+            if (t1 < end)
+            {
+                candidate = chain = true;
+                qs = make_double2(t1, L);
+            }
+        }
+    }
+    else if (valid)
     {
         double4 c = cached[s];
         double m = measure[s];
@@ -544,7 +741,29 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         int base = 0;
         if (lane == leader) base = atomicAdd(G.qcount, __popcll(ball));
         base = __builtin_amdgcn_readlane(base, leader);
-        if (chain) G.queue[w.beg[gi] + base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
+        const int pos = w.beg[gi] + base + __popcll(ball & ((1ull << lane) - 1ull));
+        if (chain) G.queue[pos] = s;
+        if (chain && learned) G.qstart[pos] = qs;
+    }
+    if (learned)
+    {
+        const unsigned n_thin = __popcll(__ballot(thin)), n_cand = __popcll(__ballot(candidate));
+        const unsigned n_check = __popcll(__ballot(check)), n_viol = __popcll(__ballot(violation));
+        if (lane == 0)
+        {
+            if (n_thin) atomicAdd(G.counters + GDX_CTR_THINNED, (unsigned long long)n_thin);
+            if (n_cand) atomicAdd(G.counters + GDX_CTR_CANDIDATES, (unsigned long long)n_cand);
+            if (n_check)
+            {
+                atomicAdd(G.counters + GDX_CTR_CHECKS, (unsigned long long)n_check);
+                atomicAdd(G.wave_ctr + GDX_WAVE_CHECKS, (unsigned long long)n_check);
+            }
+            if (n_viol)
+            {
+                atomicAdd(G.counters + GDX_CTR_VIOLATIONS, (unsigned long long)n_viol);
+                atomicAdd(G.wave_ctr + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
+            }
+        }
     }
     __syncthreads();
     flush_audit(a, G.raw_audit);
@@ -599,7 +818,11 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     const unsigned long long seg_lead = (N_K == 64) ? 1ull : (N_K == 32) ? 0x0000000100000001ull
                                                                          : 0x0001000100010001ull;
     unsigned long long n_events = 0, n_chain = 0, n_pairs = 0, n_blocks = 0;
+    unsigned n_checks = 0, n_viol = 0, n_acc = 0;
     int longest = 0;
+    // thinning candidate of a learned swarm: its first rate evaluation decides at the candidate time
+    bool pending = false;
+    double bound = 0.0;
     bool more = true, active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
     double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
@@ -623,8 +846,20 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             if (longest) atomicMax(wave_of + 0, (unsigned long long)longest);
             if (n_blocks) atomicAdd(wave_of + 1, n_blocks);
             if (n_chain) atomicAdd(wave_of + 2, n_chain);
+            if (n_checks)
+            {
+                atomicAdd(ctr_of + GDX_CTR_CHECKS, (unsigned long long)n_checks);
+                atomicAdd(wave_of + GDX_WAVE_CHECKS, (unsigned long long)n_checks);
+            }
+            if (n_viol)
+            {
+                atomicAdd(ctr_of + GDX_CTR_VIOLATIONS, (unsigned long long)n_viol);
+                atomicAdd(wave_of + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
+            }
+            if (n_acc) atomicAdd(ctr_of + GDX_CTR_ACCEPTED, (unsigned long long)n_acc);
         }
         n_events = n_chain = n_pairs = n_blocks = 0;
+        n_checks = n_viol = n_acc = 0;
         longest = 0;
     };
     // the wavefront's current run of queue positions [cb, ce), taken CHAIN_CHUNK at a time from the
@@ -678,6 +913,14 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 // the partners are the published states: fixed for the whole chain
                 q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
                 time = hazard = j1 = j2 = jmax = 0.0;
+                pending = false;
+                if (w.learned)
+                {
+                    const double2 qs = G->qstart[pos];
+                    pending = qs.y > 0.0;
+                    time = qs.x;
+                    bound = qs.y;
+                }
                 events = 0;
                 active = true;
             }
@@ -701,15 +944,42 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         n_pairs += (kl == 0) ? N_K : 0;
         bool fin = false;
         int code = 0;
+        bool now = false;     // accepted thinning candidate: the event happens at the current time
         if (LAST == -2) { code = 1; fin = true; }
         else if (!isfinite(R) || R < 0.0) { code = 2; fin = true; }
+        else
+        {
+            if (pending)
+            {
+                // exact rate at the candidate time: above the bound, the owner restarts the plain chain
+                // from the bath start on fresh draws; otherwise the candidate is accepted with R / bound
+                // and the compensators take R over [0, candidate)
+                pending = false;
+                ++n_checks;
+                if (R > bound)
+                {
+                    ++n_viol;
+                    time = 0.0;
+                }
+                else
+                {
+                    hazard = R*time;
+                    j1 = J1*time;
+                    j2 = J2*time;
+                    jmax = JM;
+                    now = col_uniform(pidx, seed, ctr)*bound < R;
+                    n_acc += now ? 1u : 0u;
+                }
+            }
+        }
+        if (fin) {}
         else if (R == 0.0) { time = end; fin = true; }
         else
         {
-            double wait = -log(col_uniform(pidx, seed, ctr)) / R;
+            double wait = now ? 0.0 : -log(col_uniform(pidx, seed, ctr)) / R;
             double remaining = end - time;
-            if (!isfinite(wait) || !(wait > 0.0)) { code = 3; fin = true; }
-            else if (wait >= remaining)
+            if (!now && (!isfinite(wait) || !(wait > 0.0))) { code = 3; fin = true; }
+            else if (!now && wait >= remaining)
             {
                 hazard += R*remaining;
                 j1 += J1*remaining;
@@ -721,7 +991,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             else
             {
                 double event_time = time + wait;
-                if (!(event_time > time)) { code = 4; fin = true; }
+                if (!now && !(event_time > time)) { code = 4; fin = true; }
                 else
                 {
                     hazard += R*wait;
@@ -826,6 +1096,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_wave w)
     const int pos = s0 + int(threadIdx.x);
     // owners that ended in an error are left out: the operator call fails on them
     bool take = pos < s1 && pos < q1 && qevents[pos] >= 0;
+    // learned swarms audit their exact subsample only
+    if (take && w.learned) take = exact_owner(w, gi, queue[pos]);
     double2 old = make_double2(1.0, 0.0);
     double size_end = 1.0, rate0 = 0.0;
     double4 sum = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -882,7 +1154,8 @@ void launch_publish(hipStream_t st, const gdx_wave &w, int blocks)
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks)
 {
     if (!blocks) return;
-    k_bath_rate<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    if (w.learned) k_bath_rate<true><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    else k_bath_rate<false><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch bath rate");
 }
 
@@ -913,6 +1186,20 @@ void launch_event_sum(context &c, hipStream_t st)
 {
     k_event_sum<<<GDX_EVENT_CATEGORIES, 256, 0, st>>>(c.evw.p, c.n_active, c.ev_sum.p);
     hip_check(hipGetLastError(), "launch event sum");
+}
+
+void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks)
+{
+    if (!blocks) return;
+    k_majorant<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    hip_check(hipGetLastError(), "launch majorant");
+}
+
+void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, double *rate)
+{
+    if (c.n_active <= 0) return;
+    k_sample<<<(c.n_active + 255) / 256, 256, 0, c.stream>>>(G, c.n_active, feat, scale, rate);
+    hip_check(hipGetLastError(), "launch sample");
 }
 
 void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix)

@@ -52,7 +52,8 @@ extern "C" {
 #define GDX_COL_KERNEL_SCREEN    2
 #define GDX_COL_KERNEL_CHAIN     3
 #define GDX_COL_KERNEL_AUDIT     4
-#define GDX_COL_KERNELS          5
+#define GDX_COL_KERNEL_MAJORANT  5   /* learned: features and network per due owner */
+#define GDX_COL_KERNELS          6
 
 typedef struct gdx_col_ctx *gdx_col_handle;
 
@@ -84,6 +85,12 @@ typedef struct {
     double dt_col;                  /* smallest bath duration of the last wave (_col_chain.cuh:481-482) */
     double clock_dyn;               /* operator clock at return: the horizon */
     long long max_chain_events;     /* most events of one owner in one bath (the longest serial chain) */
+    /* learned majorant (gdx_col_use_majorant; zero for the plain method) */
+    long long thinned;              /* owner baths screened against the majorant instead of the exact rate */
+    long long candidates;           /* thinning candidates inside the bath */
+    long long accepted;             /* candidates accepted as events */
+    long long exact_checks;         /* owner baths whose exact rate was compared with the majorant */
+    long long violations;           /* of those, exact rate above the majorant */
 } gdx_col_stats;
 
 /* library and model */
@@ -189,6 +196,47 @@ int gdx_morton_get(gdx_morton_handle h, unsigned long long *keys, int *idx, floa
                    int *begin, int *count, int *child8, int *child_count);
 /* exact top-K for device float4 queries {x, y, z, search cap}: device outputs codes (3*idx, -1) and d^2 */
 int gdx_morton_knn(gdx_morton_handle h, const void *queries4, int nq, int k, int *codes, float *d2);
+
+/* ---- learned rate majorant: no reference counterpart --------------------------------------------
+ * An 8 -> 64 -> 64 -> 1 perceptron (SiLU, softplus output) in BF16 with FP32 accumulation, trained on
+ * the GPU. For owner i it predicts t_i = R_i / S_i, the exact bath-start rate R_i (col_bath_rate.cu:
+ * 13-129) over a cheap partner sum S_i; the operator then draws event times by Poisson thinning against
+ * Lambda_i = safety * S_i * softplus(net(f_i)) and accepts with the exact FP64 rate. Features (device
+ * rows of GDX_COL_MAJORANT_FEATURES floats) and targets come from gdx_col_majorant_sample. */
+#define GDX_COL_MAJORANT_FEATURES 8
+#define GDX_COL_MAJORANT_PARAMS   4801   /* W1[8][64], b1[64], W2[64][64], b2[64], w3[64], b3 */
+
+typedef struct gdx_col_majorant_ctx *gdx_col_majorant;
+
+typedef struct {
+    int epochs, batch;              /* full minibatches per epoch: n / batch (the remainder is dropped) */
+    double lr, beta1, beta2, eps;   /* Adam; beta1, beta2, eps <= 0 select 0.9, 0.999, 1e-8 */
+    unsigned long long seed;        /* sample order (one shuffle per epoch) */
+} gdx_col_train_config;
+
+int gdx_col_majorant_create(unsigned long long seed, gdx_col_majorant *out);
+int gdx_col_majorant_destroy(gdx_col_majorant m);
+const char *gdx_col_majorant_last_error(gdx_col_majorant m);
+/* host copies of the FP32 parameters and the input normalization {mean[8], scale[8]} (either may be NULL);
+ * set_params resets the Adam moments */
+int gdx_col_majorant_get_params(gdx_col_majorant m, float *params, float *norm16);
+int gdx_col_majorant_set_params(gdx_col_majorant m, const float *params, const float *norm16);
+/* device rows in, device softplus outputs out */
+int gdx_col_majorant_infer(gdx_col_majorant m, const float *features, long long n, float *out);
+/* gradient (host, GDX_COL_MAJORANT_PARAMS) and value of the mean loss (ln softplus(z) - log_target)^2 */
+int gdx_col_majorant_grad(gdx_col_majorant m, const float *features, const float *log_target, int n, float *grad,
+                          double *loss);
+/* Adam training over device samples; epoch_loss (host, epochs entries, may be NULL) gets each epoch's
+ * mean minibatch loss before the updates */
+int gdx_col_majorant_train(gdx_col_majorant m, const float *features, const float *log_target, long long n,
+                           const gdx_col_train_config *cfg, float *epoch_loss);
+/* training samples of the current state in particle order (device outputs, n rows; inactive rows are
+ * zero): features, the scale S_i and the exact bath-start rate R_i (negative when invalid). Publishes the
+ * reservoir from the owners' states as gdx_col_rate_pass does. */
+int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, double *rate);
+/* operator calls of h use the majorant (NULL: the plain method); safety >= 1 multiplies the bound. The
+ * model must outlive its use by h. */
+int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,215 @@
+"""The learned majorant inside the operator, on the GPU through the C ABI: training samples against a
+restatement of the features and the oracle's exact rates, ensemble agreement of the thinned operator
+with the plain one, the fallback when the bound fails, and what the trained network gains."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _lognormal_2d(n, seed):
+    """plain_2d disk with log-normal sizes (equal-mass representatives) so the features vary."""
+    from gamedev_amd import synth
+    st = synth.plain_2d_state(n, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    s = np.clip(1e-4*np.exp(rng.standard_normal(n)), 1e-5, 1e-2)
+    st["size"] = s
+    st["numr"] = (st["total_mass"]/n)/synth.grain_mass(s)
+    return st
+
+
+def _operator(model, st, gpu, seed):
+    from gamedev_amd import CollisionOperator, synth
+    d = synth.to_device(st, gpu)
+    op = CollisionOperator(model, len(st["x"]), st["total_mass"], seed=seed)
+    op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
+    op.geometry()
+    return op, d
+
+
+def _restated_features(model, st, codes, meas, act, lambda_0):
+    """owner_features of the library in numpy: partner weights, log-size statistics and S."""
+    s = st["size"]
+    N = st["numr"]
+    n, K = codes.shape
+    f = np.zeros((n, 8), np.float64)
+    S = np.zeros(n)
+    ls = np.log(s.astype(np.float32)).astype(np.float64)
+    for i in np.nonzero(act)[0]:
+        j = codes[i][codes[i] >= 0]//3
+        if model == "plain_2d":
+            w = N[j]
+        else:
+            w = N[j]*(s[i] + s[j])**2
+        ok = w > 0
+        j, w = j[ok], w[ok]
+        f[i, 0] = ls[i]
+        if len(j) == 0 or not meas[i] > 0:
+            continue
+        d = ls[j] - ls[i]
+        W = w.sum()
+        mean = (w*d).sum()/W
+        f[i, 2] = mean
+        f[i, 3] = np.sqrt(max(0.0, (w*d*d).sum()/W - mean*mean))
+        f[i, 4] = d.min()
+        f[i, 5] = d.max()
+        S[i] = (W if model == "fiducial_3d" else lambda_0*W)/meas[i]
+    return f, S
+
+
+@pytest.mark.parametrize("model", ["plain_2d", "fiducial_3d"])
+def test_samples_match_restatement_and_oracle_rates(gpu, oracles, model):
+    from gamedev_amd import synth
+    n = 20000
+    st = _lognormal_2d(n, 4) if model == "plain_2d" else synth.fiducial_3d_state(n, seed=4)
+    op, _ = _operator(model, st, gpu, seed=1)
+    feats, S, R = (t.cpu().numpy() for t in op.majorant_sample())
+    pts, act, _ = op.sites()
+    codes, meas = op.neighbors()
+    lambda_0 = n/op.info["n_k"]/st["total_mass"]
+    f_ref, S_ref = _restated_features(model, st, codes, meas, act, lambda_0)
+    a = act == 1
+    np.testing.assert_allclose(S[a], S_ref[a], rtol=1e-12, atol=0)
+    cols = [0, 2, 3, 4, 5]
+    np.testing.assert_allclose(feats[a][:, cols], f_ref[a][:, cols], rtol=1e-5, atol=2e-5)
+    if model == "plain_2d":
+        assert np.all(feats[:, [1, 6, 7]] == 0)
+    R_ref = oracles[model].bath_rates(st["y"], st["z"], st["size"], st["numr"], act, codes, meas, lambda_0)[:, 0]
+    np.testing.assert_allclose(R[a], R_ref[a], rtol=1e-12, atol=0)
+    assert np.all(feats[~a] == 0) and np.all(S[~a] == 0)
+
+
+def _train(model, op, horizon, sample_steps, gpu, epochs=20, seed=3):
+    """Samples from plain steps of the operator, then GPU training."""
+    import torch
+    from gamedev_amd import Majorant
+    X, T = [], []
+    for _ in range(sample_steps):
+        f, S, R = op.majorant_sample()
+        ok = (S > 0) & (R > 0)
+        X.append(f[ok])
+        T.append(torch.log(R[ok]/S[ok]).float())
+        op.evolve(horizon)
+    X = torch.cat(X)
+    T = torch.cat(T)
+    m = Majorant(model, seed=seed)
+    batch = 8192 if len(X) >= 8192*4 else 1024
+    losses, safety = m.fit(X, T, epochs=epochs, batch=batch, lr=1e-3, seed=seed)
+    return m, losses, safety
+
+
+def _moments(d):
+    from gamedev_amd import synth
+    s = d["size"].cpu().numpy()
+    N = d["numr"].cpu().numpy()
+    m = np.pi*synth.RHO_0*s**3/6
+    return np.array([N.sum(), (N*m).sum(), (N*m*m).sum(), (N*m**3).sum()])
+
+
+def test_learned_operator_matches_plain_ensemble(gpu):
+    """Thinning against the trained bound with exact FP64 acceptance: ensemble mass moments M0..M3 of the
+    learned operator agree with the plain operator's, events within statistical error."""
+    from gamedev_amd import synth
+    n, h, steps, seeds = 32768, 0.002, 2, 6
+    st = synth.fiducial_3d_state(n, seed=11)
+    op, _ = _operator("fiducial_3d", st, gpu, seed=100)
+    m, losses, safety = _train("fiducial_3d", op, h, 4, gpu)
+    print("[learned] training losses", losses, "safety", safety, flush=True)
+    assert losses[-1] < losses[0]
+    res = {"plain": [], "learned": []}
+    for mode in res:
+        for k in range(seeds):
+            print(f"[learned] ensemble {mode} seed {k}", flush=True)
+            o, d = _operator("fiducial_3d", st, gpu, seed=1000 + k)
+            if mode == "learned":
+                o.use_majorant(m, safety)
+            ev = 0
+            stats = []
+            for _ in range(steps):
+                s = o.evolve(h)
+                ev += s["events"]
+                stats.append(s)
+            res[mode].append((ev, _moments(d), stats))
+    M0 = _moments(synth.to_device(st, gpu))
+    ev_p = np.array([r[0] for r in res["plain"]], float)
+    ev_l = np.array([r[0] for r in res["learned"]], float)
+    mp = np.array([r[1] for r in res["plain"]]).mean(axis=0)
+    ml = np.array([r[1] for r in res["learned"]]).mean(axis=0)
+    print("[learned] events plain", ev_p, "learned", ev_l, flush=True)
+    assert ev_p.mean() > 100
+    sig = np.sqrt(ev_p.var(ddof=1)/seeds + ev_l.var(ddof=1)/seeds + ev_p.mean()/seeds)
+    assert abs(ev_l.mean() - ev_p.mean()) <= 4*sig + 0.03*ev_p.mean()
+    # mass is conserved exactly by both; the other moments agree to 1%
+    assert abs(ml[1]/M0[1] - 1) < 1e-10 and abs(mp[1]/M0[1] - 1) < 1e-10
+    for k in (0, 2, 3):
+        assert abs(ml[k]/mp[k] - 1) < 0.01, (k, ml[k], mp[k])
+    # the change of M0 (collisions remove grains) agrees within its ensemble error
+    d_p = np.array([r[1][0] for r in res["plain"]]) - M0[0]
+    d_l = np.array([r[1][0] for r in res["learned"]]) - M0[0]
+    s0 = np.sqrt(d_p.var(ddof=1)/seeds + d_l.var(ddof=1)/seeds)
+    assert abs(d_l.mean() - d_p.mean()) <= 4*s0 + 0.05*abs(d_p.mean())
+    st_l = [s for r in res["learned"] for s in r[2]]
+    checks = sum(s["exact_checks"] for s in st_l)
+    viol = sum(s["violations"] for s in st_l)
+    cand = sum(s["candidates"] for s in st_l)
+    acc = sum(s["accepted"] for s in st_l)
+    print(f"[learned] checks {checks} violations {viol} candidates {cand} accepted {acc}", flush=True)
+    assert checks > 0 and viol/checks < 0.01
+    assert cand > 0 and 0 < acc <= cand
+
+
+def test_failed_bound_falls_back_to_plain(gpu):
+    """A bound far below the exact rates: every exact check fails, the operator reverts to the plain
+    method after the first wave and still conserves mass."""
+    from gamedev_amd import Majorant, synth
+    n = 16384
+    st = synth.fiducial_3d_state(n, seed=5)
+    op, d = _operator("fiducial_3d", st, gpu, seed=9)
+    m = Majorant("fiducial_3d", seed=0)
+    p, norm = m.params()
+    p[-1] = -30.0                          # softplus(b3) ~ 1e-13: the bound is far too small
+    m.set_params(p, None)
+    op.use_majorant(m, 1.0)
+    M_before = _moments(d)
+    s = op.evolve(0.002)
+    M_after = _moments(d)
+    print("[learned] fallback stats", s, flush=True)
+    assert s["exact_checks"] > 0 and s["violations"] > 0.5*s["exact_checks"]
+    assert abs(M_after[1]/M_before[1] - 1) < 1e-10
+    assert s["events"] > 0
+    s2 = op.evolve(0.002)
+    assert s2["violations"] > 0
+
+
+def test_trained_network_gains_over_untrained_and_plain(gpu):
+    """The gain of training: with the trained bound far fewer owners need the exact K-sum (pair
+    evaluations per owner update drop against the plain operator), and candidates are accepted far
+    more often than with the untrained network."""
+    from gamedev_amd import Majorant, synth
+    n, h = 32768, 0.002
+    st = synth.fiducial_3d_state(n, seed=21)
+    op, _ = _operator("fiducial_3d", st, gpu, seed=200)
+    trained, _, safety = _train("fiducial_3d", op, h, 4, gpu)
+    untrained = Majorant("fiducial_3d", seed=3)
+    f, _, _ = op.majorant_sample()
+    untrained.fit_normalization(f)
+    out = {}
+    for name, mdl in (("plain", None), ("untrained", untrained), ("trained", trained)):
+        o, _ = _operator("fiducial_3d", st, gpu, seed=300)
+        if mdl is not None:
+            o.use_majorant(mdl, safety if mdl is trained else 1.2)
+        tot = {}
+        for _ in range(2):
+            s = o.evolve(h)
+            for k, v in s.items():
+                if isinstance(v, (int, float)):
+                    tot[k] = tot.get(k, 0) + v
+        out[name] = tot
+        print(f"[learned] {name}: pair evals/update {tot['pair_evals']/tot['owner_updates']:.2f} "
+              f"candidates {tot['candidates']} accepted {tot['accepted']} checks {tot['exact_checks']} "
+              f"violations {tot['violations']}", flush=True)
+    per = {k: v["pair_evals"]/v["owner_updates"] for k, v in out.items()}
+    assert per["trained"] < 0.5*per["plain"]
+    acc_t = out["trained"]["accepted"]/max(1, out["trained"]["candidates"])
+    acc_u = out["untrained"]["accepted"]/max(1, out["untrained"]["candidates"])
+    assert acc_t > 2*acc_u and acc_t > 0.3
