@@ -7,7 +7,11 @@ h_op = 0.05 per step. A step is one operator call evolve_local_collisions(h_op) 
 with state resident in HBM; the geometry epoch (sites, sort, hierarchy, top-K) is built once before
 the timed region, as in collision-only runs, and timed separately.
 
-value = sum over timed steps of active owners / wall time (collision updates/s), one GPU.
+value = sum over timed steps of active owners / wall time (collision updates/s), one GPU, for the learned
+operator (a majorant trained on samples of ten plain steps of the same swarm, Poisson thinning with exact
+FP64 acceptance); the plain operator on the same swarm is timed first and reported beside it, with the
+batched ensemble (64 realizations, plain and learned) and the clumped batched stress case (64
+realizations totalling 1e7, vortex and ring overdensities) as secondary legs.
 The roofline object is for the kernel with the largest measured time: its algorithmic bytes per
 launch (owners x bytes per owner, listed in KERNEL_BYTES) over its mean HIP-event duration.
 The CPU baseline runs the oracle (host restatement of the same operator, OpenMP) on a bounded sample.
@@ -36,15 +40,18 @@ K = 32
 #   screen:      cached rate 8 + measure 8 + RNG counter read + write 16 + particle index 4 + published (s, N) 16
 #   chain:       per chained owner 140 (state, clock, RNG, sums), plus codes 4 + partner (s, N) 16 per pair term
 #   chain_audit: per chained owner: queue entry 4 + chain sums 36 + published and final state 24
-KERNEL_BYTES = {"publish": 32, "bath_rate": 20*K + 29, "screen": 52}
+#   majorant:    codes 4K + partner (s, N) 16K + owner (s, N) 16 + measure 8 + environment 64 + bound write 8
+KERNEL_BYTES = {"publish": 32, "bath_rate": 20*K + 29, "screen": 52, "majorant": 20*K + 96}
 CHAIN_OWNER_BYTES, CHAIN_PAIR_BYTES, AUDIT_OWNER_BYTES = 140, 20, 64
 
 
 def kernel_bytes(kernel: str, rec: dict, tot: dict) -> float:
-    """Algorithmic bytes moved by all timed launches of one kernel."""
+    """Algorithmic bytes moved by all timed launches of one kernel (bath_rate: the owners it rated)."""
+    if kernel == "bath_rate":
+        return tot["rated_owners"]*KERNEL_BYTES[kernel]
     if kernel in KERNEL_BYTES:
         return rec["owners"]*KERNEL_BYTES[kernel]
-    chain_pairs = tot["pair_evals"] - K*tot["owner_updates"]
+    chain_pairs = tot["pair_evals"] - K*tot["rated_owners"]
     if kernel == "chain":
         return tot["chain_owners"]*CHAIN_OWNER_BYTES + chain_pairs*CHAIN_PAIR_BYTES
     return tot["chain_owners"]*AUDIT_OWNER_BYTES
@@ -91,10 +98,11 @@ def cpu_baseline(n_sample: int, steps: int, horizon: float, threads: int) -> dic
                       f"h_op={horizon}, operator-only (geometry epoch {geo_s:.2f} s excluded)"}
 
 
-def batch_leg(count: int, steps: int, horizon: float, dev) -> dict:
+def batch_leg(count: int, steps: int, horizon: float, dev, model=None, learned=None) -> dict:
     """Ensemble of independent realizations of log-uniform size (floor(2^U[14,20]) representatives,
     seeds 0..count-1), each with its own lambda_0, controller and random key: one batched operator call
-    per step (shared launches) against the same realizations advanced one call at a time."""
+    per step (shared launches) against the same realizations advanced one call at a time; with a trained
+    majorant, the batched call again with every realization thinning against it."""
     import numpy as np
     import torch
     from gamedev_amd import CollisionOperator, evolve_batch, synth
@@ -113,18 +121,23 @@ def batch_leg(count: int, steps: int, horizon: float, dev) -> dict:
             ops.append((op, d))
         return ops
 
+    def run_batched(ops):
+        evolve_batch([op for op, _ in ops], horizon)                  # warmup
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        waves, stats = [], []
+        for _ in range(steps):
+            out = evolve_batch([op for op, _ in ops], horizon)
+            waves.append([s["waves"] for s in out])
+            stats.extend(out)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, np.array(waves), stats
+
     log(f"batch leg: {count} realizations, {sum(sizes)} representatives")
     ops = build()
     active = sum(op.active_count() for op, _ in ops)
-    evolve_batch([op for op, _ in ops], horizon)                  # warmup
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    waves = []
-    for _ in range(steps):
-        waves.append([s["waves"] for s in evolve_batch([op for op, _ in ops], horizon)])
-    torch.cuda.synchronize()
-    batched = steps*active/(time.perf_counter() - t0)
-    waves = np.array(waves)
+    el, waves, _ = run_batched(ops)
+    batched = steps*active/el
     # the same realizations advanced one operator call at a time
     for op, _ in ops:
         op.evolve(horizon)
@@ -136,10 +149,194 @@ def batch_leg(count: int, steps: int, horizon: float, dev) -> dict:
     torch.cuda.synchronize()
     sequential = steps*active/(time.perf_counter() - t0)
     log(f"batch leg: {batched:.3e} updates/s batched, {sequential:.3e} one call at a time")
-    return {"realizations": count, "representatives": sum(sizes), "active_owners": active, "steps": steps,
-            "updates_per_s_batched": batched, "updates_per_s_one_call_at_a_time": sequential,
-            "batched_over_sequential": batched/sequential,
-            "waves_per_step_mean": float(waves.mean()), "waves_per_step_max": int(waves.max())}
+    out = {"realizations": count, "representatives": sum(sizes), "active_owners": active, "steps": steps,
+           "updates_per_s_batched": batched, "updates_per_s_one_call_at_a_time": sequential,
+           "batched_over_sequential": batched/sequential,
+           "waves_per_step_mean": float(waves.mean()), "waves_per_step_max": int(waves.max())}
+    if model is not None:
+        for op, _ in ops:
+            op.use_majorant(model, learned["training"]["safety"])
+        el, waves, stats = run_batched(ops)
+        out["updates_per_s_batched_learned"] = steps*active/el
+        out["learned_waves_per_step_mean"] = float(waves.mean())
+        chk = sum(s["exact_checks"] for s in stats)
+        out["learned_violation_rate"] = sum(s["violations"] for s in stats)/max(1, chk)
+        log(f"batch leg: {out['updates_per_s_batched_learned']:.3e} updates/s batched with the majorant")
+    return out
+
+
+def ensemble_moments(ds) -> "np.ndarray":
+    """M0..M3 of the mass distribution, sum_i N_i m_i^k, over every realization."""
+    import numpy as np
+    from gamedev_amd import synth
+    M = np.zeros(4)
+    for d in ds:
+        s = d["size"].double().cpu().numpy()
+        N = d["numr"].double().cpu().numpy()
+        m = np.pi*synth.RHO_0*s**3/6.0
+        M += [N.sum(), (N*m).sum(), (N*m*m).sum(), (N*m**3).sum()]
+    return M
+
+
+def clumped_leg(count: int, total: int, steps: int, horizon: float, dev, samples: int, epochs: int) -> dict:
+    """The clumped stress case, batched: `count` realizations totalling `total` representatives (sizes
+    floor(2^U[16,20]) rescaled), vortex and ring overdensities, sigma_ln = 1.5. The majorant is trained on
+    clumped states (10 plain steps of eight extra realizations), then the plain and the learned batched
+    operators advance identical copies of the ensemble; reports both rates, the violation and acceptance
+    rates and the ensemble moments' relative difference."""
+    import numpy as np
+    import torch
+    from gamedev_amd import CollisionOperator, Majorant, evolve_batch, synth
+
+    sizes = synth.ensemble_sizes(count, 16, 20, seed=54321, total=total)
+
+    def build(ns, seed0, rng_seed0):
+        ops = []
+        for r, n in enumerate(ns):
+            st = synth.clumped_3d_state(n, seed=seed0 + r)
+            d = synth.to_device(st, dev)
+            op = CollisionOperator("fiducial_3d", n, st["total_mass"], seed=rng_seed0 + r, defer_writeback=True)
+            op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
+            op.geometry()
+            ops.append((op, d))
+        return ops
+
+    log(f"clumped leg: {count} realizations, {sum(sizes)} representatives")
+    # training set: ten plain steps of eight realizations of the same distribution (other seeds)
+    train_ops = build(sizes[:8], 9000, 500)
+    per = max(1, samples//(10*len(train_ops)))
+    X, T = [], []
+    for _ in range(10):
+        for op, _ in train_ops:
+            f, S, R = op.majorant_sample()
+            ok = torch.nonzero((S > 0) & (R > 0)).squeeze(1)
+            pick = ok[torch.randperm(len(ok), device=dev)[:per]]
+            X.append(f[pick].clone())
+            T.append(torch.log(R[pick]/S[pick]).float())
+        evolve_batch([op for op, _ in train_ops], horizon)
+    del train_ops
+    X, T = torch.cat(X), torch.cat(T)
+    model = Majorant("fiducial_3d", seed=1)
+    losses, safety = model.fit(X, T, epochs=epochs, batch=8192, lr=1e-3, seed=1)
+    log(f"clumped majorant: {len(X)} samples, loss {losses[0]:.4f} -> {losses[-1]:.4f}, safety {safety:.3f}")
+    out = {"realizations": count, "representatives": sum(sizes), "steps": steps, "horizon": horizon,
+           "training_samples": int(len(X)), "first_epoch_loss": float(losses[0]),
+           "last_epoch_loss": float(losses[-1]), "safety": safety}
+    del X, T
+    M_start = None
+    for mode in ("plain", "learned"):
+        ops = build(sizes, 7000, 0)
+        if M_start is None:
+            M_start = ensemble_moments([d for _, d in ops])
+        if mode == "learned":
+            for op, _ in ops:
+                op.use_majorant(model, safety)
+        active = sum(op.active_count() for op, _ in ops)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = []
+        for i in range(steps):
+            stats.extend(evolve_batch([op for op, _ in ops], horizon))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        for op, _ in ops:
+            op.store_state()
+        torch.cuda.synchronize()
+        out[f"updates_per_s_{mode}"] = steps*active/el
+        out[f"ms_per_step_{mode}"] = 1e3*el/steps
+        out[f"events_{mode}"] = int(sum(s["events"] for s in stats))
+        out[f"moments_{mode}"] = ensemble_moments([d for _, d in ops]).tolist()
+        if mode == "learned":
+            chk = sum(s["exact_checks"] for s in stats)
+            cand = sum(s["candidates"] for s in stats)
+            out["violation_rate"] = sum(s["violations"] for s in stats)/max(1, chk)
+            out["acceptance_rate"] = sum(s["accepted"] for s in stats)/max(1, cand)
+            out["exact_pair_evals_per_update"] = sum(s["pair_evals"] for s in stats)/max(1, steps*active)
+        log(f"clumped leg {mode}: {out[f'updates_per_s_{mode}']:.3e} updates/s")
+        del ops
+        torch.cuda.empty_cache()
+    mp, ml = np.array(out["moments_plain"]), np.array(out["moments_learned"])
+    out["moment_rel_diff_learned_vs_plain"] = (ml/mp - 1.0).tolist()
+    out["moment_change_plain"] = (mp/M_start - 1.0).tolist()
+    out["learned_over_plain"] = out["updates_per_s_learned"]/out["updates_per_s_plain"]
+    return out
+
+
+def train_majorant(op, horizon: float, steps: int, samples: int, epochs: int, dev):
+    """Training set from `steps` plain operator steps of the benchmark swarm (`samples` random owners in
+    all, features and ln(R / S) at each step's start), then the library's BF16-MFMA training kernels
+    (Adam, lr 1e-3, batch 8192) and the calibrated safety factor."""
+    import torch
+    from gamedev_amd import Majorant
+    per = max(1, samples//steps)
+    X, T = [], []
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f, S, R = op.majorant_sample()
+        ok = torch.nonzero((S > 0) & (R > 0)).squeeze(1)
+        pick = ok[torch.randperm(len(ok), device=dev, generator=gen)[:per]]
+        X.append(f[pick].clone())
+        T.append(torch.log(R[pick]/S[pick]).float())
+        del f, S, R, ok, pick
+        op.evolve(horizon)
+    X = torch.cat(X)
+    T = torch.cat(T)
+    torch.cuda.synchronize()
+    sample_s = time.perf_counter() - t0
+    m = Majorant("fiducial_3d", seed=0)
+    t0 = time.perf_counter()
+    losses, safety = m.fit(X, T, epochs=epochs, batch=8192, lr=1e-3, seed=0)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    info = {"samples": int(len(X)), "sample_steps": steps, "epochs": epochs, "batch": 8192, "lr": 1e-3,
+            "first_epoch_loss": float(losses[0]), "last_epoch_loss": float(losses[-1]), "safety": safety,
+            "sampling_s": sample_s, "training_s": train_s}
+    log(f"majorant trained on {len(X)} samples: loss {losses[0]:.4f} -> {losses[-1]:.4f}, safety {safety:.3f}, "
+        f"{train_s:.2f} s")
+    return m, safety, info
+
+
+def timed_steps(op, steps: int, horizon: float, label: str) -> tuple[float, dict, int]:
+    """Run `steps` operator calls; returns wall seconds, summed stats and active owners per step."""
+    import torch
+    keys = ("owner_updates", "events", "waves", "baths", "pair_evals", "chain_owners", "thinned", "candidates",
+            "accepted", "exact_checks", "violations", "rated_owners")
+    tot = {k: 0 for k in keys}
+    tot["longest_chain"] = 0
+    updates = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        s = op.evolve(horizon)
+        updates += s["active_owners"]
+        tot["longest_chain"] = max(tot["longest_chain"], s["max_chain_events"])
+        for k in keys:
+            tot[k] += s[k]
+        if (i + 1) % 20 == 0:
+            log(f"{label} step {i + 1}/{steps}")
+    op.store_state()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, tot, updates
+
+
+def roofline_of(kt: dict, tot: dict, dom: str | None = None) -> dict:
+    dom = dom or max(kt, key=lambda k: kt[k]["ms"])
+    rec = kt[dom]
+    bytes_per_launch = kernel_bytes(dom, rec, tot)/max(1, rec["launches"])
+    avg_s = rec["ms"]/max(1, rec["launches"])*1e-3
+    achieved = bytes_per_launch/avg_s/1e9 if avg_s > 0 else 0.0
+    traffic = None
+    tfile = ROOT/"profiles"/f"traffic_{dom}_fiducial_3d.json"
+    if tfile.exists():
+        # HBM bytes per owner from the FETCH_SIZE / WRITE_SIZE counter runs, scaled to this run's launches
+        per_owner = json.loads(tfile.read_text()).get("bytes_per_owner")
+        if per_owner:
+            traffic = per_owner*rec["owners"]/max(1, rec["launches"])
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}
 
 
 def main() -> int:
@@ -156,6 +353,14 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=64,
                     help="realizations of the batched ensemble leg (0 skips it): sizes floor(2^U[14,20]), seeds 0..B-1")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--learned", type=int, default=1, help="1: train the majorant and time the learned operator")
+    ap.add_argument("--train-steps", type=int, default=10)
+    ap.add_argument("--train-samples", type=int, default=1_000_000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--clumped", type=int, default=64,
+                    help="realizations of the clumped batched leg (0 skips it), totalling --clumped-total")
+    ap.add_argument("--clumped-total", type=int, default=10_000_000)
+    ap.add_argument("--clumped-steps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size dispatch that calibrates "
                          "hardware byte counters)")
@@ -194,24 +399,10 @@ def main() -> int:
         op.evolve(args.horizon)
     log("warmup done")
     op.reset_kernel_times()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tot = {"owner_updates": 0, "events": 0, "waves": 0, "baths": 0, "pair_evals": 0, "chain_owners": 0}
-    updates = 0
-    longest_chain = 0
-    for i in range(args.steps):
-        s = op.evolve(args.horizon)
-        updates += s["active_owners"]
-        longest_chain = max(longest_chain, s["max_chain_events"])
-        for k in tot:
-            tot[k] += s[k]
-        if (i + 1) % 20 == 0:
-            log(f"step {i + 1}/{args.steps}")
-    op.store_state()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    value = updates/elapsed
+    elapsed, tot, updates = timed_steps(op, args.steps, args.horizon, "plain")
+    plain_value = updates/elapsed
     kt = op.kernel_times()
+    log(f"plain: {plain_value:.4e} updates/s, {1e3*elapsed/args.steps:.3f} ms/step")
 
     # with a geometry rebuild before every operator call (transported runs)
     torch.cuda.synchronize()
@@ -222,27 +413,60 @@ def main() -> int:
     torch.cuda.synchronize()
     with_geo = args.geometry_steps*n_active/(time.perf_counter() - t0)
 
+    learned = None
+    model = None
+    if args.learned:
+        model, safety, train_info = train_majorant(op, args.horizon, args.train_steps, args.train_samples,
+                                                   args.epochs, dev)
+        op.use_majorant(model, safety)
+        for i in range(args.warmup):
+            op.evolve(args.horizon)
+        op.reset_kernel_times()
+        l_elapsed, l_tot, l_updates = timed_steps(op, args.steps, args.horizon, "learned")
+        l_kt = op.kernel_times()
+        learned_value = l_updates/l_elapsed
+        log(f"learned: {learned_value:.4e} updates/s, {1e3*l_elapsed/args.steps:.3f} ms/step")
+        learned = {"value": learned_value, "ms_per_step": 1e3*l_elapsed/args.steps,
+                   "learned_over_plain": learned_value/plain_value, "training": train_info,
+                   "violation_rate": l_tot["violations"]/max(1, l_tot["exact_checks"]),
+                   "acceptance_rate": l_tot["accepted"]/max(1, l_tot["candidates"]),
+                   "candidates_per_owner_step": l_tot["candidates"]/max(1, l_updates),
+                   "exact_pair_evals_per_update": l_tot["pair_evals"]/max(1, l_updates),
+                   "events_per_s": l_tot["events"]/l_elapsed, "waves_per_step": l_tot["waves"]/args.steps,
+                   "longest_chain_events": l_tot["longest_chain"],
+                   "kernel_ms_per_step": {k: v["ms"]/args.steps for k, v in l_kt.items()},
+                   "roofline": roofline_of(l_kt, l_tot), "totals": l_tot}
+        op.use_majorant(None)
+
     batched = None
     if args.batch > 0:
         del op, d
         torch.cuda.empty_cache()
-        batched = batch_leg(args.batch, args.batch_steps, args.horizon, dev)
+        batched = batch_leg(args.batch, args.batch_steps, args.horizon, dev, model, learned)
 
-    dom = max(kt, key=lambda k: kt[k]["ms"])
-    rec = kt[dom]
-    bytes_per_launch = kernel_bytes(dom, rec, tot)/max(1, rec["launches"])
-    avg_s = rec["ms"]/max(1, rec["launches"])*1e-3
-    achieved = bytes_per_launch/avg_s/1e9 if avg_s > 0 else 0.0
-    traffic = None
-    tfile = ROOT/"profiles"/f"traffic_{dom}_fiducial_3d.json"
-    if tfile.exists():
-        # HBM bytes per owner from the FETCH_SIZE / WRITE_SIZE counter runs, scaled to this run's launches
-        per_owner = json.loads(tfile.read_text()).get("bytes_per_owner")
-        if per_owner:
-            traffic = per_owner*rec["owners"]/max(1, rec["launches"])
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}
+    clumped = None
+    if args.clumped > 0:
+        torch.cuda.empty_cache()
+        clumped = clumped_leg(args.clumped, args.clumped_total, args.clumped_steps, args.horizon, dev,
+                              args.train_samples, args.epochs)
+
+    roofline_plain = roofline_of(kt, tot)
+    plain = {"value": plain_value, "ms_per_step": 1e3*elapsed/args.steps, "roofline": roofline_plain,
+             "events_per_s": tot["events"]/elapsed, "pair_evals_per_s": tot["pair_evals"]/elapsed,
+             "mean_baths_per_owner_step": tot["owner_updates"]/max(1, updates),
+             "waves_per_step": tot["waves"]/args.steps, "chain_owners_per_step": tot["chain_owners"]/args.steps,
+             "longest_chain_events": tot["longest_chain"],
+             "kernel_ms_per_step": {k: v["ms"]/args.steps for k, v in kt.items()}}
+    workload = (f"fiducial 3D disk, single swarm of {args.n} representatives, K={K}, coagulation+fragmentation, "
+                f"h_op={args.horizon}, operator-only with the state resident in HBM")
+    if learned is not None:
+        # the metric's subject: the learned operator on the 1e7-representative swarm, beside the plain one
+        value, ms = learned["value"], learned["ms_per_step"]
+        roofline = learned["roofline"]
+        variant = ("learned: trained BF16 majorant, Poisson thinning with exact FP64 acceptance, exact 1/8 "
+                   "subsample for the controller")
+    else:
+        value, ms, roofline, variant = plain_value, 1e3*elapsed/args.steps, roofline_plain, "plain"
 
     cpu = None
     if not args.no_cpu:
@@ -252,26 +476,21 @@ def main() -> int:
 
     out = {
         "metric": METRIC, "value": value, "unit": "collision updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3*elapsed/args.steps, "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "plain operator (no learned majorant, single swarm): fiducial 3D disk, "
-                               f"{args.n} representatives, K={K}, coagulation+fragmentation, h_op={args.horizon}, "
-                               "operator-only with the state resident in HBM",
-                   "n": args.n, "k": K, "horizon": args.horizon, "variant": "plain"},
+        "config": {"workload": f"{variant.split(':')[0]} operator, {workload}", "n": args.n, "k": K,
+                   "horizon": args.horizon, "variant": variant},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "secondary": {
+            "plain": plain,
+            "learned": learned,
+            "learned_over_plain": learned["learned_over_plain"] if learned else None,
             "geometry_epoch_ms": geometry_ms,
             "updates_per_s_with_geometry_rebuild": with_geo,
-            "events_per_s": tot["events"]/elapsed,
-            "pair_evals_per_s": tot["pair_evals"]/elapsed,
-            "mean_baths_per_owner_step": tot["owner_updates"]/max(1, updates),
-            "waves_per_step": tot["waves"]/args.steps,
-            "chain_owners_per_step": tot["chain_owners"]/args.steps,
-            "longest_chain_events": longest_chain,
-            "kernel_ms_per_step": {k: v["ms"]/args.steps for k, v in kt.items()},
             "active_owners": n_active,
             "batched_ensemble": batched,
+            "clumped_batched": clumped,
         },
     }
     print(json.dumps(out))

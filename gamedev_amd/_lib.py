@@ -52,7 +52,8 @@ class Stats(C.Structure):
                 ("active_owners", C.c_longlong), ("events", C.c_longlong), ("chain_owners", C.c_longlong),
                 ("pair_evals", C.c_longlong), ("dt_col", C.c_double), ("clock_dyn", C.c_double),
                 ("max_chain_events", C.c_longlong), ("thinned", C.c_longlong), ("candidates", C.c_longlong),
-                ("accepted", C.c_longlong), ("exact_checks", C.c_longlong), ("violations", C.c_longlong)]
+                ("accepted", C.c_longlong), ("exact_checks", C.c_longlong), ("violations", C.c_longlong),
+                ("rated_owners", C.c_longlong)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -102,7 +102,7 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.lambda_0 = c.n / static_cast<double>(N_K) / c.total_mass;
     r.seed = c.seed;
     r.mlp = c.mlp;
-    r.safety = c.safety;
+    r.safety = c.safety*c.safety_scale;
     r.lam = c.lam.p;
     r.qstart = c.qstart.p;
     return r;
@@ -265,7 +265,7 @@ struct op_run
     wall::time_point begin, chain_begin;
     // learned majorant: in use for this call (switched off for the rest of the call when the exact
     // checks of a wave find the bound too often below the exact rate); subsample phase per group
-    bool learned = false, rate_learned = false;
+    bool learned = false, rate_learned = false, chain_learned = false, table_dirty = false;
     std::vector<int> sub_phase;
 
     op_run(context &ctx, diag_state *d, double h)
@@ -298,7 +298,7 @@ struct op_run
 
     void entries(std::vector<phase_entry> &e, bool chain)
     {
-        if (!chain) rate_learned = learned;
+        (chain ? chain_learned : rate_learned) = learned;
         for (int g : groups)
             if (c.group_cnt[g] > 0)
                 e.push_back(phase_entry{tbase + g, c.group_beg[g], c.group_cnt[g], chain ? steps[g] : 0.0,
@@ -347,13 +347,26 @@ struct op_run
     // merged rate bins of the rated groups (_col_chain.cuh:205-243)
     void after_rate()
     {
-        for (int r = 0; r < RAW_BINS; ++r) stats.pair_evals += (long long)raw_rate[r].count*N_K;
-        // the bound failed on more than 1% of the exact subsample: the swarm reverts to the plain
-        // method for the rest of the call, starting with a full rate pass of the same groups
-        if (rate_learned && wave_ctr[GDX_WAVE_RATE_VIOLATIONS]*100 > wave_ctr[GDX_WAVE_RATE_CHECKS])
+        for (int r = 0; r < RAW_BINS; ++r)
         {
-            learned = false;
-            return;
+            stats.pair_evals += (long long)raw_rate[r].count*N_K;
+            stats.rated_owners += raw_rate[r].count;
+        }
+        // the exact subsample tests the bound: more than 5% above it and the swarm reverts to the plain
+        // method for the rest of the call (a full rate pass of the same groups follows); otherwise the
+        // bound's scale follows the violation rate towards 0.2%
+        if (rate_learned)
+        {
+            const double checks = double(wave_ctr[GDX_WAVE_RATE_CHECKS]);
+            const double viol = double(wave_ctr[GDX_WAVE_RATE_VIOLATIONS]);
+            if (checks >= 64 && viol*20.0 > checks)
+            {
+                learned = false;
+                adapt_bound(2.0);
+                return;
+            }
+            if (viol > 0.002*checks + 2.0*std::sqrt(0.002*checks) + 1.0) adapt_bound(1.25);
+            else if (viol*2000.0 < checks) adapt_bound(0.97);
         }
         // a learned rate pass rated one owner in THIN_STRIDE: bins merge on the population estimate
         for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count*(rate_learned ? THIN_STRIDE : 1);
@@ -412,6 +425,17 @@ struct op_run
         }
         phase = CHAIN;
         before_chain();
+    }
+
+    // scale the swarm's bound (within [1, 16]); the group table is uploaded again before the next rate pass
+    void adapt_bound(double f)
+    {
+        double v = std::min(16.0, std::max(1.0, c.safety_scale*f));
+        if (v != c.safety_scale)
+        {
+            c.safety_scale = v;
+            table_dirty = true;
+        }
     }
 
     void before_chain()
@@ -510,12 +534,17 @@ struct op_run
         stats.owner_updates += due_owners();
         dt_col = duration;
         for (int g : groups) dt_col = std::min(dt_col, steps[g]);
-        if (learned)
+        if (chain_learned)
         {
-            for (int g : groups) sub_phase[g]++;
-            // more than 1% of at least 256 exact checks above the bound: the plain method takes over
-            const unsigned long long checks = wave_ctr[GDX_WAVE_CHECKS], viol = wave_ctr[GDX_WAVE_VIOLATIONS];
-            if (checks >= 256 && viol*100 > checks) learned = false;
+            for (int g : groups)
+            {
+                const int cnt = std::max(0, c.group_cnt[g]), sub = sub_phase[g] % THIN_STRIDE;
+                stats.thinned += cnt - std::max(0, (cnt - sub + THIN_STRIDE - 1) / THIN_STRIDE);
+                sub_phase[g]++;
+            }
+            // candidates above their bound (each restarted on the plain chain) also raise the scale
+            const double checks = double(wave_ctr[GDX_WAVE_CHECKS]), viol = double(wave_ctr[GDX_WAVE_VIOLATIONS]);
+            if (viol > 0.01*checks + 3.0) adapt_bound(1.25);
         }
         sched->advance(groups);
         next_groups();
@@ -539,7 +568,7 @@ struct op_run
         stats.chain_owners = (long long)counters[GDX_CTR_CHAINED];
         stats.pair_evals += (long long)counters[GDX_CTR_PAIRS];
         stats.max_chain_events = (long long)counters[GDX_CTR_LONGEST];
-        stats.thinned = (long long)counters[GDX_CTR_THINNED];
+
         stats.candidates = (long long)counters[GDX_CTR_CANDIDATES];
         stats.accepted = (long long)counters[GDX_CTR_ACCEPTED];
         stats.exact_checks = (long long)counters[GDX_CTR_CHECKS];
@@ -596,6 +625,18 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
     if (!rb.resident) rb.resident = chain_resident_blocks();
     for (op_run *r : runs) r->start(st, rb);
     launch_reset_scratch(st, rb.table.p, ng);
+    // swarms whose bound scale changed: their table entries again, before the next rate pass
+    auto upload_changed = [&](const std::vector<op_run *> &rs) {
+        for (op_run *r : rs)
+        {
+            if (!r->table_dirty) continue;
+            r->table_dirty = false;
+            for (int g = 0; g < LOCAL_GROUPS; ++g)
+                rb.h_table[size_t(r->tbase + g)] = make_gref(r->c, g, rb, r->index, r->tbase);
+            hip_check(hipMemcpyAsync(rb.table.p + r->tbase, rb.h_table.data() + r->tbase,
+                                     sizeof(gdx_gref)*LOCAL_GROUPS, hipMemcpyHostToDevice, st), "group table");
+        }
+    };
     std::vector<phase_entry> e;
     std::vector<op_run *> now;
     while (true)
@@ -622,6 +663,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             hip_check(hipStreamSynchronize(st), "rating pass");
             timer.resolve();
             for (op_run *r : now) r->after_rate();
+            upload_changed(now);
         }
         // ---- chain phase ----
         e.clear();
@@ -647,6 +689,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             hip_check(hipStreamSynchronize(st), "chain pass");
             timer.resolve();
             for (op_run *r : now) r->after_chain();
+            upload_changed(now);
         }
         if (!work) break;
     }

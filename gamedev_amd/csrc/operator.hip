@@ -259,8 +259,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     const gdx_env *env = GP->env;
     const double *hgs = GP->hgs;
     const double lambda_0 = GP->lambda_0;
+    // the exact subsample of a learned swarm checks the bound of the same bath: block totals
+    __shared__ unsigned checked[2];
     if (threadIdx.x < COL_BIN_S*4) (&accd[0][0])[threadIdx.x] = 0.0;
     if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
+    if (threadIdx.x < 2) checked[threadIdx.x] = 0u;
     __syncthreads();
     const int lane = threadIdx.x % WAVE;
     const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : sub + THIN_STRIDE*i);
@@ -328,26 +331,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     }
     const double R = total.r, J1 = total.a, J2 = total.b;
     if (own) GP->cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
-    if constexpr (SUB)
-    {
-        // the exact subsample of a learned swarm checks the bound of the same bath
-        if (sub >= 0)
-        {
-            const bool check = own && live && V;
-            const bool viol = check && !(R <= GP->lam[s]);
-            const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
-            if (lane == 0 && nc)
-            {
-                atomicAdd(GP->counters + GDX_CTR_CHECKS, (unsigned long long)nc);
-                atomicAdd(GP->wave_ctr + GDX_WAVE_RATE_CHECKS, (unsigned long long)nc);
-                if (nv)
-                {
-                    atomicAdd(GP->counters + GDX_CTR_VIOLATIONS, (unsigned long long)nv);
-                    atomicAdd(GP->wave_ctr + GDX_WAVE_RATE_VIOLATIONS, (unsigned long long)nv);
-                }
-            }
-        }
-    }
 
     // moving bounds from this wave's extrema (col_size_bnds.cu:12-21); read after the pair loop so
     // they hold no registers during it
@@ -361,6 +344,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         if (threadIdx.x == 0 && xcd_block() == w.bstart[gi]) { *G.lower = lo; *G.upper = up; }
     }
 
+    if constexpr (SUB)
+    {
+        if (sub >= 0)
+        {
+            const bool check = own && live && V;
+            const bool viol = check && !(R <= GP->lam[s]);
+            const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
+            if (lane == 0 && nc) atomicAdd(&checked[0], nc);
+            if (lane == 0 && nv) atomicAdd(&checked[1], nv);
+        }
+    }
     // controller-bin sums, reduced over each distinct bin of the wavefront
     int sb = own ? size_bin(me.x, lo, up) : -1;
     double weight = me.y*grain_mass(me.x);
@@ -390,6 +384,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         todo &= ~__ballot(in);
     }
     __syncthreads();
+    if constexpr (SUB)
+    {
+        if (threadIdx.x == 255 && checked[0])
+        {
+            atomicAdd(G.counters + GDX_CTR_CHECKS, (unsigned long long)checked[0]);
+            atomicAdd(G.wave_ctr + GDX_WAVE_RATE_CHECKS, (unsigned long long)checked[0]);
+            if (checked[1])
+            {
+                atomicAdd(G.counters + GDX_CTR_VIOLATIONS, (unsigned long long)checked[1]);
+                atomicAdd(G.wave_ctr + GDX_WAVE_RATE_VIOLATIONS, (unsigned long long)checked[1]);
+            }
+        }
+    }
     if (threadIdx.x < COL_BIN_S*4)
     {
         int b = threadIdx.x / 4, f = threadIdx.x % 4;
@@ -440,24 +447,37 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
     const float ls = logf(static_cast<float>(size_i));
     double hg_i = 0.0;
     if constexpr (PARTNER_HG) hg_i = G.hgs[s];
-    const int *row = G.nbr + (long long)s*N_K;
+    const int4 *row = reinterpret_cast<const int4 *>(G.nbr + (long long)s*N_K);
     double W = 0.0, m1 = 0.0, m2 = 0.0;
     float dmin = __builtin_inff(), dmax = -__builtin_inff();
-    for (int k = 0; k < N_K; ++k)
+    // eight partners per step: their slots in two vector loads, then eight independent gathers
+#pragma unroll 1
+    for (int c = 0; c < N_K / 8; ++c)
     {
-        const int j = row[k];
-        if (j < 0) continue;
-        const double2 q = G.res[j];
-        double hg_j = 0.0;
-        if constexpr (PARTNER_HG) hg_j = G.hgs[j];
-        const double w = partner_weight(size_i, mass_i, q, hg_i, hg_j);
-        if (!(w > 0.0) || !isfinite(w)) continue;
-        const float d = logf(static_cast<float>(q.x)) - ls;
-        W += w;
-        m1 += w*d;
-        m2 += w*d*d;
-        dmin = fminf(dmin, d);
-        dmax = fmaxf(dmax, d);
+        const int4 ja = row[2*c], jb = row[2*c + 1];
+        const int jj[8] = {ja.x, ja.y, ja.z, ja.w, jb.x, jb.y, jb.z, jb.w};
+        double2 q[8];
+        double hg[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+        {
+            q[t] = jj[t] >= 0 ? G.res[jj[t]] : make_double2(0.0, 0.0);
+            hg[t] = 0.0;
+            if constexpr (PARTNER_HG) hg[t] = jj[t] >= 0 ? G.hgs[jj[t]] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+        {
+            const double w0 = partner_weight(size_i, mass_i, q[t], hg_i, hg[t]);
+            const bool ok = jj[t] >= 0 && w0 > 0.0 && isfinite(w0);
+            const double w = ok ? w0 : 0.0;
+            const float d = ok ? logf(static_cast<float>(q[t].x)) - ls : 0.0f;
+            W += w;
+            m1 += w*d;
+            m2 += w*d*d;
+            dmin = ok ? fminf(dmin, d) : dmin;
+            dmax = ok ? fmaxf(dmax, d) : dmax;
+        }
     }
     for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
     f[0] = ls;
@@ -663,7 +683,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     bool chain = false, done = false;
     // learned swarms: owners outside the exact subsample draw a candidate time from their bound
     // (thinning); the subsample follows the plain screen (its bound was checked by the rate pass)
-    bool thin = false, candidate = false, check = false, violation = false;
+    bool thin = false, violation = false;
     double2 qs = make_double2(0.0, 0.0);
     double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
     double2 old = make_double2(1.0, 0.0);
@@ -674,7 +694,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         if (thin && (!isfinite(L) || L < 0.0))
         {
             // unusable bound: the owner takes the plain chain from the bath start
-            check = violation = chain = true;
+            violation = chain = true;
         }
         else if (thin && L > 0.0)
         {
@@ -683,7 +703,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
             rng[s] = trial;
             if (t1 < end)
             {
-                candidate = chain = true;
+                chain = true;
                 qs = make_double2(t1, L);
             }
         }
@@ -745,24 +765,17 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         if (chain) G.queue[pos] = s;
         if (chain && learned) G.qstart[pos] = qs;
     }
-    if (learned)
+    // unusable bounds (rare) are counted here; candidates are counted by the chain, screened owners by
+    // the host
+    if (learned && __ballot(violation))
     {
-        const unsigned n_thin = __popcll(__ballot(thin)), n_cand = __popcll(__ballot(candidate));
-        const unsigned n_check = __popcll(__ballot(check)), n_viol = __popcll(__ballot(violation));
+        const unsigned n_viol = __popcll(__ballot(violation));
         if (lane == 0)
         {
-            if (n_thin) atomicAdd(G.counters + GDX_CTR_THINNED, (unsigned long long)n_thin);
-            if (n_cand) atomicAdd(G.counters + GDX_CTR_CANDIDATES, (unsigned long long)n_cand);
-            if (n_check)
-            {
-                atomicAdd(G.counters + GDX_CTR_CHECKS, (unsigned long long)n_check);
-                atomicAdd(G.wave_ctr + GDX_WAVE_CHECKS, (unsigned long long)n_check);
-            }
-            if (n_viol)
-            {
-                atomicAdd(G.counters + GDX_CTR_VIOLATIONS, (unsigned long long)n_viol);
-                atomicAdd(G.wave_ctr + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
-            }
+            atomicAdd(G.counters + GDX_CTR_CHECKS, (unsigned long long)n_viol);
+            atomicAdd(G.wave_ctr + GDX_WAVE_CHECKS, (unsigned long long)n_viol);
+            atomicAdd(G.counters + GDX_CTR_VIOLATIONS, (unsigned long long)n_viol);
+            atomicAdd(G.wave_ctr + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
         }
     }
     __syncthreads();
@@ -818,7 +831,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     const unsigned long long seg_lead = (N_K == 64) ? 1ull : (N_K == 32) ? 0x0000000100000001ull
                                                                          : 0x0001000100010001ull;
     unsigned long long n_events = 0, n_chain = 0, n_pairs = 0, n_blocks = 0;
-    unsigned n_checks = 0, n_viol = 0, n_acc = 0;
+    unsigned n_checks = 0, n_viol = 0, n_acc = 0, n_cand = 0;
     int longest = 0;
     // thinning candidate of a learned swarm: its first rate evaluation decides at the candidate time
     bool pending = false;
@@ -857,9 +870,10 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 atomicAdd(wave_of + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
             }
             if (n_acc) atomicAdd(ctr_of + GDX_CTR_ACCEPTED, (unsigned long long)n_acc);
+            if (n_cand) atomicAdd(ctr_of + GDX_CTR_CANDIDATES, (unsigned long long)n_cand);
         }
         n_events = n_chain = n_pairs = n_blocks = 0;
-        n_checks = n_viol = n_acc = 0;
+        n_checks = n_viol = n_acc = n_cand = 0;
         longest = 0;
     };
     // the wavefront's current run of queue positions [cb, ce), taken CHAIN_CHUNK at a time from the
@@ -920,6 +934,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                     pending = qs.y > 0.0;
                     time = qs.x;
                     bound = qs.y;
+                    n_cand += pending ? 1u : 0u;
                 }
                 events = 0;
                 active = true;

@@ -7,6 +7,9 @@ fiducial_3d_state:  1e7 representatives by default, full 3D, r uniform on [0.3, 
                     Gaussian about pi/2 with h/r = 0.05 r^0.25, log-normal sizes (median 1e-4,
                     sigma_ln = 1, clipped to [1e-5, 1e-1]), equal-mass representatives, total dust
                     mass = 0.01 x the gas mass 2 pi Sigma_0 (3.0 - 0.3).
+clumped_3d_state:   the clumped stress case: the fiducial background with a vortex overdensity (x10,
+                    width 0.05 r, at r = 1, phi = pi) and a pressure-bump ring (x5, width 0.03, at
+                    r = 1.5), sizes log-normal with sigma_ln = 1.5.
 Velocities (Keplerian plus dispersion) do not enter any collision rate: the reference's relative speed
 is query-local and analytic (inc/swarm/_collision.cuh:275-325), so the state holds none.
 The size variable is the reference's par_size and takes the drawn value.
@@ -48,6 +51,51 @@ def fiducial_3d_state(n: int = 10_000_000, seed: int = 0, r_min: float = 0.3, r_
     w = total_mass/n
     numr = w/grain_mass(s)
     return {"x": phi, "y": r, "z": theta, "size": s, "numr": numr, "total_mass": total_mass}
+
+
+def clumped_3d_state(n: int, seed: int = 0, r_min: float = 0.3, r_max: float = 3.0, sigma_ln: float = 1.5,
+                     vortex=(1.0, math.pi, 0.05, 10.0), ring=(1.5, 0.03, 5.0)) -> dict:
+    """The fiducial 3D background with a Gaussian azimuthal vortex overdensity (centre r, phi, width as a
+    fraction of r, peak factor) and a radial pressure-bump ring (radius, width, peak factor), equal-mass
+    representatives placed by rejection from the background, log-normal sizes of the given spread."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rv, pv, wv, av = vortex
+    rr, wr, ar = ring
+    peak = max(av, ar)
+    r = np.empty(0)
+    phi = np.empty(0)
+    while len(r) < n:
+        m = 2*(n - len(r)) + 1024
+        rc = rng.uniform(r_min, r_max, m)
+        pc = rng.uniform(-math.pi, math.pi, m)
+        dphi = np.angle(np.exp(1j*(pc - pv)))
+        w = wv*rv
+        gv = np.exp(-((rc - rv)**2 + (rv*dphi)**2)/(2*w*w))
+        gr = np.exp(-(rc - rr)**2/(2*wr*wr))
+        f = 1.0 + (av - 1.0)*gv + (ar - 1.0)*gr
+        keep = rng.uniform(0.0, peak, m) < f
+        r = np.concatenate([r, rc[keep]])
+        phi = np.concatenate([phi, pc[keep]])
+    r, phi = r[:n], phi[:n]
+    aspect = 0.05*r**0.25
+    theta = 0.5*math.pi + aspect*rng.standard_normal(n)
+    s = np.clip(1e-4*np.exp(sigma_ln*rng.standard_normal(n)), 1e-5, 1e-1)
+    gas_mass = 2.0*math.pi*1e-2*(r_max - r_min)
+    total_mass = 0.01*gas_mass
+    w = total_mass/n
+    numr = w/grain_mass(s)
+    return {"x": phi, "y": r, "z": theta, "size": s, "numr": numr, "total_mass": total_mass}
+
+
+def ensemble_sizes(count: int, lo: float, hi: float, seed: int = 12345, total: int | None = None) -> list[int]:
+    """Realization sizes floor(2^U[lo, hi]); with `total`, rescaled so that they sum to it."""
+    rng = np.random.default_rng(seed)
+    sizes = [int(2.0**rng.uniform(lo, hi)) for _ in range(count)]
+    if total:
+        scale = total/sum(sizes)
+        sizes = [max(64, int(round(v*scale))) for v in sizes]
+        sizes[-1] += total - sum(sizes)
+    return sizes
 
 
 def to_device(state: dict, device="cuda:0") -> dict:

@@ -91,6 +91,7 @@ typedef struct {
     long long accepted;             /* candidates accepted as events */
     long long exact_checks;         /* owner baths whose exact rate was compared with the majorant */
     long long violations;           /* of those, exact rate above the majorant */
+    long long rated_owners;         /* owner baths rated with the exact K-sum by rate passes */
 } gdx_col_stats;
 
 /* library and model */
