@@ -357,6 +357,7 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=10)
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--autotune", type=int, default=1, help="1: pick the Morton leaf target by timing the epoch")
     ap.add_argument("--clumped", type=int, default=64,
                     help="realizations of the clumped batched leg (0 skips it), totalling --clumped-total")
     ap.add_argument("--clumped-total", type=int, default=10_000_000)
@@ -387,6 +388,10 @@ def main() -> int:
     d = synth.to_device(st, dev)
     op = CollisionOperator("fiducial_3d", args.n, st["total_mass"], seed=0, defer_writeback=True, profile=True)
     op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
+    tune = None
+    if args.autotune:
+        tune = op.autotune((16, 32, 64, 128, 256), repeats=1)
+        log(f"autotuned Morton leaf target {tune['best']}: " + ", ".join(f"{k}: {v:.1f} ms" for k, v in tune["ms"].items()))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     op.geometry()
@@ -487,6 +492,7 @@ def main() -> int:
             "learned": learned,
             "learned_over_plain": learned["learned_over_plain"] if learned else None,
             "geometry_epoch_ms": geometry_ms,
+            "geometry_autotune": tune,
             "updates_per_s_with_geometry_rebuild": with_geo,
             "active_owners": n_active,
             "batched_ensemble": batched,

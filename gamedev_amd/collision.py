@@ -194,6 +194,19 @@ class CollisionOperator:
         self._call(self.lib.gdx_col_get_graph, _np_ptr(out))
         return out
 
+    # ---- search tuning ----
+    def set_leaf_target(self, leaf_target: int) -> None:
+        self._call(self.lib.gdx_col_set_leaf_target, int(leaf_target))
+
+    def autotune(self, candidates=(16, 32, 64, 128, 256), repeats: int = 2) -> dict:
+        """Time the geometry epoch per Morton leaf target, keep the fastest (a valid epoch is left built
+        with it); returns {"best": leaf, "ms": {leaf: ms}}."""
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c), np.float64)
+        best = C.c_int()
+        self._call(self.lib.gdx_col_autotune, _np_ptr(c), len(c), int(repeats), C.byref(best), _np_ptr(ms))
+        return {"best": best.value, "ms": {int(k): float(v) for k, v in zip(c, ms)}}
+
     # ---- learned majorant ----
     def majorant_sample(self):
         """Features (n x 8, float32), partner scale S and exact bath-start rate R per particle at the

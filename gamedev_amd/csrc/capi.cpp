@@ -1172,6 +1172,54 @@ int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, do
     });
 }
 
+int gdx_col_set_leaf_target(gdx_col_handle h, int leaf_target)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(leaf_target >= 1 && leaf_target <= 4096, GDX_COL_ERR_MORTON_ARGS, "leaf target must be in [1, 4096]");
+        h->c.leaf_target = leaf_target;
+        h->c.geometry_valid = h->c.state_loaded = false;
+    });
+}
+
+int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(candidates && count > 0 && repeats > 0, GDX_COL_ERR_ARG, "invalid autotuning arguments");
+        require(c.user.bound, GDX_COL_ERR_STATE, "no particle state is bound");
+        for (int i = 0; i < count; ++i)
+            require(candidates[i] >= 1 && candidates[i] <= 4096, GDX_COL_ERR_MORTON_ARGS, "leaf target must be in [1, 4096]");
+        // the epoch reloads the bound state: a deferred state goes back to the bound arrays first
+        if (c.state_loaded) store_state(c);
+        hip_check(hipStreamSynchronize(c.stream), "autotune");
+        int pick = c.leaf_target;
+        double fastest = 0.0;
+        for (int i = 0; i < count; ++i)
+        {
+            c.leaf_target = candidates[i];
+            geometry_epoch(c);                         // first epoch sizes the buffers
+            double t = 0.0;
+            for (int r = 0; r < repeats; ++r)
+            {
+                hip_check(hipStreamSynchronize(c.stream), "autotune");
+                auto t0 = std::chrono::steady_clock::now();
+                geometry_epoch(c);
+                hip_check(hipStreamSynchronize(c.stream), "autotune");
+                double v = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                t = r ? std::min(t, v) : v;
+            }
+            if (ms) ms[i] = t;
+            if (i == 0 || t < fastest) { fastest = t; pick = candidates[i]; }
+        }
+        c.leaf_target = pick;
+        geometry_epoch(c);
+        hip_check(hipStreamSynchronize(c.stream), "autotune");
+        if (best) *best = pick;
+    });
+}
+
 int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety)
 {
     if (!h) return GDX_COL_ERR_ARG;

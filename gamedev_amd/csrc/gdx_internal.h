@@ -184,6 +184,7 @@ struct context
     uint64_t seed = 0;
     int flags = 0;
     int search = GDX_COL_SEARCH_MORTON;
+    int leaf_target = MORTON_LEAF_TARGET;   // Morton refinement threshold (gdx_col_set_leaf_target, autotuner)
     hipStream_t stream = nullptr;
     bound_state user;
 

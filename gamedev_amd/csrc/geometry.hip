@@ -828,7 +828,7 @@ void geometry_epoch(context &c)
     // root cube of morton_ghost_index::build_index: width 2.0002*Y_MAX centred on the origin
     float root_width = 2.0002f*static_cast<float>(Y_MAX);
     float o = -0.5f*root_width;
-    morton_build(c.index, c.pts.p, c.active.p, n, o, o, o, root_width, DIM, MORTON_LEAF_TARGET, MORTON_MAX_LEVEL,
+    morton_build(c.index, c.pts.p, c.active.p, n, o, o, o, root_width, DIM, c.leaf_target, MORTON_MAX_LEVEL,
                  c.scratch, st);
 
     // slot order: active owners grouped by controller group, Morton order inside each group

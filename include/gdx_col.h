@@ -119,6 +119,16 @@ int gdx_col_check_state(gdx_col_handle h);
 int gdx_col_load_state(gdx_col_handle h);
 int gdx_col_store_state(gdx_col_handle h);
 
+/* Morton refinement threshold of the next geometry epochs (default: the model's MORTON_LEAF_TARGET,
+ * morton_index::build's leaf_target argument, morton_index.cuh:168-229). Neighbour rows, slot order and
+ * every operator result do not depend on it; only the search time does. */
+int gdx_col_set_leaf_target(gdx_col_handle h, int leaf_target);
+/* autotuner (no reference counterpart; the reference fixes MORTON_LEAF_TARGET per model): times the
+ * geometry epoch of the bound state for each candidate leaf target (best of `repeats`, host wall clock
+ * around a synchronised epoch), keeps the fastest, and leaves a valid epoch built with it. A deferred
+ * state is written back first. ms (may be NULL) gets `count` times. */
+int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms);
+
 /* one operator call over `duration`: evolve_local_collisions (inc/swarm/_col_chain.cuh:108-539) */
 int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);
 /* one operator call on each of `count` independent swarms (realizations, each with its own handle,
