@@ -104,6 +104,7 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.mlp = c.mlp;
     r.safety = c.safety*c.safety_scale;
     r.lam = c.lam.p;
+    r.res32 = c.res32.p;
     r.qstart = c.qstart.p;
     return r;
 }

@@ -225,6 +225,7 @@ struct context
     dbuf<int> qevents;         // chain events per queue position, -1 after an error
     dbuf<double2> qstart;      // learned: {candidate time, bound} per queue position
     dbuf<double> lam;          // learned: per-slot bound of the current bath
+    dbuf<float2> res32;        // published {size, number} in FP32 for the majorant's partner gathers
     const void *mlp = nullptr; // learned: packed network of the attached majorant (null: plain)
     double safety = 1.2;
     double safety_scale = 1.0; // learned: adaptive factor on the bound, kept across operator calls

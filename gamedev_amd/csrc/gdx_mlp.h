@@ -48,7 +48,9 @@ __device__ __forceinline__ short bf16_rn(float f)
 }
 __device__ __forceinline__ float bf16_f(short b) { return __uint_as_float(static_cast<unsigned int>(static_cast<unsigned short>(b)) << 16); }
 
-__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+// the hardware exponential and reciprocal (about 1 ulp each): far below the BF16 rounding that
+// follows every hidden activation
+__device__ __forceinline__ float sigmoid(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 __device__ __forceinline__ float silu(float x) { return x*sigmoid(x); }
 __device__ __forceinline__ float silu_grad(float x)
 {

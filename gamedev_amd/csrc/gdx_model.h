@@ -110,6 +110,7 @@ struct gdx_gref
     const void *mlp;                          // packed network of the swarm's majorant
     double safety;                            // bound = safety * S * softplus(net)
     double *lam;                              // per slot: the bound of the current bath
+    float2 *res32;                            // per slot: published {size, number} in FP32 (feature gathers)
     double2 *qstart;                          // per queue position: {candidate time, bound}; bound 0 = plain start
 };
 

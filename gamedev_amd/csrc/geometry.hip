@@ -894,6 +894,7 @@ void load_state(context &c)
     const size_t na = size_t(std::max(1, c.n_active));
     c.st.alloc(na);
     c.res.alloc(na);
+    c.res32.alloc(na);
     c.cached.alloc(na);
     c.rng.alloc(na);
     c.queue.alloc(na);

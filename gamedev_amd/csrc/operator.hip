@@ -194,6 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w)
     {
         double2 v = st[s];
         res[s] = v;
+        G.res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
         // positive doubles order like their bit patterns (col_size_scan.cu:20-24)
         if (v.x > 0 && isfinite(v.x)) { lo = min(lo, dbits(v.x)); hi = max(hi, dbits(v.x)); }
     }
@@ -414,21 +415,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
 // ---------------------------------------------------------------------------------------------
 // learned majorant: features, bound per owner (no reference counterpart)
 // ---------------------------------------------------------------------------------------------
-// Kernel weight of one partner (the factors of _get_col_chain_rate that need no transcendental,
-// _col_rates.cuh:167-223): N_j for the constant kernel, N_j (m_i + m_j) and N_j m_i m_j for the sum
-// and product kernels, N_j (s_i + s_j)^2 for the physical one (over the vertical-overlap root in a
-// vertically integrated disk). The scale S = sum over partners / measure (times lambda_0 for the
-// synthetic kernels) leaves the network the relative speed and outcome factors to predict.
-__device__ __forceinline__ double partner_weight(double size_i, double mass_i, double2 q, double hg_i, double hg_j)
+// Kernel weight of one partner from its FP32 published record (the factors of _get_col_chain_rate
+// that need no transcendental, _col_rates.cuh:167-223): N_j for the constant kernel, N_j (m_i + m_j)
+// and N_j m_i m_j for the sum and product kernels, N_j (s_i + s_j)^2 for the physical one (over the
+// vertical-overlap root in a vertically integrated disk). The scale S = sum over partners / measure
+// (times lambda_0 for the synthetic kernels) leaves the network the relative speed and outcome factors.
+__device__ __forceinline__ double partner_weight(double size_i, double mass_i, float2 q, double hg_i, double hg_j)
 {
-    if constexpr (COAG_KERNEL == 0) { (void)size_i; (void)mass_i; (void)hg_i; (void)hg_j; return q.y; }
-    else if constexpr (COAG_KERNEL == 1) { (void)hg_i; (void)hg_j; return q.y*(mass_i + grain_mass(q.x)); }
-    else if constexpr (COAG_KERNEL == 2) { (void)hg_i; (void)hg_j; return q.y*mass_i*grain_mass(q.x); }
+    const double sj = q.x, nj = q.y;
+    if constexpr (COAG_KERNEL == 0) { (void)size_i; (void)mass_i; (void)hg_i; (void)hg_j; (void)sj; return nj; }
+    else if constexpr (COAG_KERNEL == 1) { (void)hg_i; (void)hg_j; return nj*(mass_i + grain_mass(sj)); }
+    else if constexpr (COAG_KERNEL == 2) { (void)hg_i; (void)hg_j; return nj*mass_i*grain_mass(sj); }
     else
     {
         (void)mass_i;
-        double a = size_i + q.x;
-        double w = q.y*a*a;
+        double a = size_i + sj;
+        double w = nj*a*a;
         if constexpr (PARTNER_HG) w /= sqrt(hg_i*hg_i + hg_j*hg_j);
         else { (void)hg_i; (void)hg_j; }
         return w;
@@ -444,34 +446,40 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
     const double2 me = G.st[s];
     const double m = G.measure[s];
     const double size_i = me.x, mass_i = grain_mass(size_i);
-    const float ls = logf(static_cast<float>(size_i));
+    const float ls = __logf(static_cast<float>(size_i));
     double hg_i = 0.0;
     if constexpr (PARTNER_HG) hg_i = G.hgs[s];
     const int4 *row = reinterpret_cast<const int4 *>(G.nbr + (long long)s*N_K);
     double W = 0.0, m1 = 0.0, m2 = 0.0;
     float dmin = __builtin_inff(), dmax = -__builtin_inff();
-    // eight partners per step: their slots in two vector loads, then eight independent gathers
+    // sixteen partners per step: their slots in four vector loads, then sixteen independent 8-byte
+    // gathers of the FP32 published records
 #pragma unroll 1
-    for (int c = 0; c < N_K / 8; ++c)
+    for (int c = 0; c < N_K / 16; ++c)
     {
-        const int4 ja = row[2*c], jb = row[2*c + 1];
-        const int jj[8] = {ja.x, ja.y, ja.z, ja.w, jb.x, jb.y, jb.z, jb.w};
-        double2 q[8];
-        double hg[8];
+        int jj[16];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
+        for (int v = 0; v < 4; ++v)
         {
-            q[t] = jj[t] >= 0 ? G.res[jj[t]] : make_double2(0.0, 0.0);
+            const int4 j4 = row[4*c + v];
+            jj[4*v] = j4.x; jj[4*v + 1] = j4.y; jj[4*v + 2] = j4.z; jj[4*v + 3] = j4.w;
+        }
+        float2 q[16];
+        double hg[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+        {
+            q[t] = jj[t] >= 0 ? G.res32[jj[t]] : make_float2(0.0f, 0.0f);
             hg[t] = 0.0;
             if constexpr (PARTNER_HG) hg[t] = jj[t] >= 0 ? G.hgs[jj[t]] : 0.0;
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
+        for (int t = 0; t < 16; ++t)
         {
             const double w0 = partner_weight(size_i, mass_i, q[t], hg_i, hg[t]);
             const bool ok = jj[t] >= 0 && w0 > 0.0 && isfinite(w0);
             const double w = ok ? w0 : 0.0;
-            const float d = ok ? logf(static_cast<float>(q[t].x)) - ls : 0.0f;
+            const float d = ok ? __logf(q[t].x) - ls : 0.0f;
             W += w;
             m1 += w*d;
             m2 += w*d*d;

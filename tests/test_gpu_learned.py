@@ -28,9 +28,12 @@ def _operator(model, st, gpu, seed):
 
 
 def _restated_features(model, st, codes, meas, act, lambda_0):
-    """owner_features of the library in numpy: partner weights, log-size statistics and S."""
+    """owner_features of the library in numpy: partner weights from the FP32 published records,
+    log-size statistics and S."""
     s = st["size"]
     N = st["numr"]
+    s32 = s.astype(np.float32).astype(np.float64)
+    N32 = N.astype(np.float32).astype(np.float64)
     n, K = codes.shape
     f = np.zeros((n, 8), np.float64)
     S = np.zeros(n)
@@ -38,9 +41,9 @@ def _restated_features(model, st, codes, meas, act, lambda_0):
     for i in np.nonzero(act)[0]:
         j = codes[i][codes[i] >= 0]//3
         if model == "plain_2d":
-            w = N[j]
+            w = N32[j]
         else:
-            w = N[j]*(s[i] + s[j])**2
+            w = N32[j]*(s[i] + s32[j])**2
         ok = w > 0
         j, w = j[ok], w[ok]
         f[i, 0] = ls[i]
