@@ -159,6 +159,8 @@ def batch_leg(count: int, steps: int, horizon: float, dev, model=None, learned=N
         el, waves, stats = run_batched(ops)
         out["updates_per_s_batched_learned"] = steps*active/el
         out["learned_waves_per_step_mean"] = float(waves.mean())
+        out["learned_waves_per_step_max"] = int(waves.max())
+        out["learned_candidates"] = int(sum(s["candidates"] for s in stats))
         chk = sum(s["exact_checks"] for s in stats)
         out["learned_violation_rate"] = sum(s["violations"] for s in stats)/max(1, chk)
         log(f"batch leg: {out['updates_per_s_batched_learned']:.3e} updates/s batched with the majorant")

@@ -92,6 +92,9 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.raw_rate = rb.raw_rate.p + size_t(tbase + g)*COL_BIN_S;
     r.raw_audit = rb.raw_audit.p + size_t(tbase + g)*COL_BIN_S;
     r.qcount = rb.qcount.p + tbase + g;
+    r.ccount = rb.ccount.p + tbase + g;
+    r.cqueue = c.cqueue.p;
+    r.cstart = c.cstart.p;
     r.scr_min = c.scr_min.p + g;
     r.scr_max = c.scr_max.p + g;
     r.lower = c.lower.p + g;
@@ -677,6 +680,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             hip_check(hipMemsetAsync(rb.raw_audit.p, 0, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st), "zero audit bins");
             hip_check(hipMemsetAsync(rb.err.p, 0, sizeof(int)*nr, st), "zero errors");
             hip_check(hipMemsetAsync(rb.qcount.p, 0, sizeof(int)*ng, st), "zero queue counters");
+            hip_check(hipMemsetAsync(rb.ccount.p, 0, sizeof(int)*ng, st), "zero candidate counters");
             hip_check(hipMemsetAsync(rb.wave_ctr.p, 0, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st),
                       "zero wave counters");
             hip_check(hipMemsetAsync(rb.work.p, 0, sizeof(int), st), "zero chain counter");

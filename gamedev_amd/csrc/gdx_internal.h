@@ -107,7 +107,7 @@ struct run_buffers
     // each phase clears and reads them back with one call each
     dbuf<gdx_raw_rate> raw_rate;
     dbuf<gdx_raw_audit> raw_audit;
-    dbuf<int> qcount;
+    dbuf<int> qcount, ccount;
     dbuf<int> err;
     dbuf<unsigned long long> counters, wave_ctr;
     std::vector<gdx_raw_rate> h_rate;
@@ -119,6 +119,7 @@ struct run_buffers
         raw_rate.alloc(size_t(groups)*COL_BIN_S);
         raw_audit.alloc(size_t(groups)*COL_BIN_S);
         qcount.alloc(groups);
+        ccount.alloc(groups);
         err.alloc(swarms);
         counters.alloc(size_t(swarms)*GDX_CTRS);
         wave_ctr.alloc(size_t(swarms)*GDX_WAVE_CTRS);
@@ -223,7 +224,9 @@ struct context
     dbuf<int> queue;           // event owners of one wave, in their group's slot range
     dbuf<double4> qsum;        // chain sums (hazard, j1, j2, jmax) per queue position
     dbuf<int> qevents;         // chain events per queue position, -1 after an error
-    dbuf<double2> qstart;      // learned: {candidate time, bound} per queue position
+    dbuf<double2> qstart;      // learned: {start time, event at the start} per queue position
+    dbuf<int> cqueue;          // learned: thinning candidates per group slot range
+    dbuf<double2> cstart;      // learned: {candidate time, bound} per candidate
     dbuf<double> lam;          // learned: per-slot bound of the current bath
     dbuf<float2> res32;        // published {size, number} in FP32 for the majorant's partner gathers
     const void *mlp = nullptr; // learned: packed network of the attached majorant (null: plain)

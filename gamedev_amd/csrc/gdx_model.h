@@ -111,7 +111,10 @@ struct gdx_gref
     double safety;                            // bound = safety * S * softplus(net)
     double *lam;                              // per slot: the bound of the current bath
     float2 *res32;                            // per slot: published {size, number} in FP32 (feature gathers)
-    double2 *qstart;                          // per queue position: {candidate time, bound}; bound 0 = plain start
+    double2 *qstart;                          // per queue position: {start time, 1 = event at the start}
+    int *cqueue;                              // thinning candidates of the bath, in the group's slot range
+    double2 *cstart;                          // per candidate: {candidate time, bound}
+    int *ccount;                              // the group's candidate count
 };
 
 // operator counters per swarm

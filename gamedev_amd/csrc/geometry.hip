@@ -899,6 +899,8 @@ void load_state(context &c)
     c.rng.alloc(na);
     c.queue.alloc(na);
     c.qstart.alloc(na);
+    c.cqueue.alloc(na);
+    c.cstart.alloc(na);
     c.lam.alloc(na);
     c.qsum.alloc(na);
     if (c.diag) c.evw.alloc(na);

@@ -231,62 +231,40 @@ struct fold3
 };
 __device__ __forceinline__ fold3 add3(const fold3 &x, const fold3 &y) { return {x.r + y.r, x.a + y.a, x.b + y.b}; }
 
-template <bool SUB>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_wave w)
+// exact bath-start sums of one owner per lane (col_bath_rate.cu:13-129): the row streamed in
+// bit-reversed slot order as int4 chunks of four slots, indices one chunk ahead of the partner loads;
+// chunk sums are level-2 partials of the reference's upper-half fold and higher levels combine through
+// a carry stack whose branches depend only on the chunk index, so the sums equal the fold bit for bit.
+// cat3 is the lane's LDS slot for the owner's catastrophic moments.
+struct rate_sums
 {
-    __shared__ double accd[COL_BIN_S][4];     // mass, weighted rate, change, second
-    __shared__ int acci[COL_BIN_S][3];        // count, owner count, invalid count
-    // blocks cover the group's rated owners (every owner, or a learned swarm's subsample)
-    const int b = xcd_block();
-    if (b >= w.bstart[w.n]) return;
-    int gi = 0;
-    {
-        int lo = 0, hi = w.n - 1;
-        while (lo < hi)
-        {
-            int mid = (lo + hi + 1) >> 1;
-            if (w.bstart[mid] <= b) lo = mid;
-            else hi = mid - 1;
-        }
-        gi = lo;
-    }
-    const int i = (b - w.bstart[gi])*RATE_OWNERS + int(threadIdx.x);
-    const int sub = SUB ? w.sub[gi] : -1;
-    const gdx_gref *GP = w.table + w.ref[gi];
+    fold3 total;
+    double JM;
+    bool V;
+};
+__device__ __forceinline__ rate_sums owner_rate_sums(const gdx_gref *GP, int s, bool own, double2 me, double m,
+                                                     double *cat3)
+{
     const int *nbr = GP->nbr;
-    const double2 *st = GP->st;
     const double2 *res = GP->res;
-    const double *measure = GP->measure;
     const gdx_env *env = GP->env;
     const double *hgs = GP->hgs;
     const double lambda_0 = GP->lambda_0;
-    // the exact subsample of a learned swarm checks the bound of the same bath: block totals
-    __shared__ unsigned checked[2];
-    if (threadIdx.x < COL_BIN_S*4) (&accd[0][0])[threadIdx.x] = 0.0;
-    if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
-    if (threadIdx.x < 2) checked[threadIdx.x] = 0u;
-    __syncthreads();
-    const int lane = threadIdx.x % WAVE;
-    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : sub + THIN_STRIDE*i);
-    const bool own = s < w.beg[gi] + w.cnt[gi];
-    double2 me = own ? st[s] : make_double2(1.0, 0.0);
-    const double m = own ? measure[s] : 0.0;
     gdx_env e{};
     double hg_i = 0.0;
     if constexpr (COAG_KERNEL == 3) { if (own) e = env[s]; }
     if constexpr (PARTNER_HG) { if (own) hg_i = hgs[s]; }
     const owner_terms t = make_owner(e, me.x, m, hg_i);
     // read only by lanes with a catastrophic pair: kept in LDS rather than in registers
-    __shared__ double cats[BLOCK][3];
     {
         cat_moments c = catastrophic_moments(me.x);
-        cats[threadIdx.x][0] = c.mean;
-        cats[threadIdx.x][1] = c.second;
-        cats[threadIdx.x][2] = c.maximum;
+        cat3[0] = c.mean;
+        cat3[1] = c.second;
+        cat3[2] = c.maximum;
     }
-    const cat_moments_lds cm{&cats[threadIdx.x][0]};
+    const cat_moments_lds cm{cat3};
     const bool live = own && m > 0.0;
-    const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : w.beg[gi])*N_K);
+    const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : 0)*N_K);
 
     fold3 stack[LOG2K];
     fold3 total{0.0, 0.0, 0.0};
@@ -330,6 +308,50 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         if (!placed) total = part;
         jj = jn;
     }
+    return rate_sums{total, JM, V};
+}
+
+template <bool SUB>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_wave w)
+{
+    __shared__ double accd[COL_BIN_S][4];     // mass, weighted rate, change, second
+    __shared__ int acci[COL_BIN_S][3];        // count, owner count, invalid count
+    // blocks cover the group's rated owners (every owner, or a learned swarm's subsample)
+    const int b = xcd_block();
+    if (b >= w.bstart[w.n]) return;
+    int gi = 0;
+    {
+        int lo = 0, hi = w.n - 1;
+        while (lo < hi)
+        {
+            int mid = (lo + hi + 1) >> 1;
+            if (w.bstart[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        gi = lo;
+    }
+    const int i = (b - w.bstart[gi])*RATE_OWNERS + int(threadIdx.x);
+    const int sub = SUB ? w.sub[gi] : -1;
+    const gdx_gref *GP = w.table + w.ref[gi];
+    const double2 *st = GP->st;
+    const double *measure = GP->measure;
+    // the exact subsample of a learned swarm checks the bound of the same bath: block totals
+    __shared__ unsigned checked[2];
+    if (threadIdx.x < COL_BIN_S*4) (&accd[0][0])[threadIdx.x] = 0.0;
+    if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
+    if (threadIdx.x < 2) checked[threadIdx.x] = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x % WAVE;
+    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : sub + THIN_STRIDE*i);
+    const bool own = s < w.beg[gi] + w.cnt[gi];
+    double2 me = own ? st[s] : make_double2(1.0, 0.0);
+    const double m = own ? measure[s] : 0.0;
+    __shared__ double cats[BLOCK][3];
+    const bool live = own && m > 0.0;
+    const rate_sums rs = owner_rate_sums(GP, s, own, me, m, &cats[threadIdx.x][0]);
+    const fold3 total = rs.total;
+    const double JM = rs.JM;
+    const bool V = rs.V;
     const double R = total.r, J1 = total.a, J2 = total.b;
     if (own) GP->cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
 
@@ -691,8 +713,8 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     bool chain = false, done = false;
     // learned swarms: owners outside the exact subsample draw a candidate time from their bound
     // (thinning); the subsample follows the plain screen (its bound was checked by the rate pass)
-    bool thin = false, violation = false;
-    double2 qs = make_double2(0.0, 0.0);
+    bool thin = false, violation = false, candidate = false;
+    double2 cs = make_double2(0.0, 0.0);
     double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
     double2 old = make_double2(1.0, 0.0);
     if (valid && learned && !exact_owner(w, gi, s))
@@ -711,8 +733,8 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
             rng[s] = trial;
             if (t1 < end)
             {
-                chain = true;
-                qs = make_double2(t1, L);
+                candidate = true;
+                cs = make_double2(t1, L);
             }
         }
     }
@@ -771,7 +793,25 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         base = __builtin_amdgcn_readlane(base, leader);
         const int pos = w.beg[gi] + base + __popcll(ball & ((1ull << lane) - 1ull));
         if (chain) G.queue[pos] = s;
-        if (chain && learned) G.qstart[pos] = qs;
+        if (chain && learned) G.qstart[pos] = make_double2(0.0, 0.0);
+    }
+    // thinning candidates go to the group's candidate list
+    if (learned)
+    {
+        unsigned long long cb = __ballot(candidate);
+        if (cb)
+        {
+            int leader = __builtin_ctzll(cb);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(G.ccount, __popcll(cb));
+            base = __builtin_amdgcn_readlane(base, leader);
+            const int pos = w.beg[gi] + base + __popcll(cb & ((1ull << lane) - 1ull));
+            if (candidate)
+            {
+                G.cqueue[pos] = s;
+                G.cstart[pos] = cs;
+            }
+        }
     }
     // unusable bounds (rare) are counted here; candidates are counted by the chain, screened owners by
     // the host
@@ -788,6 +828,113 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     }
     __syncthreads();
     flush_audit(a, G.raw_audit);
+}
+
+// thinning candidates of learned swarms, one lane per candidate: the exact K-sum at the candidate time
+// (the owner and its frozen partners are unchanged since the bath start, so this is the bath-start
+// rate). Above the bound: the owner restarts the plain chain from the bath start on fresh draws.
+// Otherwise the candidate is an event with probability R / bound (the chain takes it from the
+// candidate time); a rejected candidate continues on R from the candidate time, and only owners whose
+// next draw falls inside the bath enter the chain, which repeats that draw.
+__global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
+{
+    __shared__ double cats[BLOCK][3];
+    __shared__ unsigned tally[4];        // candidates, exact checks, violations, accepted
+    int gi, s0, s1;
+    if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
+    if (w.sub[gi] < 0) return;
+    const gdx_gref *GP = w.table + w.ref[gi];
+    const int beg = w.beg[gi];
+    const int n_cand = *GP->ccount;
+    if (s0 - beg >= n_cand) return;
+    if (threadIdx.x < 4) tally[threadIdx.x] = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x % WAVE;
+    const int pos = s0 + int(threadIdx.x);
+    const bool own = pos - beg < n_cand;
+    const int s = own ? GP->cqueue[pos] : beg;
+    const double2 me = own ? GP->st[s] : make_double2(1.0, 0.0);
+    const double m = own ? GP->measure[s] : 0.0;
+    const rate_sums rs = owner_rate_sums(GP, s, own, me, m, &cats[threadIdx.x][0]);
+    const double R = rs.total.r;
+    bool chain = false, check = false, viol = false, accepted = false;
+    double2 qs = make_double2(0.0, 0.0);
+    if (own)
+    {
+        const double2 c = GP->cstart[pos];
+        const double end = w.dt[gi];
+        const uint32_t pidx = uint32_t(GP->slot_pidx[s]);
+        uint64_t ctr = GP->rng[s];
+        if (!rs.V || !isfinite(R) || R < 0.0)
+        {
+            chain = true;                                  // the chain reports the invalid rate
+        }
+        else if (R > c.y)
+        {
+            check = viol = chain = true;
+        }
+        else
+        {
+            check = true;
+            accepted = col_uniform(pidx, GP->seed, ctr)*c.y < R;
+            if (accepted)
+            {
+                chain = true;
+                qs = make_double2(c.x, 1.0);
+            }
+            else if (R > 0.0)
+            {
+                uint64_t trial = ctr;
+                const double wait = -log(col_uniform(pidx, GP->seed, trial)) / R;
+                if (isfinite(wait) && wait > 0.0 && wait >= end - c.x) ctr = trial;   // no event in the bath
+                else { chain = true; qs = make_double2(c.x, 0.0); }                   // the chain repeats the draw
+            }
+        }
+        GP->rng[s] = ctr;
+    }
+    unsigned long long ball = __ballot(chain);
+    if (ball)
+    {
+        int leader = __builtin_ctzll(ball);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(GP->qcount, __popcll(ball));
+        base = __builtin_amdgcn_readlane(base, leader);
+        const int q = beg + base + __popcll(ball & ((1ull << lane) - 1ull));
+        if (chain)
+        {
+            GP->queue[q] = s;
+            GP->qstart[q] = qs;
+        }
+    }
+    const unsigned nc = __popcll(__ballot(own)), nk = __popcll(__ballot(check));
+    const unsigned nv = __popcll(__ballot(viol)), na = __popcll(__ballot(accepted));
+    if (lane == 0)
+    {
+        if (nc) atomicAdd(&tally[0], nc);
+        if (nk) atomicAdd(&tally[1], nk);
+        if (nv) atomicAdd(&tally[2], nv);
+        if (na) atomicAdd(&tally[3], na);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        if (tally[0])
+        {
+            atomicAdd(GP->counters + GDX_CTR_CANDIDATES, (unsigned long long)tally[0]);
+            atomicAdd(GP->counters + GDX_CTR_PAIRS, (unsigned long long)tally[0]*N_K);
+        }
+        if (tally[1])
+        {
+            atomicAdd(GP->counters + GDX_CTR_CHECKS, (unsigned long long)tally[1]);
+            atomicAdd(GP->wave_ctr + GDX_WAVE_CHECKS, (unsigned long long)tally[1]);
+        }
+        if (tally[2])
+        {
+            atomicAdd(GP->counters + GDX_CTR_VIOLATIONS, (unsigned long long)tally[2]);
+            atomicAdd(GP->wave_ctr + GDX_WAVE_VIOLATIONS, (unsigned long long)tally[2]);
+        }
+        if (tally[3]) atomicAdd(GP->counters + GDX_CTR_ACCEPTED, (unsigned long long)tally[3]);
+    }
 }
 
 // exclusive prefix of the due groups' queue lengths (one block)
@@ -839,11 +986,9 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     const unsigned long long seg_lead = (N_K == 64) ? 1ull : (N_K == 32) ? 0x0000000100000001ull
                                                                          : 0x0001000100010001ull;
     unsigned long long n_events = 0, n_chain = 0, n_pairs = 0, n_blocks = 0;
-    unsigned n_checks = 0, n_viol = 0, n_acc = 0, n_cand = 0;
     int longest = 0;
-    // thinning candidate of a learned swarm: its first rate evaluation decides at the candidate time
-    bool pending = false;
-    double bound = 0.0;
+    // accepted thinning candidate of a learned swarm: its first event happens at its start time
+    bool first_event = false;
     bool more = true, active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
     double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
@@ -867,21 +1012,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             if (longest) atomicMax(wave_of + 0, (unsigned long long)longest);
             if (n_blocks) atomicAdd(wave_of + 1, n_blocks);
             if (n_chain) atomicAdd(wave_of + 2, n_chain);
-            if (n_checks)
-            {
-                atomicAdd(ctr_of + GDX_CTR_CHECKS, (unsigned long long)n_checks);
-                atomicAdd(wave_of + GDX_WAVE_CHECKS, (unsigned long long)n_checks);
-            }
-            if (n_viol)
-            {
-                atomicAdd(ctr_of + GDX_CTR_VIOLATIONS, (unsigned long long)n_viol);
-                atomicAdd(wave_of + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
-            }
-            if (n_acc) atomicAdd(ctr_of + GDX_CTR_ACCEPTED, (unsigned long long)n_acc);
-            if (n_cand) atomicAdd(ctr_of + GDX_CTR_CANDIDATES, (unsigned long long)n_cand);
         }
         n_events = n_chain = n_pairs = n_blocks = 0;
-        n_checks = n_viol = n_acc = n_cand = 0;
         longest = 0;
     };
     // the wavefront's current run of queue positions [cb, ce), taken CHAIN_CHUNK at a time from the
@@ -935,14 +1067,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 // the partners are the published states: fixed for the whole chain
                 q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
                 time = hazard = j1 = j2 = jmax = 0.0;
-                pending = false;
+                first_event = false;
                 if (w.learned)
                 {
                     const double2 qs = G->qstart[pos];
-                    pending = qs.y > 0.0;
                     time = qs.x;
-                    bound = qs.y;
-                    n_cand += pending ? 1u : 0u;
+                    first_event = qs.y > 0.0;
                 }
                 events = 0;
                 active = true;
@@ -967,35 +1097,11 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         n_pairs += (kl == 0) ? N_K : 0;
         bool fin = false;
         int code = 0;
-        bool now = false;     // accepted thinning candidate: the event happens at the current time
+        // accepted thinning candidate: the event happens at the start time, without a wait draw
+        const bool now = first_event;
+        first_event = false;
         if (LAST == -2) { code = 1; fin = true; }
         else if (!isfinite(R) || R < 0.0) { code = 2; fin = true; }
-        else
-        {
-            if (pending)
-            {
-                // exact rate at the candidate time: above the bound, the owner restarts the plain chain
-                // from the bath start on fresh draws; otherwise the candidate is accepted with R / bound
-                // and the compensators take R over [0, candidate)
-                pending = false;
-                ++n_checks;
-                if (R > bound)
-                {
-                    ++n_viol;
-                    time = 0.0;
-                }
-                else
-                {
-                    hazard = R*time;
-                    j1 = J1*time;
-                    j2 = J2*time;
-                    jmax = JM;
-                    now = col_uniform(pidx, seed, ctr)*bound < R;
-                    n_acc += now ? 1u : 0u;
-                }
-            }
-        }
-        if (fin) {}
         else if (R == 0.0) { time = end; fin = true; }
         else
         {
@@ -1230,6 +1336,11 @@ void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix)
     if (!blocks) return;
     k_screen<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch screen");
+    if (w.learned)
+    {
+        k_candidates<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+        hip_check(hipGetLastError(), "launch candidates");
+    }
     k_queue_prefix<<<1, 256, 0, st>>>(w, qprefix);
     hip_check(hipGetLastError(), "launch queue prefix");
 }
