@@ -378,33 +378,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
             if (lane == 0 && nv) atomicAdd(&checked[1], nv);
         }
     }
-    // controller-bin sums, reduced over each distinct bin of the wavefront
+    // controller-bin sums: each owner adds to the block's LDS bins (native FP64 LDS atomics)
     int sb = own ? size_bin(me.x, lo, up) : -1;
     double weight = me.y*grain_mass(me.x);
     bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(R) || R < 0.0 || !isfinite(J1) || J1 < 0.0
         || !isfinite(J2) || J2 < 0.0;
-    unsigned long long todo = __ballot(own);
-    while (todo)
+    if (own)
     {
-        int leader = __builtin_ctzll(todo);
-        int b = __builtin_amdgcn_readlane(sb, leader);
-        bool in = own && sb == b;
-        bool good = in && !bad;
-        double w = good ? weight : 0.0;
-        double a0 = wave_sum(w), a1 = wave_sum(good ? weight*R : 0.0);
-        double a2 = wave_sum(good ? weight*J1 : 0.0), a3 = wave_sum(good ? weight*J2 : 0.0);
-        int n_in = __popcll(__ballot(in)), n_bad = __popcll(__ballot(in && bad));
-        if (lane == leader)
+        atomicAdd(&acci[sb][0], 1);
+        atomicAdd(&acci[sb][1], 1);
+        if (bad)
         {
-            atomicAdd(&acci[b][0], n_in);
-            atomicAdd(&acci[b][1], n_in);
-            atomicAdd(&acci[b][2], n_bad);
-            atomicAdd(&accd[b][0], a0);
-            atomicAdd(&accd[b][1], a1);
-            atomicAdd(&accd[b][2], a2);
-            atomicAdd(&accd[b][3], a3);
+            atomicAdd(&acci[sb][2], 1);
         }
-        todo &= ~__ballot(in);
+        else
+        {
+            atomicAdd(&accd[sb][0], weight);
+            atomicAdd(&accd[sb][1], weight*R);
+            atomicAdd(&accd[sb][2], weight*J1);
+            atomicAdd(&accd[sb][3], weight*J2);
+        }
     }
     __syncthreads();
     if constexpr (SUB)
@@ -611,77 +604,43 @@ __device__ __forceinline__ void flush_audit(const audit_lds &a, gdx_raw_audit *d
 }
 
 // audit terms of one owner per lane (col_audit_bin.cu:31-59): every field but the end mass goes to the
-// start bin bs, the end mass to the end bin be; each distinct bin of the wavefront takes one LDS atomic
-// per field. Without events (EVENTS false) the end bin is the start bin and the event fields are zero.
+// start bin bs, the end mass to the end bin be. Each lane adds its terms to the block's LDS bins with
+// native FP64 LDS atomics (lanes of one bin meet on one address and the LDS serialises them). Without
+// events (EVENTS false) the end bin is the start bin and the event fields are zero.
 template <bool EVENTS>
 __device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int be, double weight,
                                            double probability, double hazard, double j1, double j2, double jmax,
                                            double ev, double growth, bool bad)
 {
-    const int lane = threadIdx.x % WAVE;
-    unsigned long long todo = __ballot(take);
-    while (todo)
+    if (!take) return;
+    atomicAdd(&a.cnt[bs][0], 1);
+    if (bad)
     {
-        int leader = __builtin_ctzll(todo);
-        int b = __builtin_amdgcn_readlane(bs, leader);
-        bool in = take && bs == b;
-        bool good = in && !bad;
-        double w = good ? weight : 0.0;
-        double v0 = wave_sum(w);
-        double v1 = wave_sum(good ? w*probability : 0.0);
-        double v2 = wave_sum(good ? w*hazard : 0.0);
-        double v3 = wave_sum(good ? w*w*probability*(1.0 - probability) : 0.0);
-        double v4 = wave_sum(good ? w*w*hazard : 0.0);
-        double v5 = wave_sum(good ? w*j1 : 0.0);
-        double v6 = wave_sum(good ? w*w*j2 : 0.0);
-        double x0 = wave_max(w);
-        double x1 = wave_max(good ? w*jmax : 0.0);
-        double v7 = 0.0, v8 = 0.0, v9 = 0.0;
-        if constexpr (EVENTS)
-        {
-            v7 = wave_sum(good && ev > 0.0 ? w : 0.0);
-            v8 = wave_sum(good ? w*ev : 0.0);
-            v9 = wave_sum(good ? w*growth : 0.0);
-        }
-        int n_in = __popcll(__ballot(in)), n_bad = __popcll(__ballot(in && bad));
-        if (lane == leader)
-        {
-            atomicAdd(&a.cnt[b][0], n_in);
-            if (n_bad) atomicAdd(&a.cnt[b][1], n_bad);
-            atomicAdd(&a.d[b][0], v0);
-            atomicAdd(&a.d[b][1], v1);
-            atomicAdd(&a.d[b][2], v2);
-            atomicAdd(&a.d[b][3], v3);
-            atomicAdd(&a.d[b][4], v4);
-            atomicAdd(&a.d[b][5], v5);
-            atomicAdd(&a.d[b][6], v6);
-            if (x0 > 0.0) atomicMax(&a.mx[b][0], dbits(x0));
-            if (x1 > 0.0) atomicMax(&a.mx[b][1], dbits(x1));
-            if constexpr (EVENTS)
-            {
-                atomicAdd(&a.d[b][7], v7);
-                atomicAdd(&a.d[b][8], v8);
-                atomicAdd(&a.d[b][9], v9);
-            }
-            else
-            {
-                atomicAdd(&a.d[b][10], v0);
-            }
-        }
-        todo &= ~__ballot(in);
+        atomicAdd(&a.cnt[bs][1], 1);
+        return;
     }
+    const double w = weight;
+    atomicAdd(&a.d[bs][0], w);
+    atomicAdd(&a.d[bs][1], w*probability);
+    atomicAdd(&a.d[bs][2], w*hazard);
+    atomicAdd(&a.d[bs][3], w*w*probability*(1.0 - probability));
+    atomicAdd(&a.d[bs][4], w*w*hazard);
+    atomicAdd(&a.d[bs][5], w*j1);
+    atomicAdd(&a.d[bs][6], w*w*j2);
+    if (w > 0.0) atomicMax(&a.mx[bs][0], dbits(w));
+    const double x1 = w*jmax;
+    if (x1 > 0.0) atomicMax(&a.mx[bs][1], dbits(x1));
     if constexpr (EVENTS)
     {
-        todo = __ballot(take && !bad);
-        while (todo)
-        {
-            int leader = __builtin_ctzll(todo);
-            int b = __builtin_amdgcn_readlane(be, leader);
-            bool in = take && !bad && be == b;
-            double v = wave_sum(in ? weight : 0.0);
-            if (lane == leader) atomicAdd(&a.d[b][10], v);
-            todo &= ~__ballot(in);
-        }
+        if (ev > 0.0) atomicAdd(&a.d[bs][7], w);
+        atomicAdd(&a.d[bs][8], w*ev);
+        atomicAdd(&a.d[bs][9], w*growth);
+        atomicAdd(&a.d[be][10], w);
+    }
+    else
+    {
+        (void)be; (void)ev; (void)growth;
+        atomicAdd(&a.d[bs][10], w);
     }
 }
 
