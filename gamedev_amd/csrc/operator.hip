@@ -649,6 +649,7 @@ __device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int 
 __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 {
     __shared__ audit_lds a;
+    __shared__ int block_off[2][WAVES], block_tot[2], block_base[2];
     int gi, s0, s1;
     if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
@@ -661,6 +662,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     const double end = w.dt[gi];
     const double lo = *G.lower, up = *G.upper;
     clear_audit(a);
+    if (threadIdx.x < 2) block_tot[threadIdx.x] = 0;
     // this group's extrema scratch was consumed by k_bath_rate; clear it for the group's next refresh
     if (threadIdx.x == 0 && xcd_block() == w.bstart[gi]) { *G.scr_min = dbits(__builtin_inf()); *G.scr_max = 0ull; }
     __syncthreads();
@@ -742,35 +744,31 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
         || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
     audit_wave<false>(a, done, bs, bs, weight, probability, hazard, j1, j2, jmax, 0.0, 0.0, bad);
-    // queue the owners with an event, one atomic per wavefront
-    unsigned long long ball = __ballot(chain);
-    if (ball)
+    // owners with an event join the group's queue and thinning candidates its candidate list: wavefront
+    // offsets from LDS, one returning global atomic per block and list (the group counters are shared
+    // by hundreds of blocks on every XCD)
+    const unsigned long long ball = __ballot(chain), cb = __ballot(candidate);
+    const int wv = threadIdx.x / WAVE;
+    if (lane == 0)
     {
-        int leader = __builtin_ctzll(ball);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(G.qcount, __popcll(ball));
-        base = __builtin_amdgcn_readlane(base, leader);
-        const int pos = w.beg[gi] + base + __popcll(ball & ((1ull << lane) - 1ull));
-        if (chain) G.queue[pos] = s;
-        if (chain && learned) G.qstart[pos] = make_double2(0.0, 0.0);
+        block_off[0][wv] = ball ? atomicAdd(&block_tot[0], __popcll(ball)) : 0;
+        block_off[1][wv] = cb ? atomicAdd(&block_tot[1], __popcll(cb)) : 0;
     }
-    // thinning candidates go to the group's candidate list
-    if (learned)
+    __syncthreads();
+    if (threadIdx.x == 0) block_base[0] = block_tot[0] ? atomicAdd(G.qcount, block_tot[0]) : 0;
+    if (threadIdx.x == WAVE) block_base[1] = block_tot[1] ? atomicAdd(G.ccount, block_tot[1]) : 0;
+    __syncthreads();
+    if (chain)
     {
-        unsigned long long cb = __ballot(candidate);
-        if (cb)
-        {
-            int leader = __builtin_ctzll(cb);
-            int base = 0;
-            if (lane == leader) base = atomicAdd(G.ccount, __popcll(cb));
-            base = __builtin_amdgcn_readlane(base, leader);
-            const int pos = w.beg[gi] + base + __popcll(cb & ((1ull << lane) - 1ull));
-            if (candidate)
-            {
-                G.cqueue[pos] = s;
-                G.cstart[pos] = cs;
-            }
-        }
+        const int pos = w.beg[gi] + block_base[0] + block_off[0][wv] + __popcll(ball & ((1ull << lane) - 1ull));
+        G.queue[pos] = s;
+        if (learned) G.qstart[pos] = make_double2(0.0, 0.0);
+    }
+    if (candidate)
+    {
+        const int pos = w.beg[gi] + block_base[1] + block_off[1][wv] + __popcll(cb & ((1ull << lane) - 1ull));
+        G.cqueue[pos] = s;
+        G.cstart[pos] = cs;
     }
     // unusable bounds (rare) are counted here; candidates are counted by the chain, screened owners by
     // the host
@@ -799,6 +797,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
 {
     __shared__ double cats[BLOCK][3];
     __shared__ unsigned tally[4];        // candidates, exact checks, violations, accepted
+    __shared__ int block_off[WAVES], block_tot, block_base;
     int gi, s0, s1;
     if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
     if (w.sub[gi] < 0) return;
@@ -807,6 +806,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     const int n_cand = *GP->ccount;
     if (s0 - beg >= n_cand) return;
     if (threadIdx.x < 4) tally[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) block_tot = 0;
     __syncthreads();
     const int lane = threadIdx.x % WAVE;
     const int pos = s0 + int(threadIdx.x);
@@ -851,19 +851,18 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
         }
         GP->rng[s] = ctr;
     }
-    unsigned long long ball = __ballot(chain);
-    if (ball)
+    // queue appends: wavefront offsets from LDS, one returning global atomic per block
+    const unsigned long long ball = __ballot(chain);
+    const int wv = threadIdx.x / WAVE;
+    if (lane == 0) block_off[wv] = ball ? atomicAdd(&block_tot, __popcll(ball)) : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) block_base = block_tot ? atomicAdd(GP->qcount, block_tot) : 0;
+    __syncthreads();
+    if (chain)
     {
-        int leader = __builtin_ctzll(ball);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(GP->qcount, __popcll(ball));
-        base = __builtin_amdgcn_readlane(base, leader);
-        const int q = beg + base + __popcll(ball & ((1ull << lane) - 1ull));
-        if (chain)
-        {
-            GP->queue[q] = s;
-            GP->qstart[q] = qs;
-        }
+        const int q = beg + block_base + block_off[wv] + __popcll(ball & ((1ull << lane) - 1ull));
+        GP->queue[q] = s;
+        GP->qstart[q] = qs;
     }
     const unsigned nc = __popcll(__ballot(own)), nk = __popcll(__ballot(check));
     const unsigned nv = __popcll(__ballot(viol)), na = __popcll(__ballot(accepted));
