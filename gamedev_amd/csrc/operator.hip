@@ -33,7 +33,7 @@ constexpr int WAVES = BLOCK / WAVE;
 constexpr int PUBLISH_OWNERS = 1024;
 constexpr int RATE_OWNERS = 256;
 constexpr int CHAIN_OWNERS = 256;
-constexpr int CHAIN_CHUNK = 4*NSEG;     // queue positions a chain wavefront takes per counter atomic
+constexpr int CHAIN_CHUNK = 8*NSEG;     // queue positions a chain wavefront takes per counter atomic
 // events one owner may take in one bath: the reference allows COL_EVENT_CAP events per launch and
 // 1e6 continuation launches (inc/swarm/_col_chain.cuh:381-384); exceeding this bound is status 7
 constexpr int EVENT_LIMIT = 1 << 22;
@@ -929,6 +929,10 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
 {
     __shared__ double walk[WAVES][NSEG][N_K];
     __shared__ int qp[CHAIN_PREFIX_LDS + 1];
+    __shared__ unsigned long long end_key, end_sum[5];
+    __shared__ unsigned long long *end_wave;
+    if (threadIdx.x == 0) end_key = 0ull;
+    if (threadIdx.x < 5) end_sum[threadIdx.x] = 0ull;
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     // the queue prefix of the due groups, searched at every refill: LDS when it fits
     const bool in_lds = w.n <= CHAIN_PREFIX_LDS;
@@ -1160,7 +1164,41 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             active = false;
         }
     }
+    // end of the kernel: segments of the block's first swarm add their counters in LDS and one lane
+    // flushes them (every segment flushing at once would queue thousands of global atomics on a few
+    // addresses); segments of another swarm flush directly
+    if (kl == 0 && ctr_of)
+    {
+        const unsigned long long key = reinterpret_cast<unsigned long long>(ctr_of);
+        const unsigned long long prev = atomicCAS(&end_key, 0ull, key);
+        if (prev == 0ull || prev == key)
+        {
+            atomicAdd(&end_sum[0], n_events);
+            atomicAdd(&end_sum[1], n_chain);
+            atomicAdd(&end_sum[2], n_pairs);
+            atomicMax(&end_sum[3], (unsigned long long)longest);
+            atomicAdd(&end_sum[4], n_blocks);
+            n_events = n_chain = n_pairs = n_blocks = 0;
+            longest = 0;
+            if (prev == 0ull) end_wave = wave_of;
+        }
+    }
     flush();
+    __syncthreads();
+    if (threadIdx.x == 0 && end_key)
+    {
+        unsigned long long *c = reinterpret_cast<unsigned long long *>(end_key);
+        if (end_sum[0]) atomicAdd(c + 0, end_sum[0]);
+        if (end_sum[1]) atomicAdd(c + 1, end_sum[1]);
+        if (end_sum[2]) atomicAdd(c + 2, end_sum[2]);
+        if (end_sum[3])
+        {
+            atomicMax(c + 3, end_sum[3]);
+            atomicMax(end_wave + 0, end_sum[3]);
+        }
+        if (end_sum[4]) atomicAdd(end_wave + 1, end_sum[4]);
+        if (end_sum[1]) atomicAdd(end_wave + 2, end_sum[1]);
+    }
 }
 
 // audit of the chained owners, one lane per queue position of each due group (col_audit_bin.cu:13-60)
