@@ -148,7 +148,7 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
         x += (e[i].cnt + 1023) / 1024;
         y += (e[i].cnt + 255) / 256;
         // the same count as rated_count on the device
-        int rated = e[i].sub < 0 ? e[i].cnt : std::max(0, (e[i].cnt - e[i].sub + THIN_STRIDE - 1) / THIN_STRIDE);
+        int rated = e[i].sub < 0 ? e[i].cnt : thin_rated(e[i].cnt, e[i].sub);
         z += (rated + 255) / 256;
         u += e[i].sub < 0 ? 0 : (e[i].cnt + 255) / 256;
         learned |= e[i].sub >= 0;
@@ -543,7 +543,7 @@ struct op_run
             for (int g : groups)
             {
                 const int cnt = std::max(0, c.group_cnt[g]), sub = sub_phase[g] % THIN_STRIDE;
-                stats.thinned += cnt - std::max(0, (cnt - sub + THIN_STRIDE - 1) / THIN_STRIDE);
+                stats.thinned += cnt - thin_rated(cnt, sub);
                 sub_phase[g]++;
             }
             // candidates above their bound (each restarted on the plain chain) also raise the scale

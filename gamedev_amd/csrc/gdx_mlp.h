@@ -39,13 +39,8 @@ struct pack
     float mu[F], inv_sigma[F];
 };
 
-// round to nearest even (finite inputs)
-__device__ __forceinline__ short bf16_rn(float f)
-{
-    unsigned int u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<short>(u >> 16);
-}
+// round to nearest even (finite inputs): the hardware conversion (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ short bf16_rn(float f) { return __builtin_bit_cast(short, static_cast<__bf16>(f)); }
 __device__ __forceinline__ float bf16_f(short b) { return __uint_as_float(static_cast<unsigned int>(static_cast<unsigned short>(b)) << 16); }
 
 // the hardware exponential and reciprocal (about 1 ulp each): far below the BF16 rounding that

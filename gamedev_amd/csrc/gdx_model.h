@@ -129,8 +129,18 @@ enum
     GDX_WAVE_LONGEST, GDX_WAVE_BLOCKS, GDX_WAVE_CHAINED, GDX_WAVE_CHECKS, GDX_WAVE_VIOLATIONS,
     GDX_WAVE_RATE_CHECKS, GDX_WAVE_RATE_VIOLATIONS, GDX_WAVE_CTRS
 };
-// learned swarms rate one owner in THIN_STRIDE exactly per bath (the controller and audit subsample)
+// learned swarms rate one run of THIN_RUN consecutive slots in every THIN_STRIDE runs exactly per bath
+// (the controller and audit subsample; consecutive slots keep the accesses coalesced)
 constexpr int THIN_STRIDE = 8;
+constexpr int THIN_RUN = 32;
+// exactly rated owners among the first cnt slots of a group for subsample phase sub
+constexpr int thin_rated(int cnt, int sub)
+{
+    return (cnt / (THIN_RUN*THIN_STRIDE))*THIN_RUN
+        + (cnt % (THIN_RUN*THIN_STRIDE) - sub*THIN_RUN < 0 ? 0
+           : (cnt % (THIN_RUN*THIN_STRIDE) - sub*THIN_RUN > THIN_RUN ? THIN_RUN
+              : cnt % (THIN_RUN*THIN_STRIDE) - sub*THIN_RUN));
+}
 
 // the due groups of one phase of a run, in device memory: group-table index, owner slot range in the
 // group's swarm, first block of each group for the launching kernel's owners per block, bath end

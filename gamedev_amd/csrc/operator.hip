@@ -63,13 +63,13 @@ __device__ __forceinline__ int rated_count(const gdx_wave &w, int gi)
 {
     const int sub = w.learned ? w.sub[gi] : -1;
     const int cnt = w.cnt[gi];
-    return sub < 0 ? cnt : max(0, (cnt - sub + THIN_STRIDE - 1) / THIN_STRIDE);
+    return sub < 0 ? cnt : thin_rated(cnt, sub);
 }
 __device__ __forceinline__ bool exact_owner(const gdx_wave &w, int gi, int s)
 {
     if (!w.learned) return true;
     const int sub = w.sub[gi];
-    return sub < 0 || (s - w.beg[gi]) % THIN_STRIDE == sub;
+    return sub < 0 || ((s - w.beg[gi]) / THIN_RUN) % THIN_STRIDE == sub;
 }
 
 // wavefront reductions of a double through DPP lane moves (no LDS): quad exchanges, row shifts by 4
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     if (threadIdx.x < 2) checked[threadIdx.x] = 0u;
     __syncthreads();
     const int lane = threadIdx.x % WAVE;
-    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : sub + THIN_STRIDE*i);
+    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : ((i / THIN_RUN)*THIN_STRIDE + sub)*THIN_RUN + i % THIN_RUN);
     const bool own = s < w.beg[gi] + w.cnt[gi];
     double2 me = own ? st[s] : make_double2(1.0, 0.0);
     const double m = own ? measure[s] : 0.0;
@@ -435,18 +435,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
 // and N_j m_i m_j for the sum and product kernels, N_j (s_i + s_j)^2 for the physical one (over the
 // vertical-overlap root in a vertically integrated disk). The scale S = sum over partners / measure
 // (times lambda_0 for the synthetic kernels) leaves the network the relative speed and outcome factors.
-__device__ __forceinline__ double partner_weight(double size_i, double mass_i, float2 q, double hg_i, double hg_j)
+__device__ __forceinline__ float partner_weight(float size_i, float mass_i, float2 q, float hg_i, float hg_j)
 {
-    const double sj = q.x, nj = q.y;
+    const float sj = q.x, nj = q.y;
+    constexpr float MASS = static_cast<float>(PI*RHO_0/6.0);
     if constexpr (COAG_KERNEL == 0) { (void)size_i; (void)mass_i; (void)hg_i; (void)hg_j; (void)sj; return nj; }
-    else if constexpr (COAG_KERNEL == 1) { (void)hg_i; (void)hg_j; return nj*(mass_i + grain_mass(sj)); }
-    else if constexpr (COAG_KERNEL == 2) { (void)hg_i; (void)hg_j; return nj*mass_i*grain_mass(sj); }
+    else if constexpr (COAG_KERNEL == 1) { (void)hg_i; (void)hg_j; return nj*(mass_i + MASS*sj*sj*sj); }
+    else if constexpr (COAG_KERNEL == 2) { (void)hg_i; (void)hg_j; return nj*mass_i*(MASS*sj*sj*sj); }
     else
     {
         (void)mass_i;
-        double a = size_i + sj;
-        double w = nj*a*a;
-        if constexpr (PARTNER_HG) w /= sqrt(hg_i*hg_i + hg_j*hg_j);
+        const float a = size_i + sj;
+        float w = nj*a*a;
+        if constexpr (PARTNER_HG) w /= sqrtf(hg_i*hg_i + hg_j*hg_j);
         else { (void)hg_i; (void)hg_j; }
         return w;
     }
@@ -460,12 +461,13 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
 {
     const double2 me = G.st[s];
     const double m = G.measure[s];
-    const double size_i = me.x, mass_i = grain_mass(size_i);
-    const float ls = __logf(static_cast<float>(size_i));
-    double hg_i = 0.0;
-    if constexpr (PARTNER_HG) hg_i = G.hgs[s];
+    const double size_i = me.x;
+    const float sf = static_cast<float>(size_i), mass_i = static_cast<float>(grain_mass(size_i));
+    const float ls = __logf(sf);
+    float hg_i = 0.0f;
+    if constexpr (PARTNER_HG) hg_i = static_cast<float>(G.hgs[s]);
     const int4 *row = reinterpret_cast<const int4 *>(G.nbr + (long long)s*N_K);
-    double W = 0.0, m1 = 0.0, m2 = 0.0;
+    float W = 0.0f, m1 = 0.0f, m2 = 0.0f;
     float dmin = __builtin_inff(), dmax = -__builtin_inff();
     // sixteen partners per step: their slots in four vector loads, then sixteen independent 8-byte
     // gathers of the FP32 published records
@@ -480,20 +482,20 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
             jj[4*v] = j4.x; jj[4*v + 1] = j4.y; jj[4*v + 2] = j4.z; jj[4*v + 3] = j4.w;
         }
         float2 q[16];
-        double hg[16];
+        float hg[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t)
         {
             q[t] = jj[t] >= 0 ? G.res32[jj[t]] : make_float2(0.0f, 0.0f);
-            hg[t] = 0.0;
-            if constexpr (PARTNER_HG) hg[t] = jj[t] >= 0 ? G.hgs[jj[t]] : 0.0;
+            hg[t] = 0.0f;
+            if constexpr (PARTNER_HG) hg[t] = jj[t] >= 0 ? static_cast<float>(G.hgs[jj[t]]) : 0.0f;
         }
 #pragma unroll
         for (int t = 0; t < 16; ++t)
         {
-            const double w0 = partner_weight(size_i, mass_i, q[t], hg_i, hg[t]);
-            const bool ok = jj[t] >= 0 && w0 > 0.0 && isfinite(w0);
-            const double w = ok ? w0 : 0.0;
+            const float w0 = partner_weight(sf, mass_i, q[t], hg_i, hg[t]);
+            const bool ok = jj[t] >= 0 && w0 > 0.0f && isfinite(w0);
+            const float w = ok ? w0 : 0.0f;
             const float d = ok ? __logf(q[t].x) - ls : 0.0f;
             W += w;
             m1 += w*d;
@@ -504,11 +506,11 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
     }
     for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
     f[0] = ls;
-    if (W > 0.0)
+    if (W > 0.0f)
     {
-        double mean = m1 / W;
-        f[2] = static_cast<float>(mean);
-        f[3] = static_cast<float>(sqrt(fmax(0.0, m2 / W - mean*mean)));
+        const float mean = m1 / W;
+        f[2] = mean;
+        f[3] = sqrtf(fmaxf(0.0f, m2 / W - mean*mean));
         f[4] = dmin;
         f[5] = dmax;
     }
@@ -519,8 +521,8 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
         f[6] = logf(static_cast<float>(e.radial));
         f[7] = static_cast<float>(e.Z*e.omega / e.cs);
     }
-    if (!(m > 0.0) || !(W > 0.0)) return 0.0;
-    return (COAG_KERNEL == 3 ? W : G.lambda_0*W) / m;
+    if (!(m > 0.0) || !(W > 0.0f)) return 0.0;
+    return (COAG_KERNEL == 3 ? double(W) : G.lambda_0*double(W)) / m;
 }
 
 // bound of every due owner of a learned swarm: features per lane, then the network over the

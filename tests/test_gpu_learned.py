@@ -28,8 +28,8 @@ def _operator(model, st, gpu, seed):
 
 
 def _restated_features(model, st, codes, meas, act, lambda_0):
-    """owner_features of the library in numpy: partner weights from the FP32 published records,
-    log-size statistics and S."""
+    """owner_features of the library in numpy (float64 sums): partner weights from the FP32 published
+    records, log-size statistics and S."""
     s = st["size"]
     N = st["numr"]
     s32 = s.astype(np.float32).astype(np.float64)
@@ -72,9 +72,11 @@ def test_samples_match_restatement_and_oracle_rates(gpu, oracles, model):
     lambda_0 = n/op.info["n_k"]/st["total_mass"]
     f_ref, S_ref = _restated_features(model, st, codes, meas, act, lambda_0)
     a = act == 1
-    np.testing.assert_allclose(S[a], S_ref[a], rtol=1e-12, atol=0)
+    # the library sums the weights in FP32 in its row order
+    np.testing.assert_allclose(S[a], S_ref[a], rtol=2e-6, atol=0)
     cols = [0, 2, 3, 4, 5]
-    np.testing.assert_allclose(feats[a][:, cols], f_ref[a][:, cols], rtol=1e-5, atol=2e-5)
+    # FP32 sums in the library (the spread cancels in FP32), float64 here
+    np.testing.assert_allclose(feats[a][:, cols], f_ref[a][:, cols], rtol=1e-5, atol=5e-4)
     if model == "plain_2d":
         assert np.all(feats[:, [1, 6, 7]] == 0)
     R_ref = oracles[model].bath_rates(st["y"], st["z"], st["size"], st["numr"], act, codes, meas, lambda_0)[:, 0]
