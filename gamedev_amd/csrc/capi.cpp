@@ -654,9 +654,10 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
         if (!now.empty())
         {
             work = true;
-            hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st), "zero rate bins");
-            hip_check(hipMemsetAsync(rb.wave_ctr.p, 0, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st),
-                      "zero wave counters");
+            zero_ranges z;
+            z.add(rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng);
+            z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
+            launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
             timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024); });
             timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
@@ -677,13 +678,14 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
         if (!now.empty())
         {
             work = true;
-            hip_check(hipMemsetAsync(rb.raw_audit.p, 0, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st), "zero audit bins");
-            hip_check(hipMemsetAsync(rb.err.p, 0, sizeof(int)*nr, st), "zero errors");
-            hip_check(hipMemsetAsync(rb.qcount.p, 0, sizeof(int)*ng, st), "zero queue counters");
-            hip_check(hipMemsetAsync(rb.ccount.p, 0, sizeof(int)*ng, st), "zero candidate counters");
-            hip_check(hipMemsetAsync(rb.wave_ctr.p, 0, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st),
-                      "zero wave counters");
-            hip_check(hipMemsetAsync(rb.work.p, 0, sizeof(int), st), "zero chain counter");
+            zero_ranges z;
+            z.add(rb.raw_audit.p, sizeof(gdx_raw_audit)*COL_BIN_S*ng);
+            z.add(rb.err.p, sizeof(int)*nr);
+            z.add(rb.qcount.p, sizeof(int)*ng);
+            z.add(rb.ccount.p, sizeof(int)*ng);
+            z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
+            z.add(rb.work.p, sizeof(int));
+            launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
             timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256, rb.qprefix.p); });
             timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.work.p, rb.resident); });

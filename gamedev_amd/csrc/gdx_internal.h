@@ -59,6 +59,38 @@ struct dbuf
     }
 };
 
+// page-locked host array: device-to-host copies land without a staging copy
+template <typename T>
+struct pinned
+{
+    T *p = nullptr;
+    size_t n = 0;
+    pinned() = default;
+    pinned(const pinned &) = delete;
+    pinned &operator=(const pinned &) = delete;
+    ~pinned() { if (p) (void)hipHostFree(p); }
+    void resize(size_t count)
+    {
+        if (count <= n && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&p), sizeof(T)*std::max<size_t>(count, 1)), "pinned host buffer");
+        n = count;
+    }
+    T *data() { return p; }
+    const T *data() const { return p; }
+};
+
+// up to eight device ranges cleared by one launch (byte counts are multiples of 4)
+struct zero_ranges
+{
+    void *p[8];
+    unsigned long long bytes[8];
+    int n = 0;
+    void add(void *ptr, size_t b) { p[n] = ptr; bytes[n] = b; ++n; }
+};
+
 // particle-order view of one bound state field: element i lives at base + i*stride bytes
 struct field
 {
@@ -110,10 +142,10 @@ struct run_buffers
     dbuf<int> qcount, ccount;
     dbuf<int> err;
     dbuf<unsigned long long> counters, wave_ctr;
-    std::vector<gdx_raw_rate> h_rate;
-    std::vector<gdx_raw_audit> h_audit;
-    std::vector<int> h_err;
-    std::vector<unsigned long long> h_counters, h_wave;
+    pinned<gdx_raw_rate> h_rate;
+    pinned<gdx_raw_audit> h_audit;
+    pinned<int> h_err;
+    pinned<unsigned long long> h_counters, h_wave;
     void size_run(int groups, int swarms)
     {
         raw_rate.alloc(size_t(groups)*COL_BIN_S);
@@ -272,6 +304,7 @@ void launch_publish(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix);
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_zero(hipStream_t st, const zero_ranges &z);
 void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, double *rate);
 // ---- majorant.hip ---------------------------------------------------------------------------
 const void *majorant_pack(gdx_col_majorant m);

@@ -1315,6 +1315,28 @@ void launch_event_sum(context &c, hipStream_t st)
     hip_check(hipGetLastError(), "launch event sum");
 }
 
+__global__ __launch_bounds__(256) void k_zero(zero_ranges z)
+{
+    for (int r = 0; r < z.n; ++r)
+    {
+        unsigned int *p = static_cast<unsigned int *>(z.p[r]);
+        const unsigned long long words = z.bytes[r] / 4;
+        for (unsigned long long i = (unsigned long long)blockIdx.x*blockDim.x + threadIdx.x; i < words;
+             i += (unsigned long long)gridDim.x*blockDim.x)
+            p[i] = 0u;
+    }
+}
+
+void launch_zero(hipStream_t st, const zero_ranges &z)
+{
+    unsigned long long words = 0;
+    for (int r = 0; r < z.n; ++r) words = std::max(words, z.bytes[r] / 4);
+    if (!words) return;
+    const int blocks = int(std::min<unsigned long long>(256, (words + 255) / 256));
+    k_zero<<<blocks, 256, 0, st>>>(z);
+    hip_check(hipGetLastError(), "launch zero");
+}
+
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks)
 {
     if (!blocks) return;
