@@ -702,6 +702,57 @@ GDX_FN pair_term eval_pair_terms(const owner_terms &t, const CM &cm, const partn
     return p;
 }
 
+// the rate factor of outcome_moments_terms alone (callers that keep no compensators): the same
+// operations, without the logarithms of the moments
+GDX_FN double outcome_factor(const owner_terms &t, double sj, bool high_speed)
+{
+    const double si = t.size_i;
+    double ratio = div_rn(sj, si, t.inv_size);
+    double q = ratio*ratio*ratio;
+    double Gp = sticking_packet(q);
+    if (!high_speed) return Gp == 1.0 ? 1.0 : 1.0 / Gp;
+    if (q <= 0.1)
+    {
+        double remnant = Gp == 1.0 ? (1.0 - q) : (1.0 - q) / Gp;
+        return remnant + q;
+    }
+    return 1.0;
+}
+
+// eval_pair_terms without the jump moments: the same rate bit for bit
+GDX_FN pair_term eval_pair_rate(const owner_terms &t, const partner &q, double lambda_0)
+{
+    pair_term p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!q.valid) return p;
+    const double size_i = t.size_i, size_j = q.size_j, numr_j = q.numr_j;
+    double vrel = 0.0;
+    double rate;
+    if constexpr (COAG_KERNEL == 0)
+    {
+        rate = lambda_0*numr_j;
+    }
+    else if constexpr (COAG_KERNEL == 1)
+    {
+        rate = lambda_0*numr_j*(grain_mass(size_i) + grain_mass(size_j));
+    }
+    else if constexpr (COAG_KERNEL == 2)
+    {
+        rate = lambda_0*numr_j*grain_mass(size_i)*grain_mass(size_j);
+    }
+    else
+    {
+        vrel = vrel_pair_terms(t, size_j);
+        rate = numr_j*vrel*PI*(size_i + size_j)*(size_i + size_j) / 4.0;
+        if constexpr (N_Z == 1) rate /= sqrt(2.0*PI*(t.hg*t.hg + q.hg_j*q.hg_j));
+    }
+    double r = div_rn(rate, t.m, t.inv_m);
+    r *= outcome_factor(t, size_j, vrel >= V_FRAG);
+    p.rate = r;
+    p.vrel = vrel;
+    p.size_j = size_j;
+    return p;
+}
+
 // moving logarithmic size bin (inc/swarm/_col_sizes.cuh:12-21)
 GDX_FN int size_bin(double size, double lower, double upper)
 {

@@ -953,6 +953,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     int longest = 0;
     // accepted thinning candidate of a learned swarm: its first event happens at its start time
     bool first_event = false;
+    bool moments = true;
     bool more = true, active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
     double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
@@ -1032,6 +1033,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
                 time = hazard = j1 = j2 = jmax = 0.0;
                 first_event = false;
+                moments = exact_owner(w, gi, s);
                 if (w.learned)
                 {
                     const double2 qs = G->qstart[pos];
@@ -1049,7 +1051,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             continue;
         }
         if (!active) continue;
-        pair_term p = eval_pair_terms(t, cm, q, lambda_0);
+        // owners outside a learned swarm's audited subsample keep no compensators: rates alone
+        const pair_term p = moments ? eval_pair_terms(t, cm, q, lambda_0) : eval_pair_rate(t, q, lambda_0);
         const bool bad_rate = !isfinite(p.rate) || p.rate < 0.0;
         double R = seg_fold<false>(p.rate);
         double J1 = seg_fold<false>(p.j1);

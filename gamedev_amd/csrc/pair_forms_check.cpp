@@ -129,7 +129,7 @@ int main(int argc, char **argv)
     }
     long long trials = argc > 1 ? std::atoll(argv[1]) : 1000000;
     lcg g{argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1ull};
-    long long bad[5] = {0, 0, 0, 0, 0}, reg[6] = {0, 0, 0, 0, 0, 0}, branch[3] = {0, 0, 0};
+    long long bad[6] = {0, 0, 0, 0, 0, 0}, reg[6] = {0, 0, 0, 0, 0, 0}, branch[3] = {0, 0, 0};
     for (long long n = 0; n < trials; ++n)
     {
         double y = Y_MIN + (Y_MAX - Y_MIN)*g.next();
@@ -150,6 +150,13 @@ int main(int argc, char **argv)
         cat_moments cm = catastrophic_moments(size_i);
         partner q{size_j, numr_j, hg_j, true};
         pair_term b = eval_pair_terms(t, cm, q, lambda_0);
+        // the rate-only form used for owners without compensators gives the same rate and speed
+        pair_term c = eval_pair_rate(t, q, lambda_0);
+        if ((bits(c.rate) != bits(b.rate) && !(std::isnan(c.rate) && std::isnan(b.rate))) || bits(c.vrel) != bits(b.vrel))
+        {
+            if (bad[5] < 3) std::printf("rate-only mismatch: size_i %.17g size_j %.17g\n", size_i, size_j);
+            bad[5]++;
+        }
 
         if constexpr (COAG_KERNEL == 3) reg[regime(e, size_i, size_j)]++;
         double ratio = size_j / size_i;
@@ -170,6 +177,6 @@ int main(int argc, char **argv)
     std::printf("model %s trials %lld\n", GDX_MODEL_NAME, trials);
     std::printf("regimes %lld %lld %lld %lld %lld %lld\n", reg[0], reg[1], reg[2], reg[3], reg[4], reg[5]);
     std::printf("branches %lld %lld %lld\n", branch[0], branch[1], branch[2]);
-    std::printf("mismatches %lld %lld %lld %lld %lld\n", bad[0], bad[1], bad[2], bad[3], bad[4]);
-    return (bad[0] | bad[1] | bad[2] | bad[3] | bad[4]) ? 1 : 0;
+    std::printf("mismatches %lld %lld %lld %lld %lld %lld\n", bad[0], bad[1], bad[2], bad[3], bad[4], bad[5]);
+    return (bad[0] | bad[1] | bad[2] | bad[3] | bad[4] | bad[5]) ? 1 : 0;
 }

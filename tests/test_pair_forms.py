@@ -16,7 +16,7 @@ def test_hoisted_pair_forms_bit_exact(model, trials):
     out = subprocess.run([str(exe), str(trials), "7"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = dict(l.split(" ", 1) for l in out.stdout.strip().splitlines() if " " in l)
-    assert lines["mismatches"].split() == ["0"]*5
+    assert lines["mismatches"].split() == ["0"]*6
     if model == "fiducial_3d":
         # every turbulence regime and every outcome branch was exercised
         assert all(int(v) > 1000 for v in lines["regimes"].split())
