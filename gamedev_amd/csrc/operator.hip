@@ -1028,12 +1028,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 if constexpr (COAG_KERNEL == 3) e = G->env[s];
                 if constexpr (PARTNER_HG) hg_i = G->hgs[s];
                 t = make_owner(e, cur.x, G->measure[s], hg_i);
-                cm = catastrophic_moments(cur.x);
+                moments = exact_owner(w, gi, s);
+                // the catastrophic moments enter only the jump moments
+                if (moments) cm = catastrophic_moments(cur.x);
                 // the partners are the published states: fixed for the whole chain
                 q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
                 time = hazard = j1 = j2 = jmax = 0.0;
                 first_event = false;
-                moments = exact_owner(w, gi, s);
                 if (w.learned)
                 {
                     const double2 qs = G->qstart[pos];
@@ -1142,7 +1143,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                         double mass_after = numr_i*size_new*size_new*size_new;
                         double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
                         owner_resize(t, size_new);
-                        cm = catastrophic_moments(size_new);
+                        if (moments) cm = catastrophic_moments(size_new);
                         if (!isfinite(size_new) || !isfinite(numr_i) || !(size_new > 0.0) || !(numr_i > 0.0)
                             || fabs(mass_after - mass_before) > 2.0e-12*mass_scale)
                         { code = 6; fin = true; }
