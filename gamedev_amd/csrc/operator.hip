@@ -463,7 +463,8 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
     const double m = G.measure[s];
     const double size_i = me.x;
     const float sf = static_cast<float>(size_i), mass_i = static_cast<float>(grain_mass(size_i));
-    const float ls = __logf(sf);
+    // size ratios in log2 (the hardware logarithm; sizes are normal floats), converted to ln at the end
+    const float ls = __builtin_amdgcn_logf(sf);
     float hg_i = 0.0f;
     if constexpr (PARTNER_HG) hg_i = static_cast<float>(G.hgs[s]);
     const int4 *row = reinterpret_cast<const int4 *>(G.nbr + (long long)s*N_K);
@@ -486,9 +487,11 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
 #pragma unroll
         for (int t = 0; t < 16; ++t)
         {
-            q[t] = jj[t] >= 0 ? G.res32[jj[t]] : make_float2(0.0f, 0.0f);
+            // empty slots read slot 0 and are masked below: sixteen loads without branches
+            const int j = max(jj[t], 0);
+            q[t] = G.res32[j];
             hg[t] = 0.0f;
-            if constexpr (PARTNER_HG) hg[t] = jj[t] >= 0 ? static_cast<float>(G.hgs[jj[t]]) : 0.0f;
+            if constexpr (PARTNER_HG) hg[t] = static_cast<float>(G.hgs[j]);
         }
 #pragma unroll
         for (int t = 0; t < 16; ++t)
@@ -496,7 +499,7 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
             const float w0 = partner_weight(sf, mass_i, q[t], hg_i, hg[t]);
             const bool ok = jj[t] >= 0 && w0 > 0.0f && isfinite(w0);
             const float w = ok ? w0 : 0.0f;
-            const float d = ok ? __logf(q[t].x) - ls : 0.0f;
+            const float d = ok ? __builtin_amdgcn_logf(q[t].x) - ls : 0.0f;
             W += w;
             m1 += w*d;
             m2 += w*d*d;
@@ -504,15 +507,16 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
             dmax = ok ? fmaxf(dmax, d) : dmax;
         }
     }
+    constexpr float LN2 = 0.69314718f;
     for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
-    f[0] = ls;
+    f[0] = ls*LN2;
     if (W > 0.0f)
     {
         const float mean = m1 / W;
-        f[2] = mean;
-        f[3] = sqrtf(fmaxf(0.0f, m2 / W - mean*mean));
-        f[4] = dmin;
-        f[5] = dmax;
+        f[2] = mean*LN2;
+        f[3] = sqrtf(fmaxf(0.0f, m2 / W - mean*mean))*LN2;
+        f[4] = dmin*LN2;
+        f[5] = dmax*LN2;
     }
     if constexpr (COAG_KERNEL == 3)
     {
