@@ -548,6 +548,7 @@ __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
     __syncthreads();
     const mlp::pack *P = static_cast<const mlp::pack *>(G.mlp);
     const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+#pragma unroll
     for (int tt = 0; tt < WAVE / 16; ++tt)
     {
         const int row = wv*WAVE + tt*16 + (lane & 15);
