@@ -131,7 +131,7 @@ enum
 };
 // learned swarms rate one run of THIN_RUN consecutive slots in every THIN_STRIDE runs exactly per bath
 // (the controller and audit subsample; consecutive slots keep the accesses coalesced)
-constexpr int THIN_STRIDE = 8;
+constexpr int THIN_STRIDE = 16;
 constexpr int THIN_RUN = 32;
 // exactly rated owners among the first cnt slots of a group for subsample phase sub
 constexpr int thin_rated(int cnt, int sub)
