@@ -110,7 +110,7 @@ _SIGS = {
     "gdx_col_majorant_grad": [P, P, P, I, P, C.POINTER(D)],
     "gdx_col_majorant_train": [P, P, P, C.c_longlong, C.POINTER(TrainConfig), P],
     "gdx_col_majorant_sample": [P, P, P, P],
-    "gdx_col_use_majorant": [P, P, D],
+    "gdx_col_use_majorant": [P, P, D, I],
     "gdx_col_set_leaf_target": [P, I],
     "gdx_col_autotune": [P, P, I, I, P, P],
 }

@@ -219,9 +219,11 @@ class CollisionOperator:
         self._call(self.lib.gdx_col_majorant_sample, _ptr(feats), _ptr(scale), _ptr(rate))
         return feats, scale, rate
 
-    def use_majorant(self, majorant, safety: float = 1.2) -> None:
-        """Draw event times by thinning against the majorant (None: the plain method)."""
-        self._call(self.lib.gdx_col_use_majorant, majorant.h if majorant is not None else None, float(safety))
+    def use_majorant(self, majorant, safety: float = 1.2, min_owners: int = 0) -> None:
+        """Draw event times by thinning against the majorant (None: the plain method). Swarms with fewer
+        than min_owners active owners (0: the library default, 262,144) keep the plain method."""
+        self._call(self.lib.gdx_col_use_majorant, majorant.h if majorant is not None else None, float(safety),
+                   int(min_owners))
         self._majorant = majorant
 
 

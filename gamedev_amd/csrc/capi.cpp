@@ -281,7 +281,7 @@ struct op_run
         if (!(duration > 0) || !std::isfinite(duration))
             throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
         begin = wall::now();
-        learned = c.mlp != nullptr;
+        learned = c.mlp != nullptr && c.n_active >= c.majorant_min_owners;
         sub_phase.assign(LOCAL_GROUPS, 0);
         stats.active_owners = c.n_active;
         dt_col = duration;
@@ -1227,13 +1227,14 @@ int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int rep
     });
 }
 
-int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety)
+int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety, int min_owners)
 {
     if (!h) return GDX_COL_ERR_ARG;
     return guard(&h->c, [&] {
         require(!m || (std::isfinite(safety) && safety >= 1.0), GDX_COL_ERR_ARG, "majorant safety factor must be >= 1");
         h->c.mlp = majorant_pack(m);
         h->c.safety = m ? safety : 1.0;
+        h->c.majorant_min_owners = min_owners > 0 ? min_owners : GDX_COL_MAJORANT_MIN_OWNERS;
     });
 }
 

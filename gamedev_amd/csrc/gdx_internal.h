@@ -264,6 +264,7 @@ struct context
     const void *mlp = nullptr; // learned: packed network of the attached majorant (null: plain)
     double safety = 1.2;
     double safety_scale = 1.0; // learned: adaptive factor on the bound, kept across operator calls
+    int majorant_min_owners = GDX_COL_MAJORANT_MIN_OWNERS;   // smaller swarms keep the plain method
     bool diag = false;         // GDX_COL_DIAGNOSTICS
     dbuf<gdx_event_work> evw;  // per-slot event statistics of the current call
     dbuf<gdx_event_work> ev_sum;

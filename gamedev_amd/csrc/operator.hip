@@ -1040,7 +1040,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
                 time = hazard = j1 = j2 = jmax = 0.0;
                 first_event = false;
-                if (w.learned)
+                // a learned group's queue entries carry their start (plain groups of a mixed wave do not)
+                if (w.learned && w.sub[gi] >= 0)
                 {
                     const double2 qs = G->qstart[pos];
                     time = qs.x;

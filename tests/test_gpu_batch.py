@@ -43,3 +43,36 @@ def test_batch_rejects_duplicates(gpu):
     op, _ = _make(gpu, synth.fiducial_3d_state(2000, seed=1), 1)
     with pytest.raises(GdxError):
         evolve_batch([op, op], 0.05)
+
+
+def test_mixed_learned_and_plain_swarms_in_one_batch(gpu):
+    """A batch whose waves mix learned and plain swarms: the plain swarms give exactly their separate
+    calls' results, and the learned swarm conserves mass (regression: plain queue entries of a mixed
+    wave carry no thinning start)."""
+    import torch
+    from gamedev_amd import Majorant, evolve_batch, synth
+    sizes = [12000, 30000, 9000]
+    states = [synth.fiducial_3d_state(n, seed=40 + i) for i, n in enumerate(sizes)]
+    single = [_make(gpu, st, 200 + i) for i, st in enumerate(states)]
+    batch = [_make(gpu, st, 200 + i) for i, st in enumerate(states)]
+    # a majorant trained briefly on the learned swarm's own samples
+    f, S, R = batch[1][0].majorant_sample()
+    ok = (S > 0) & (R > 0)
+    m = Majorant("fiducial_3d", seed=1)
+    _, safety = m.fit(f[ok], torch.log(R[ok]/S[ok]).float(), epochs=5, batch=1024)
+    batch[1][0].use_majorant(m, safety, min_owners=1)
+    mass0 = (batch[1][1]["numr"]*batch[1][1]["size"]**3).sum().item()
+    for step in range(3):
+        print(f"[batch] mixed step {step}", flush=True)
+        a = [single[i][0].evolve(0.05) for i in (0, 2)]
+        b = evolve_batch([op for op, _ in batch], 0.05)
+        assert b[1]["thinned"] > 0
+        for sa, sb in zip(a, (b[0], b[2])):
+            for k in ("events", "chain_owners", "waves", "baths", "owner_updates", "pair_evals"):
+                assert sa[k] == sb[k], k
+    for i in (0, 2):
+        assert np.array_equal(single[i][1]["size"].cpu().numpy(), batch[i][1]["size"].cpu().numpy())
+        assert np.array_equal(single[i][1]["numr"].cpu().numpy(), batch[i][1]["numr"].cpu().numpy())
+        assert np.array_equal(single[i][0].rng(), batch[i][0].rng())
+    mass1 = (batch[1][1]["numr"]*batch[1][1]["size"]**3).sum().item()
+    assert abs(mass1/mass0 - 1) < 1e-10

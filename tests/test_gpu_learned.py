@@ -127,7 +127,7 @@ def test_learned_operator_matches_plain_ensemble(gpu):
             print(f"[learned] ensemble {mode} seed {k}", flush=True)
             o, d = _operator("fiducial_3d", st, gpu, seed=1000 + k)
             if mode == "learned":
-                o.use_majorant(m, safety)
+                o.use_majorant(m, safety, min_owners=1)
             ev = 0
             stats = []
             for _ in range(steps):
@@ -174,7 +174,7 @@ def test_failed_bound_falls_back_to_plain(gpu):
     p, norm = m.params()
     p[-1] = -30.0                          # softplus(b3) ~ 1e-13: the bound is far too small
     m.set_params(p, None)
-    op.use_majorant(m, 1.0)
+    op.use_majorant(m, 1.0, min_owners=1)
     M_before = _moments(d)
     s = op.evolve(0.002)
     M_after = _moments(d)
@@ -202,7 +202,7 @@ def test_trained_network_gains_over_untrained_and_plain(gpu):
     for name, mdl in (("plain", None), ("untrained", untrained), ("trained", trained)):
         o, _ = _operator("fiducial_3d", st, gpu, seed=300)
         if mdl is not None:
-            o.use_majorant(mdl, safety if mdl is trained else 1.2)
+            o.use_majorant(mdl, safety if mdl is trained else 1.2, min_owners=1)
         tot = {}
         for _ in range(2):
             s = o.evolve(h)
