@@ -92,6 +92,7 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.raw_rate = rb.raw_rate.p + size_t(tbase + g)*COL_BIN_S;
     r.raw_audit = rb.raw_audit.p + size_t(tbase + g)*COL_BIN_S;
     r.qcount = rb.qcount.p + tbase + g;
+    r.hcount = rb.hcount.p + tbase + g;
     r.ccount = rb.ccount.p + tbase + g;
     r.cqueue = c.cqueue.p;
     r.cstart = c.cstart.p;
@@ -682,9 +683,10 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.raw_audit.p, sizeof(gdx_raw_audit)*COL_BIN_S*ng);
             z.add(rb.err.p, sizeof(int)*nr);
             z.add(rb.qcount.p, sizeof(int)*ng);
+            z.add(rb.hcount.p, sizeof(int)*ng);
             z.add(rb.ccount.p, sizeof(int)*ng);
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
-            z.add(rb.work.p, sizeof(int));
+            z.add(rb.work.p, 2*sizeof(int));
             launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
             timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256, rb.qprefix.p); });

@@ -52,12 +52,18 @@ __device__ __forceinline__ double uniform01(uint32_t pidx, uint64_t seed, uint64
     return (double(m >> 12) + 0.5)*0x1p-52;
 }
 
+// draw `counter` of a particle's stream, without advancing the stream
+__device__ __forceinline__ double col_uniform_at(uint32_t pidx, uint64_t seed, uint64_t counter)
+{
+    return fmin(uniform01(pidx, seed, counter), 0x1.fffffffffffffp-1);
+}
+
 // _get_col_uniform (inc/swarm/_col_rates.cuh:145-149)
 __device__ __forceinline__ double col_uniform(uint32_t pidx, uint64_t seed, uint64_t &counter)
 {
-    double u = uniform01(pidx, seed, counter);
+    double u = col_uniform_at(pidx, seed, counter);
     counter++;
-    return fmin(u, 0x1.fffffffffffffp-1);
+    return u;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -500,6 +506,18 @@ struct owner_terms
     double si, fi, ai, mi, vn, zo, ris, vg_sq, radial, inv_radial, strat, inv_strat, cs, hg;
 };
 
+// _get_stokes (stokes_scaled) with the owner's scalings: the same three correctly rounded divisions,
+// by the held reciprocals
+GDX_FN double stokes_partner(const owner_terms &t, double size)
+{
+    double st = STOKES_0*div_rn(size, S_0, INV_S0);
+#ifndef CONST_ST
+    st = div_rn(st, t.radial, t.inv_radial);
+    st = div_rn(st, t.strat, t.inv_strat);
+#endif
+    return st;
+}
+
 // the owner-size factors, recomputed after each event from the position factors already held
 GDX_FN void owner_resize(owner_terms &t, double size_i)
 {
@@ -507,7 +525,7 @@ GDX_FN void owner_resize(owner_terms &t, double size_i)
     t.inv_size = 1.0 / size_i;
     if constexpr (COAG_KERNEL == 3)
     {
-        t.si = stokes_scaled(size_i, t.radial, t.strat);
+        t.si = stokes_partner(t, size_i);
         t.fi = 1.0 / (1.0 + t.si*t.si);
         t.ai = t.si*t.fi;
         t.mi = fmin(t.si, 0.5);
@@ -540,16 +558,6 @@ GDX_FN owner_terms make_owner(const gdx_env &e, double size_i, double m, double 
     return t;
 }
 
-// _get_stokes for a partner size with the owner's scalings: same three divisions, uniform divisors
-GDX_FN double stokes_partner(const owner_terms &t, double size)
-{
-    double st = STOKES_0*div_rn(size, S_0, INV_S0);
-#ifndef CONST_ST
-    st = div_rn(st, t.radial, t.inv_radial);
-    st = div_rn(st, t.strat, t.inv_strat);
-#endif
-    return st;
-}
 
 GDX_FN double vrel_pair_terms(const owner_terms &t, double size_j)
 {

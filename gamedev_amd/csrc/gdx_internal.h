@@ -88,7 +88,13 @@ struct zero_ranges
     void *p[8];
     unsigned long long bytes[8];
     int n = 0;
-    void add(void *ptr, size_t b) { p[n] = ptr; bytes[n] = b; ++n; }
+    void add(void *ptr, size_t b)
+    {
+        if (n == 8) throw std::logic_error("zero_ranges: more than 8 ranges");
+        p[n] = ptr;
+        bytes[n] = b;
+        ++n;
+    }
 };
 
 // particle-order view of one bound state field: element i lives at base + i*stride bytes
@@ -133,13 +139,13 @@ struct run_buffers
     dbuf<int> d_int;
     dbuf<double> d_dt;
     dbuf<int> qprefix;
-    dbuf<int> work;
+    dbuf<int> work;                   // chain pull counters: heavy queue, light queue
     int resident = 0;                 // resident blocks of the persistent chain kernel
     // controller sums, queue lengths, counters and errors of every group and swarm of the run, so that
     // each phase clears and reads them back with one call each
     dbuf<gdx_raw_rate> raw_rate;
     dbuf<gdx_raw_audit> raw_audit;
-    dbuf<int> qcount, ccount;
+    dbuf<int> qcount, hcount, ccount;
     dbuf<int> err;
     dbuf<unsigned long long> counters, wave_ctr;
     pinned<gdx_raw_rate> h_rate;
@@ -151,6 +157,7 @@ struct run_buffers
         raw_rate.alloc(size_t(groups)*COL_BIN_S);
         raw_audit.alloc(size_t(groups)*COL_BIN_S);
         qcount.alloc(groups);
+        hcount.alloc(groups);
         ccount.alloc(groups);
         err.alloc(swarms);
         counters.alloc(size_t(swarms)*GDX_CTRS);
@@ -182,8 +189,8 @@ struct run_buffers
         hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_dt), sizeof(double)*m), "pinned due list");
         d_int.alloc(DUE_INTS*m + DUE_EXTRA);
         d_dt.alloc(m);
-        qprefix.alloc(m + 1);
-        work.alloc(1);
+        qprefix.alloc(2*(m + 1));
+        work.alloc(2);
         cap = m;
     }
 };

@@ -98,7 +98,8 @@ struct gdx_gref
     gdx_event_work *evw;                      // null without diagnostics
     gdx_raw_rate *raw_rate;                   // the group's COL_BIN_S raw bins
     gdx_raw_audit *raw_audit;
-    int *qcount;                              // the group's event-queue length
+    int *qcount;                              // the group's light queue length (from the region's end down)
+    int *hcount;                              // the group's heavy queue length (from the region's start up)
     unsigned long long *scr_min, *scr_max;    // the group's size extrema scratch
     double *lower, *upper;                    // the group's moving size bounds
     unsigned long long *counters;             // the swarm's operator counters (GDX_CTR_*)
@@ -154,7 +155,8 @@ struct gdx_wave
     const int *cnt;
     const int *bstart;           // n + 1 entries
     const double *dt;
-    const int *qprefix;          // chain: exclusive prefix of the groups' queue lengths (n + 1)
+    const int *qprefix;          // chain: exclusive prefixes of the groups' heavy queue lengths (n + 1),
+                                 // then of their light queue lengths (n + 1)
     const int *sub;              // per group: exact-rate subsample phase of a learned swarm, -1 plain
     int learned;                 // some group belongs to a learned swarm
 };

@@ -34,6 +34,10 @@ constexpr int PUBLISH_OWNERS = 1024;
 constexpr int RATE_OWNERS = 256;
 constexpr int CHAIN_OWNERS = 256;
 constexpr int CHAIN_CHUNK = 8*NSEG;     // queue positions a chain wavefront takes per counter atomic
+// owners expected to take at least this many events in the bath (rate at the bath start times the
+// bath) join the heavy end of their group's queue region, which the chain takes first: long chains
+// then start with the launch instead of holding it open after the other owners are done
+constexpr double HEAVY_EVENTS = 4.0;
 // events one owner may take in one bath: the reference allows COL_EVENT_CAP events per launch and
 // 1e6 continuation launches (inc/swarm/_col_chain.cuh:381-384); exceeding this bound is status 7
 constexpr int EVENT_LIMIT = 1 << 22;
@@ -656,7 +660,7 @@ __device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int 
 __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 {
     __shared__ audit_lds a;
-    __shared__ int block_off[2][WAVES], block_tot[2], block_base[2];
+    __shared__ int block_off[3][WAVES], block_tot[3], block_base[3];
     int gi, s0, s1;
     if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
@@ -669,7 +673,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     const double end = w.dt[gi];
     const double lo = *G.lower, up = *G.upper;
     clear_audit(a);
-    if (threadIdx.x < 2) block_tot[threadIdx.x] = 0;
+    if (threadIdx.x < 3) block_tot[threadIdx.x] = 0;
     // this group's extrema scratch was consumed by k_bath_rate; clear it for the group's next refresh
     if (threadIdx.x == 0 && xcd_block() == w.bstart[gi]) { *G.scr_min = dbits(__builtin_inf()); *G.scr_max = 0ull; }
     __syncthreads();
@@ -678,7 +682,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     const int s = s0 + int(threadIdx.x);
     const bool valid = s < s1;
     const bool learned = w.learned && w.sub[gi] >= 0;
-    bool chain = false, done = false;
+    bool chain = false, done = false, heavy = false;
     // learned swarms: owners outside the exact subsample draw a candidate time from their bound
     // (thinning); the subsample follows the plain screen (its bound was checked by the rate pass)
     bool thin = false, violation = false, candidate = false;
@@ -738,6 +742,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
             }
         }
         chain = !done;
+        heavy = chain && isfinite(c.x) && c.x*end >= HEAVY_EVENTS;
         if (done)
         {
             old = res[s];
@@ -751,23 +756,27 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
         || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
     audit_wave<false>(a, done, bs, bs, weight, probability, hazard, j1, j2, jmax, 0.0, 0.0, bad);
-    // owners with an event join the group's queue and thinning candidates its candidate list: wavefront
-    // offsets from LDS, one returning global atomic per block and list (the group counters are shared
-    // by hundreds of blocks on every XCD)
-    const unsigned long long ball = __ballot(chain), cb = __ballot(candidate);
+    // owners with an event join the group's queue (heavy ones from the region's start, the others from
+    // its end) and thinning candidates its candidate list: wavefront offsets from LDS, one returning
+    // global atomic per block and list (the group counters are shared by hundreds of blocks on every XCD)
+    const unsigned long long bl = __ballot(chain && !heavy), bh = __ballot(heavy), cb = __ballot(candidate);
     const int wv = threadIdx.x / WAVE;
     if (lane == 0)
     {
-        block_off[0][wv] = ball ? atomicAdd(&block_tot[0], __popcll(ball)) : 0;
+        block_off[0][wv] = bl ? atomicAdd(&block_tot[0], __popcll(bl)) : 0;
         block_off[1][wv] = cb ? atomicAdd(&block_tot[1], __popcll(cb)) : 0;
+        block_off[2][wv] = bh ? atomicAdd(&block_tot[2], __popcll(bh)) : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) block_base[0] = block_tot[0] ? atomicAdd(G.qcount, block_tot[0]) : 0;
     if (threadIdx.x == WAVE) block_base[1] = block_tot[1] ? atomicAdd(G.ccount, block_tot[1]) : 0;
+    if (threadIdx.x == 2*WAVE) block_base[2] = block_tot[2] ? atomicAdd(G.hcount, block_tot[2]) : 0;
     __syncthreads();
     if (chain)
     {
-        const int pos = w.beg[gi] + block_base[0] + block_off[0][wv] + __popcll(ball & ((1ull << lane) - 1ull));
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int pos = heavy ? w.beg[gi] + block_base[2] + block_off[2][wv] + __popcll(bh & below)
+                              : w.beg[gi] + w.cnt[gi] - 1 - (block_base[0] + block_off[0][wv] + __popcll(bl & below));
         G.queue[pos] = s;
         if (learned) G.qstart[pos] = make_double2(0.0, 0.0);
     }
@@ -804,7 +813,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
 {
     __shared__ double cats[BLOCK][3];
     __shared__ unsigned tally[4];        // candidates, exact checks, violations, accepted
-    __shared__ int block_off[WAVES], block_tot, block_base;
+    __shared__ int block_off[2][WAVES], block_tot[2], block_base[2];
     int gi, s0, s1;
     if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
     if (w.sub[gi] < 0) return;
@@ -813,7 +822,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     const int n_cand = *GP->ccount;
     if (s0 - beg >= n_cand) return;
     if (threadIdx.x < 4) tally[threadIdx.x] = 0u;
-    if (threadIdx.x == 0) block_tot = 0;
+    if (threadIdx.x < 2) block_tot[threadIdx.x] = 0;
     __syncthreads();
     const int lane = threadIdx.x % WAVE;
     const int pos = s0 + int(threadIdx.x);
@@ -823,7 +832,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     const double m = own ? GP->measure[s] : 0.0;
     const rate_sums rs = owner_rate_sums(GP, s, own, me, m, &cats[threadIdx.x][0]);
     const double R = rs.total.r;
-    bool chain = false, check = false, viol = false, accepted = false;
+    bool chain = false, check = false, viol = false, accepted = false, heavy = false;
     double2 qs = make_double2(0.0, 0.0);
     if (own)
     {
@@ -847,27 +856,41 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
             {
                 chain = true;
                 qs = make_double2(c.x, 1.0);
+                heavy = 1.0 + R*(end - c.x) >= HEAVY_EVENTS;
             }
             else if (R > 0.0)
             {
                 uint64_t trial = ctr;
                 const double wait = -log(col_uniform(pidx, GP->seed, trial)) / R;
                 if (isfinite(wait) && wait > 0.0 && wait >= end - c.x) ctr = trial;   // no event in the bath
-                else { chain = true; qs = make_double2(c.x, 0.0); }                   // the chain repeats the draw
+                else
+                {
+                    chain = true;                                                      // the chain repeats the draw
+                    qs = make_double2(c.x, 0.0);
+                    heavy = R*(end - c.x) >= HEAVY_EVENTS;
+                }
             }
         }
         GP->rng[s] = ctr;
     }
-    // queue appends: wavefront offsets from LDS, one returning global atomic per block
-    const unsigned long long ball = __ballot(chain);
+    // queue appends (heavy owners from the region's start, the others from its end): wavefront offsets
+    // from LDS, one returning global atomic per block and list
+    const unsigned long long bl = __ballot(chain && !heavy), bh = __ballot(chain && heavy);
     const int wv = threadIdx.x / WAVE;
-    if (lane == 0) block_off[wv] = ball ? atomicAdd(&block_tot, __popcll(ball)) : 0;
+    if (lane == 0)
+    {
+        block_off[0][wv] = bl ? atomicAdd(&block_tot[0], __popcll(bl)) : 0;
+        block_off[1][wv] = bh ? atomicAdd(&block_tot[1], __popcll(bh)) : 0;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) block_base = block_tot ? atomicAdd(GP->qcount, block_tot) : 0;
+    if (threadIdx.x == 0) block_base[0] = block_tot[0] ? atomicAdd(GP->qcount, block_tot[0]) : 0;
+    if (threadIdx.x == WAVE) block_base[1] = block_tot[1] ? atomicAdd(GP->hcount, block_tot[1]) : 0;
     __syncthreads();
     if (chain)
     {
-        const int q = beg + block_base + block_off[wv] + __popcll(ball & ((1ull << lane) - 1ull));
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int q = heavy ? beg + block_base[1] + block_off[1][wv] + __popcll(bh & below)
+                            : beg + w.cnt[gi] - 1 - (block_base[0] + block_off[0][wv] + __popcll(bl & below));
         GP->queue[q] = s;
         GP->qstart[q] = qs;
     }
@@ -902,29 +925,42 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     }
 }
 
-// exclusive prefix of the due groups' queue lengths (one block)
+// exclusive prefixes of the due groups' heavy queue lengths (qprefix[0..n]) and light queue lengths
+// (qprefix[n+1..2n+1]), one block
 __global__ __launch_bounds__(256) void k_queue_prefix(gdx_wave w, int *qprefix)
 {
-    __shared__ int part[256];
+    __shared__ int part[2][256];
     const int t = threadIdx.x;
     const int per = (w.n + 255) / 256;
-    int sum = 0;
-    for (int i = t*per; i < min(w.n, (t + 1)*per); ++i) sum += *w.table[w.ref[i]].qcount;
-    part[t] = sum;
-    __syncthreads();
-    if (t == 0)
-    {
-        int acc = 0;
-        for (int k = 0; k < 256; ++k) { int v = part[k]; part[k] = acc; acc += v; }
-    }
-    __syncthreads();
-    int acc = part[t];
+    int sh = 0, sl = 0;
     for (int i = t*per; i < min(w.n, (t + 1)*per); ++i)
     {
-        qprefix[i] = acc;
-        acc += *w.table[w.ref[i]].qcount;
+        sh += *w.table[w.ref[i]].hcount;
+        sl += *w.table[w.ref[i]].qcount;
     }
-    if (t == 255) qprefix[w.n] = acc;
+    part[0][t] = sh;
+    part[1][t] = sl;
+    __syncthreads();
+    if (t < 2)
+    {
+        int acc = 0;
+        for (int k = 0; k < 256; ++k) { int v = part[t][k]; part[t][k] = acc; acc += v; }
+    }
+    __syncthreads();
+    int ah = part[0][t], al = part[1][t];
+    int *lprefix = qprefix + w.n + 1;
+    for (int i = t*per; i < min(w.n, (t + 1)*per); ++i)
+    {
+        qprefix[i] = ah;
+        lprefix[i] = al;
+        ah += *w.table[w.ref[i]].hcount;
+        al += *w.table[w.ref[i]].qcount;
+    }
+    if (t == 255)
+    {
+        qprefix[w.n] = ah;
+        lprefix[w.n] = al;
+    }
 }
 
 // event chains (col_chain_run.cu:127-382): a persistent grid pulls the queued owners from one global
@@ -935,19 +971,23 @@ constexpr int CHAIN_PREFIX_LDS = 4096;     // due groups whose queue prefix a ch
 __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
 {
     __shared__ double walk[WAVES][NSEG][N_K];
-    __shared__ int qp[CHAIN_PREFIX_LDS + 1];
+    __shared__ int qp[2*(CHAIN_PREFIX_LDS + 1)];
     __shared__ unsigned long long end_key, end_sum[5];
     __shared__ unsigned long long *end_wave;
     if (threadIdx.x == 0) end_key = 0ull;
     if (threadIdx.x < 5) end_sum[threadIdx.x] = 0ull;
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    // the queue prefix of the due groups, searched at every refill: LDS when it fits
+    // the queue prefixes of the due groups (heavy, then light), searched at every refill: LDS when
+    // they fit. Positions [0, heavy) of the run are the heavy queues, group after group; the rest are
+    // the light queues.
     const bool in_lds = w.n <= CHAIN_PREFIX_LDS;
     if (in_lds)
-        for (int i = threadIdx.x; i <= w.n; i += BLOCK) qp[i] = w.qprefix[i];
+        for (int i = threadIdx.x; i < 2*(w.n + 1); i += BLOCK) qp[i] = w.qprefix[i];
     __syncthreads();
-    const int *qprefix = in_lds ? qp : w.qprefix;
-    const int total = qprefix[w.n];
+    const int *hprefix = in_lds ? qp : w.qprefix;
+    const int *lprefix = hprefix + w.n + 1;
+    const int heavy = hprefix[w.n];
+    const int total = heavy + lprefix[w.n];
 
     // lane kl of a segment holds row position kl, i.e. slot bitrev(kl)
     const int seg = lane / SEG, kl = lane % SEG, segbase = seg*SEG, slot_k = bitrev_slot(kl);
@@ -958,6 +998,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     int longest = 0;
     // accepted thinning candidate of a learned swarm: its first event happens at its start time
     bool first_event = false;
+    bool heavy_owner = false;
     bool moments = true;
     bool more = true, active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
@@ -971,6 +1012,21 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     owner_terms t{};
     cat_moments cm{0.0, 0.0, 0.0};
     partner q{0.0, 0.0, 0.0, false};
+    // the owner's next SEG draws, one per lane of the segment (lane kl: draw pool_base + kl, and its
+    // -log for waits), computed side by side and taken in stream order: the same numbers as drawing
+    // one at a time, for one Philox block and one logarithm per SEG draws
+    double pool_u = 0.0, pool_nl = 0.0;
+    uint64_t pool_base = 0;
+    int pool_used = 0;
+    auto refill = [&](uint64_t base) {
+        pool_base = base;
+        pool_used = 0;
+        pool_u = col_uniform_at(pidx, seed, base + uint64_t(kl));
+        pool_nl = -log(pool_u);
+    };
+    // segment-uniform calls only: the source lane is in the caller's segment
+    auto draw_u = [&]() { const double v = __shfl(pool_u, segbase + pool_used, WAVE); ++pool_used; return v; };
+    auto draw_nl = [&]() { const double v = __shfl(pool_nl, segbase + pool_used, WAVE); ++pool_used; return v; };
     // counters are flushed when a segment moves to an owner of another swarm, and at the end
     auto flush = [&]() {
         if (kl == 0 && ctr_of)
@@ -986,20 +1042,34 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         n_events = n_chain = n_pairs = n_blocks = 0;
         longest = 0;
     };
-    // the wavefront's current run of queue positions [cb, ce), taken CHAIN_CHUNK at a time from the
-    // global counter so that the counter sees one atomic per CHAIN_CHUNK owners
+    // the wavefront's current run of positions [cb, ce): heavy owners from work[0], only as many as the
+    // wavefront has free segments (so that long chains run side by side on many wavefronts), then light
+    // owners from work[1], CHAIN_CHUNK at a time so that the counter sees one atomic per CHAIN_CHUNK owners
     int cb = 0, ce = 0;
+    bool hmore = heavy > 0;
     while (true)
     {
         unsigned long long need = __ballot(!active) & seg_lead;
         if (need && cb >= ce && more)
         {
             int base = 0;
-            if (lane == 0) base = atomicAdd(work, CHAIN_CHUNK);
-            base = __builtin_amdgcn_readfirstlane(base);
-            if (base >= total) { more = false; base = total; }
-            cb = base;
-            ce = min(base + CHAIN_CHUNK, total);
+            if (hmore)
+            {
+                const int k = __popcll(need);
+                if (lane == 0) base = atomicAdd(work, k);
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (base < heavy) { cb = base; ce = min(base + k, heavy); }
+                else hmore = false;
+            }
+            if (!hmore)
+            {
+                const int light = total - heavy;
+                if (lane == 0) base = atomicAdd(work + 1, CHAIN_CHUNK);
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (base >= light) { more = false; base = light; }
+                cb = heavy + base;
+                ce = heavy + min(base + CHAIN_CHUNK, light);
+            }
         }
         if (need && cb < ce)
         {
@@ -1008,18 +1078,22 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             if (!active && rank < n)
             {
                 int v = cb + rank;
-                // due group of queue position v: last entry with qprefix <= v
+                const bool hv = v < heavy;
+                if (!hv) v -= heavy;
+                const int *prefix = hv ? hprefix : lprefix;
+                // due group of run position v: last entry with prefix <= v
                 int lo = 0, hi = w.n - 1;
                 while (lo < hi)
                 {
                     int mid = (lo + hi + 1) >> 1;
-                    if (qprefix[mid] <= v) lo = mid;
+                    if (prefix[mid] <= v) lo = mid;
                     else hi = mid - 1;
                 }
                 gi = lo;
                 G = w.table + w.ref[gi];
                 if (G->counters != ctr_of) { flush(); ctr_of = G->counters; wave_of = G->wave_ctr; }
-                pos = w.beg[gi] + (v - qprefix[gi]);
+                // heavy queues grow from the group region's start, light ones from its end
+                pos = hv ? w.beg[gi] + (v - prefix[gi]) : w.beg[gi] + w.cnt[gi] - 1 - (v - prefix[gi]);
                 s = G->queue[pos];
                 end = w.dt[gi];
                 seed = G->seed;
@@ -1033,6 +1107,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 if constexpr (COAG_KERNEL == 3) e = G->env[s];
                 if constexpr (PARTNER_HG) hg_i = G->hgs[s];
                 t = make_owner(e, cur.x, G->measure[s], hg_i);
+                refill(ctr);
                 moments = exact_owner(w, gi, s);
                 // the catastrophic moments enter only the jump moments
                 if (moments) cm = catastrophic_moments(cur.x);
@@ -1048,6 +1123,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                     first_event = qs.y > 0.0;
                 }
                 events = 0;
+                heavy_owner = hv;
                 active = true;
             }
             cb += n;
@@ -1057,7 +1133,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             if (cb >= ce && !more) break;
             continue;
         }
+        // a wavefront running a heavy owner's chain issues ahead of its SIMD's other wavefronts: the
+        // longest chains set the launch's length
+        if (__ballot(active && heavy_owner)) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(0);
         if (!active) continue;
+        // an event takes at most three draws
+        if (pool_used > SEG - 3) refill(pool_base + uint64_t(pool_used));
         // owners outside a learned swarm's audited subsample keep no compensators: rates alone
         const pair_term p = moments ? eval_pair_terms(t, cm, q, lambda_0) : eval_pair_rate(t, q, lambda_0);
         const bool bad_rate = !isfinite(p.rate) || p.rate < 0.0;
@@ -1079,7 +1161,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         else if (R == 0.0) { time = end; fin = true; }
         else
         {
-            double wait = now ? 0.0 : -log(col_uniform(pidx, seed, ctr)) / R;
+            double wait = now ? 0.0 : draw_nl() / R;
             double remaining = end - time;
             if (!now && (!isfinite(wait) || !(wait > 0.0))) { code = 3; fin = true; }
             else if (!now && wait >= remaining)
@@ -1102,7 +1184,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                     j2 += J2*wait;
                     jmax = fmax(jmax, JM);
                     // partner by the cumulative walk in slot order, last positive slot on roundoff
-                    double target = col_uniform(pidx, seed, ctr)*R;
+                    double target = draw_u()*R;
                     walk[wave][seg][slot_k] = p.rate;
                     __builtin_amdgcn_wave_barrier();
                     const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
@@ -1134,7 +1216,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                         double vrel = __shfl(p.vrel, segbase + bitrev_slot(slot), WAVE);
                         double mass_before = numr_i*size_i*size_i*size_i;
                         bool high_speed = vrel >= V_FRAG;
-                        double sample = high_speed ? col_uniform(pidx, seed, ctr) : 0.0;
+                        double sample = high_speed ? draw_u() : 0.0;
                         double size_new = sample_outcome(size_i, size_j, high_speed, sample);
                         if (kl == 0 && G->evw)
                         {
@@ -1163,7 +1245,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
             if (kl == 0)
             {
                 if (events > 0) G->st[s] = make_double2(t.size_i, numr_i);
-                G->rng[s] = ctr;
+                G->rng[s] = pool_base + uint64_t(pool_used);
                 if (code) atomicMax(G->err, code);
                 G->qsum[pos] = make_double4(hazard, j1, j2, jmax);
                 G->qevents[pos] = code ? -1 : events;
@@ -1220,8 +1302,10 @@ __global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_wave w)
     int gi, s0, s1;
     if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
-    const int q1 = w.beg[gi] + *G.qcount;
-    if (s0 >= q1) return;
+    // queued positions: the heavy queue [beg, h1) and the light queue [l0, beg + cnt)
+    const int h1 = w.beg[gi] + *G.hcount;
+    const int l0 = w.beg[gi] + w.cnt[gi] - *G.qcount;
+    if (s0 >= h1 && s1 <= l0) return;
     const double end = w.dt[gi];
     const double lo = *G.lower, up = *G.upper;
     const int *queue = G.queue;
@@ -1232,7 +1316,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_wave w)
     __syncthreads();
     const int pos = s0 + int(threadIdx.x);
     // owners that ended in an error are left out: the operator call fails on them
-    bool take = pos < s1 && pos < q1 && qevents[pos] >= 0;
+    bool take = pos < s1 && (pos < h1 || pos >= l0) && qevents[pos] >= 0;
     // learned swarms audit their exact subsample only
     if (take && w.learned) take = exact_owner(w, gi, queue[pos]);
     double2 old = make_double2(1.0, 0.0);
