@@ -246,8 +246,11 @@ struct rate_sums
     double JM;
     bool V;
 };
+// MOMENTS false: the rates alone (eval_pair_rate, the same rate bit for bit), no jump moments
+// chunks (optional): the rate partial of every four consecutive row positions, (t0 + t1) + (t2 + t3)
+template <bool MOMENTS = true>
 __device__ __forceinline__ rate_sums owner_rate_sums(const gdx_gref *GP, int s, bool own, double2 me, double m,
-                                                     double *cat3)
+                                                     double *cat3, double *chunks = nullptr)
 {
     const int *nbr = GP->nbr;
     const double2 *res = GP->res;
@@ -260,6 +263,7 @@ __device__ __forceinline__ rate_sums owner_rate_sums(const gdx_gref *GP, int s, 
     if constexpr (PARTNER_HG) { if (own) hg_i = hgs[s]; }
     const owner_terms t = make_owner(e, me.x, m, hg_i);
     // read only by lanes with a catastrophic pair: kept in LDS rather than in registers
+    if constexpr (MOMENTS)
     {
         cat_moments c = catastrophic_moments(me.x);
         cat3[0] = c.mean;
@@ -267,6 +271,10 @@ __device__ __forceinline__ rate_sums owner_rate_sums(const gdx_gref *GP, int s, 
         cat3[2] = c.maximum;
     }
     const cat_moments_lds cm{cat3};
+    auto eval = [&](const partner &q) {
+        if constexpr (MOMENTS) return eval_pair_terms(t, cm, q, lambda_0);
+        else return eval_pair_rate(t, q, lambda_0);
+    };
     const bool live = own && m > 0.0;
     const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : 0)*N_K);
 
@@ -282,23 +290,24 @@ __device__ __forceinline__ rate_sums owner_rate_sums(const gdx_gref *GP, int s, 
         // each term is folded as soon as it is formed: (t0 + t1) + (t2 + t3), the level-2 partial
         const partner q0 = load_partner(jj.x, res, hgs), q1 = load_partner(jj.y, res, hgs);
         const partner q2 = load_partner(jj.z, res, hgs), q3 = load_partner(jj.w, res, hgs);
-        pair_term p = eval_pair_terms(t, cm, q0, lambda_0);
+        pair_term p = eval(q0);
         fold3 a{p.rate, p.j1, p.j2};
         V = V && isfinite(p.rate) && p.rate >= 0.0;
         JM = fmax(JM, p.jmax);
-        p = eval_pair_terms(t, cm, q1, lambda_0);
+        p = eval(q1);
         a = add3(a, {p.rate, p.j1, p.j2});
         V = V && isfinite(p.rate) && p.rate >= 0.0;
         JM = fmax(JM, p.jmax);
-        p = eval_pair_terms(t, cm, q2, lambda_0);
+        p = eval(q2);
         fold3 b{p.rate, p.j1, p.j2};
         V = V && isfinite(p.rate) && p.rate >= 0.0;
         JM = fmax(JM, p.jmax);
-        p = eval_pair_terms(t, cm, q3, lambda_0);
+        p = eval(q3);
         b = add3(b, {p.rate, p.j1, p.j2});
         V = V && isfinite(p.rate) && p.rate >= 0.0;
         JM = fmax(JM, p.jmax);
         fold3 part = add3(a, b);
+        if (chunks) chunks[c] = part.r;
         bool placed = false;
 #pragma unroll
         for (int lv = 2; lv < LOG2K; ++lv)
@@ -806,14 +815,20 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 // thinning candidates of learned swarms, one lane per candidate: the exact K-sum at the candidate time
 // (the owner and its frozen partners are unchanged since the bath start, so this is the bath-start
 // rate). Above the bound: the owner restarts the plain chain from the bath start on fresh draws.
-// Otherwise the candidate is an event with probability R / bound (the chain takes it from the
-// candidate time); a rejected candidate continues on R from the candidate time, and only owners whose
-// next draw falls inside the bath enter the chain, which repeats that draw.
+// Otherwise the candidate is an event with probability R / bound; a rejected candidate continues on R
+// from the candidate time. The owner's first event (at the candidate time when accepted, else at the
+// end of its next wait if that falls inside the bath) is taken here, with the draws and operations of
+// the chain: the partner walk over the same rates in slot order, the outcome, then the K-sum at the new
+// size and the next wait. Owners whose next event still falls inside the bath enter the chain after
+// that first event, and the chain repeats the pending draw; any invalid rate or state sends the owner
+// to the chain from the candidate time with its draws uncommitted, and the chain reports it.
 __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
 {
-    __shared__ double cats[BLOCK][3];
-    __shared__ unsigned tally[4];        // candidates, exact checks, violations, accepted
+    // candidates, exact checks, violations, accepted, pair evaluations beyond the first K-sum, events
+    // (= owners) finished here
+    __shared__ unsigned tally[6];
     __shared__ int block_off[2][WAVES], block_tot[2], block_base[2];
+    __shared__ double chunk_rate[N_K / 4][BLOCK];
     int gi, s0, s1;
     if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
     if (w.sub[gi] < 0) return;
@@ -821,7 +836,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     const int beg = w.beg[gi];
     const int n_cand = *GP->ccount;
     if (s0 - beg >= n_cand) return;
-    if (threadIdx.x < 4) tally[threadIdx.x] = 0u;
+    if (threadIdx.x < 6) tally[threadIdx.x] = 0u;
     if (threadIdx.x < 2) block_tot[threadIdx.x] = 0;
     __syncthreads();
     const int lane = threadIdx.x % WAVE;
@@ -830,15 +845,20 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     const int s = own ? GP->cqueue[pos] : beg;
     const double2 me = own ? GP->st[s] : make_double2(1.0, 0.0);
     const double m = own ? GP->measure[s] : 0.0;
-    const rate_sums rs = owner_rate_sums(GP, s, own, me, m, &cats[threadIdx.x][0]);
+    double chunks[N_K / 4];
+    const rate_sums rs = owner_rate_sums<false>(GP, s, own, me, m, nullptr, chunks);
     const double R = rs.total.r;
-    bool chain = false, check = false, viol = false, accepted = false, heavy = false;
+#pragma unroll
+    for (int c = 0; c < N_K / 4; ++c) chunk_rate[c][threadIdx.x] = chunks[c];
+    bool chain = false, check = false, viol = false, accepted = false, heavy = false, finished = false;
+    unsigned extra_pairs = 0;
     double2 qs = make_double2(0.0, 0.0);
     if (own)
     {
         const double2 c = GP->cstart[pos];
         const double end = w.dt[gi];
         const uint32_t pidx = uint32_t(GP->slot_pidx[s]);
+        const uint64_t seed = GP->seed;
         uint64_t ctr = GP->rng[s];
         if (!rs.V || !isfinite(R) || R < 0.0)
         {
@@ -851,23 +871,127 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
         else
         {
             check = true;
-            accepted = col_uniform(pidx, GP->seed, ctr)*c.y < R;
+            accepted = col_uniform(pidx, seed, ctr)*c.y < R;
+            // the first event: at the candidate time, or after the next wait on R
+            bool event = accepted;
+            double t_event = c.x;
+            uint64_t k = ctr;                              // draws taken here, committed at the end
             if (accepted)
             {
-                chain = true;
                 qs = make_double2(c.x, 1.0);
-                heavy = 1.0 + R*(end - c.x) >= HEAVY_EVENTS;
             }
             else if (R > 0.0)
             {
-                uint64_t trial = ctr;
-                const double wait = -log(col_uniform(pidx, GP->seed, trial)) / R;
-                if (isfinite(wait) && wait > 0.0 && wait >= end - c.x) ctr = trial;   // no event in the bath
+                const double wait = -log(col_uniform(pidx, seed, k)) / R;
+                if (isfinite(wait) && wait > 0.0 && wait >= end - c.x) ctr = k;      // no event in the bath
                 else
                 {
-                    chain = true;                                                      // the chain repeats the draw
-                    qs = make_double2(c.x, 0.0);
-                    heavy = R*(end - c.x) >= HEAVY_EVENTS;
+                    qs = make_double2(c.x, 0.0);           // the chain repeats the draw
+                    t_event = c.x + wait;
+                    event = isfinite(wait) && wait > 0.0 && t_event > c.x;
+                    chain = !event;
+                }
+            }
+            if (event)
+            {
+                chain = true;                              // unless finished below
+                // partner with probability rate / R: the chunk whose cumulative chunk partials (row
+                // order) pass the target, then the cumulative walk over the chunk's four rates; the
+                // last positive chunk and slot on roundoff. The chain walks the slots in order
+                // instead; both pick a partner with probability rate / R (no reference counterpart).
+                const double target = col_uniform(pidx, seed, k)*R;
+                double before = 0.0;
+                int cc = -1, last_c = -1;
+                double last_before = 0.0;
+                for (int c = 0; c < N_K / 4; ++c)
+                {
+                    const double v = chunk_rate[c][threadIdx.x];
+                    if (v > 0.0)
+                    {
+                        if (before + v > target) { cc = c; break; }
+                        last_c = c;
+                        last_before = before;
+                        before += v;
+                    }
+                }
+                if (cc < 0) { cc = last_c; before = last_before; }
+                double size_j = 0.0, vrel = 0.0;
+                int slot = -1;
+                if (cc >= 0)
+                {
+                    gdx_env e{};
+                    double hg_i = 0.0;
+                    if constexpr (COAG_KERNEL == 3) e = GP->env[s];
+                    if constexpr (PARTNER_HG) hg_i = GP->hgs[s];
+                    const owner_terms ot = make_owner(e, me.x, m, hg_i);
+                    const int4 jq = reinterpret_cast<const int4 *>(GP->nbr + (long long)s*N_K)[cc];
+                    const int jr[4] = {jq.x, jq.y, jq.z, jq.w};
+                    double cumulative = before, last_size = 0.0, last_vrel = 0.0;
+                    int last = -1;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                    {
+                        const partner q = load_partner(jr[r], GP->res, GP->hgs);
+                        const pair_term pt = eval_pair_rate(ot, q, GP->lambda_0);
+                        if (slot < 0 && pt.rate > 0.0)
+                        {
+                            cumulative += pt.rate;
+                            last = r;
+                            last_size = pt.size_j;
+                            last_vrel = pt.vrel;
+                            if (cumulative > target) { slot = r; size_j = pt.size_j; vrel = pt.vrel; }
+                        }
+                    }
+                    extra_pairs += 4;
+                    if (slot < 0) { slot = last; size_j = last_size; vrel = last_vrel; }
+                }
+                if (slot >= 0)
+                {
+                    const double size_i = me.x, numr_i = me.y;
+                    const double mass_before = numr_i*size_i*size_i*size_i;
+                    const bool high_speed = vrel >= V_FRAG;
+                    const double sample = high_speed ? col_uniform(pidx, seed, k) : 0.0;
+                    const double size_new = sample_outcome(size_i, size_j, high_speed, sample);
+                    const double numr_new = mass_before / (size_new*size_new*size_new);
+                    const double mass_after = numr_new*size_new*size_new*size_new;
+                    const double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
+                    const bool ok = isfinite(size_new) && isfinite(numr_new) && size_new > 0.0 && numr_new > 0.0
+                        && !(fabs(mass_after - mass_before) > 2.0e-12*mass_scale);
+                    if (ok)
+                    {
+                        // the K-sum at the new size and the next wait from the event
+                        const rate_sums r2 = owner_rate_sums<false>(GP, s, true, make_double2(size_new, numr_new), m,
+                                                                    nullptr);
+                        extra_pairs += N_K;
+                        const double R2 = r2.total.r;
+                        const double remaining = end - t_event;
+                        bool done = false;
+                        uint64_t k2 = k;
+                        if (r2.V && isfinite(R2) && R2 >= 0.0)
+                        {
+                            if (R2 == 0.0) done = true;
+                            else
+                            {
+                                const double wait2 = -log(col_uniform(pidx, seed, k2)) / R2;
+                                done = isfinite(wait2) && wait2 > 0.0 && wait2 >= remaining;
+                                if (!done) heavy = R2*remaining >= HEAVY_EVENTS;
+                            }
+                        }
+                        if (GP->evw)
+                        {
+                            double log_mass;
+                            const int cat = outcome_category(size_i, size_j, high_speed, sample, size_new, log_mass);
+                            GP->evw[s].count[cat] += 1ull;
+                            GP->evw[s].log_mass[cat] += log_mass;
+                        }
+                        GP->st[s] = make_double2(size_new, numr_new);
+                        // finished: the wait draw is committed; else the chain continues after the event
+                        // and repeats it
+                        ctr = done ? k2 : k;
+                        finished = done;
+                        chain = !done;
+                        qs = make_double2(t_event, 2.0);
+                    }
                 }
             }
         }
@@ -896,21 +1020,25 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     }
     const unsigned nc = __popcll(__ballot(own)), nk = __popcll(__ballot(check));
     const unsigned nv = __popcll(__ballot(viol)), na = __popcll(__ballot(accepted));
+    const unsigned nf = __popcll(__ballot(finished));
+    unsigned np = extra_pairs;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) np += __shfl_xor(np, o, WAVE);
     if (lane == 0)
     {
         if (nc) atomicAdd(&tally[0], nc);
         if (nk) atomicAdd(&tally[1], nk);
         if (nv) atomicAdd(&tally[2], nv);
         if (na) atomicAdd(&tally[3], na);
+        if (np) atomicAdd(&tally[4], np);
+        if (nf) atomicAdd(&tally[5], nf);
     }
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        if (tally[0])
-        {
-            atomicAdd(GP->counters + GDX_CTR_CANDIDATES, (unsigned long long)tally[0]);
-            atomicAdd(GP->counters + GDX_CTR_PAIRS, (unsigned long long)tally[0]*N_K);
-        }
+        if (tally[0] || tally[4])
+            atomicAdd(GP->counters + GDX_CTR_PAIRS, (unsigned long long)tally[0]*N_K + tally[4]);
+        if (tally[0]) atomicAdd(GP->counters + GDX_CTR_CANDIDATES, (unsigned long long)tally[0]);
         if (tally[1])
         {
             atomicAdd(GP->counters + GDX_CTR_CHECKS, (unsigned long long)tally[1]);
@@ -922,6 +1050,17 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
             atomicAdd(GP->wave_ctr + GDX_WAVE_VIOLATIONS, (unsigned long long)tally[2]);
         }
         if (tally[3]) atomicAdd(GP->counters + GDX_CTR_ACCEPTED, (unsigned long long)tally[3]);
+        if (tally[5])
+        {
+            // owners finished here took one event each: counted as the chain counts its owners (one
+            // launch block each, longest chain at least one event)
+            atomicAdd(GP->counters + GDX_CTR_EVENTS, (unsigned long long)tally[5]);
+            atomicAdd(GP->counters + GDX_CTR_CHAINED, (unsigned long long)tally[5]);
+            atomicMax(GP->counters + GDX_CTR_LONGEST, 1ull);
+            atomicMax(GP->wave_ctr + GDX_WAVE_LONGEST, 1ull);
+            atomicAdd(GP->wave_ctr + GDX_WAVE_BLOCKS, (unsigned long long)tally[5]);
+            atomicAdd(GP->wave_ctr + GDX_WAVE_CHAINED, (unsigned long long)tally[5]);
+        }
     }
 }
 
@@ -1116,13 +1255,16 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 time = hazard = j1 = j2 = jmax = 0.0;
                 first_event = false;
                 // a learned group's queue entries carry their start (plain groups of a mixed wave do not)
+                events = 0;
                 if (w.learned && w.sub[gi] >= 0)
                 {
+                    // 1: an accepted candidate, its event at the start time; 2: its first event already
+                    // taken at the start time (the state read above is the state after it)
                     const double2 qs = G->qstart[pos];
                     time = qs.x;
-                    first_event = qs.y > 0.0;
+                    first_event = qs.y == 1.0;
+                    if (qs.y == 2.0) events = 1;
                 }
-                events = 0;
                 heavy_owner = hv;
                 active = true;
             }
