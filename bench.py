@@ -41,7 +41,7 @@ K = 32
 #   chain:       per chained owner 140 (state, clock, RNG, sums), plus codes 4 + partner (s, N) 16 per pair term
 #   chain_audit: per chained owner: queue entry 4 + chain sums 36 + published and final state 24
 #   majorant:    codes 4K + partner (s, N) 16K + owner (s, N) 16 + measure 8 + environment 64 + bound write 8
-KERNEL_BYTES = {"publish": 32, "bath_rate": 20*K + 29, "screen": 52, "majorant": 20*K + 96}
+KERNEL_BYTES = {"publish": 32, "bath_rate": 20*K + 29, "screen": 52, "majorant": 12*K + 72}
 CHAIN_OWNER_BYTES, CHAIN_PAIR_BYTES, AUDIT_OWNER_BYTES = 140, 20, 64
 
 
@@ -51,7 +51,7 @@ def kernel_bytes(kernel: str, rec: dict, tot: dict) -> float:
         return tot["rated_owners"]*KERNEL_BYTES[kernel]
     if kernel in KERNEL_BYTES:
         return rec["owners"]*KERNEL_BYTES[kernel]
-    chain_pairs = tot["pair_evals"] - K*tot["rated_owners"]
+    chain_pairs = tot["pair_evals"] - K*tot["rated_owners"] - tot.get("candidate_pairs", 0)
     if kernel == "chain":
         return tot["chain_owners"]*CHAIN_OWNER_BYTES + chain_pairs*CHAIN_PAIR_BYTES
     return tot["chain_owners"]*AUDIT_OWNER_BYTES
@@ -304,7 +304,7 @@ def timed_steps(op, steps: int, horizon: float, label: str) -> tuple[float, dict
     """Run `steps` operator calls; returns wall seconds, summed stats and active owners per step."""
     import torch
     keys = ("owner_updates", "events", "waves", "baths", "pair_evals", "chain_owners", "thinned", "candidates",
-            "accepted", "exact_checks", "violations", "rated_owners")
+            "accepted", "exact_checks", "violations", "rated_owners", "candidate_pairs")
     tot = {k: 0 for k in keys}
     tot["longest_chain"] = 0
     updates = 0

@@ -53,7 +53,7 @@ class Stats(C.Structure):
                 ("pair_evals", C.c_longlong), ("dt_col", C.c_double), ("clock_dyn", C.c_double),
                 ("max_chain_events", C.c_longlong), ("thinned", C.c_longlong), ("candidates", C.c_longlong),
                 ("accepted", C.c_longlong), ("exact_checks", C.c_longlong), ("violations", C.c_longlong),
-                ("rated_owners", C.c_longlong)]
+                ("rated_owners", C.c_longlong), ("candidate_pairs", C.c_longlong)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -578,6 +578,7 @@ struct op_run
         stats.accepted = (long long)counters[GDX_CTR_ACCEPTED];
         stats.exact_checks = (long long)counters[GDX_CTR_CHECKS];
         stats.violations = (long long)counters[GDX_CTR_VIOLATIONS];
+        stats.candidate_pairs = (long long)counters[GDX_CTR_CAND_PAIRS];
         stats.dt_col = dt_col;
         stats.clock_dyn = duration;
         if (diag && dg->log.is_open())

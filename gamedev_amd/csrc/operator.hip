@@ -1037,7 +1037,10 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
     if (threadIdx.x == 0)
     {
         if (tally[0] || tally[4])
+        {
             atomicAdd(GP->counters + GDX_CTR_PAIRS, (unsigned long long)tally[0]*N_K + tally[4]);
+            atomicAdd(GP->counters + GDX_CTR_CAND_PAIRS, (unsigned long long)tally[0]*N_K + tally[4]);
+        }
         if (tally[0]) atomicAdd(GP->counters + GDX_CTR_CANDIDATES, (unsigned long long)tally[0]);
         if (tally[1])
         {

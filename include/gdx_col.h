@@ -92,6 +92,7 @@ typedef struct {
     long long exact_checks;         /* owner baths whose exact rate was compared with the majorant */
     long long violations;           /* of those, exact rate above the majorant */
     long long rated_owners;         /* owner baths rated with the exact K-sum by rate passes */
+    long long candidate_pairs;      /* of pair_evals, those of the thinning-candidate pass */
 } gdx_col_stats;
 
 /* library and model */
