@@ -198,6 +198,11 @@ class CollisionOperator:
     def set_leaf_target(self, leaf_target: int) -> None:
         self._call(self.lib.gdx_col_set_leaf_target, int(leaf_target))
 
+    def set_bound_reuse(self, on: bool) -> None:
+        """Learned: keep the bounds of owners whose inputs did not change (default on); results do not
+        depend on it."""
+        self._call(self.lib.gdx_col_set_bound_reuse, int(bool(on)))
+
     def autotune(self, candidates=(16, 32, 64, 128, 256), repeats: int = 2) -> dict:
         """Time the geometry epoch per Morton leaf target, keep the fastest (a valid epoch is left built
         with it); returns {"best": leaf, "ms": {leaf: ms}}."""

@@ -108,6 +108,13 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.mlp = c.mlp;
     r.safety = c.safety*c.safety_scale;
     r.lam = c.lam.p;
+    r.dirty = c.dirty.p;
+    const bool track = c.model && c.rev_valid;
+    r.stale = track ? c.stale.p : nullptr;
+    r.rev_beg = track ? c.rev_beg.p : nullptr;
+    r.rev_end = track ? c.rev_end.p : nullptr;
+    r.rev = track ? c.rev.p : nullptr;
+    r.bound_reuse = c.bound_reuse;
     r.res32 = c.res32.p;
     r.qstart = c.qstart.p;
     return r;
@@ -616,6 +623,16 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
     const int ng = nr*LOCAL_GROUPS;
     rb.size_run(ng, nr);
     rb.h_table.clear();
+    for (op_run *r : runs)
+    {
+        // swarms with a majorant track which bounds their state changes touch: the reverse neighbour
+        // lists of the epoch once, and a full evaluation after the model's parameters change
+        context &c = r->c;
+        if (c.model && !c.rev_valid) build_reverse(c);
+        else if (c.model && majorant_version(c.model) != c.model_version) invalidate_bounds(c);
+    }
+    bool track = false;
+    for (op_run *r : runs) track |= r->c.model && r->c.rev_valid;
     for (int i = 0; i < nr; ++i)
     {
         op_run *r = runs[i];
@@ -661,7 +678,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
             launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
-            timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024); });
+            timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024, rb, track); });
             timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
             timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.wsub, pw.blocks_sub); });
             copy_d2h(rb.h_rate.data(), rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st);
@@ -1160,7 +1177,7 @@ int gdx_col_rate_pass(gdx_col_handle h)
         launch_reset_scratch(c.stream, rb.table.p, LOCAL_GROUPS);
         hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, c.stream), "zero rate bins");
         phase_wave pw = upload_phase(rb, e, c.stream);
-        launch_publish(c.stream, pw.w1024, pw.blocks1024);
+        launch_publish(c.stream, pw.w1024, pw.blocks1024, rb, c.model && c.rev_valid);
         launch_bath_rate(c.stream, pw.wsub, pw.blocks_sub);
         hip_check(hipStreamSynchronize(c.stream), "rating pass");
     });
@@ -1230,12 +1247,23 @@ int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int rep
     });
 }
 
+int gdx_col_set_bound_reuse(gdx_col_handle h, int on)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        h->c.bound_reuse = on ? 1 : 0;
+        invalidate_bounds(h->c);
+    });
+}
+
 int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety, int min_owners)
 {
     if (!h) return GDX_COL_ERR_ARG;
     return guard(&h->c, [&] {
         require(!m || (std::isfinite(safety) && safety >= 1.0), GDX_COL_ERR_ARG, "majorant safety factor must be >= 1");
         h->c.mlp = majorant_pack(m);
+        h->c.model = m;
+        invalidate_bounds(h->c);
         h->c.safety = m ? safety : 1.0;
         h->c.majorant_min_owners = min_owners > 0 ? min_owners : GDX_COL_MAJORANT_MIN_OWNERS;
     });

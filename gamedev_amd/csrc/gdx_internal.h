@@ -140,6 +140,8 @@ struct run_buffers
     dbuf<double> d_dt;
     dbuf<int> qprefix;
     dbuf<int> work;                   // chain pull counters: heavy queue, light queue
+    dbuf<int2> changed;               // a publish's changed slots {table index, slot} (stale marking)
+    dbuf<int> nchanged;
     int resident = 0;                 // resident blocks of the persistent chain kernel
     // controller sums, queue lengths, counters and errors of every group and swarm of the run, so that
     // each phase clears and reads them back with one call each
@@ -191,6 +193,7 @@ struct run_buffers
         d_dt.alloc(m);
         qprefix.alloc(2*(m + 1));
         work.alloc(2);
+        nchanged.alloc(1);
         cap = m;
     }
 };
@@ -266,9 +269,17 @@ struct context
     dbuf<double2> qstart;      // learned: {start time, event at the start} per queue position
     dbuf<int> cqueue;          // learned: thinning candidates per group slot range
     dbuf<double2> cstart;      // learned: {candidate time, bound} per candidate
-    dbuf<double> lam;          // learned: per-slot bound of the current bath
+    dbuf<double> lam;          // learned: per-slot S * softplus(net) of the last evaluation
+    dbuf<unsigned char> dirty; // state changed since the slot's last publish
+    dbuf<unsigned char> stale; // learned: bound inputs changed since the slot's last evaluation
+    dbuf<unsigned> rev_beg, rev_end;   // learned: reverse neighbour lists of the geometry epoch
+    dbuf<int> rev;
+    bool rev_valid = false;
     dbuf<float2> res32;        // published {size, number} in FP32 for the majorant's partner gathers
     const void *mlp = nullptr; // learned: packed network of the attached majorant (null: plain)
+    gdx_col_majorant model = nullptr;
+    unsigned long long model_version = 0;   // the model's version the cached bounds were computed with
+    int bound_reuse = 1;       // learned: keep the bounds of owners whose inputs are unchanged
     double safety = 1.2;
     double safety_scale = 1.0; // learned: adaptive factor on the bound, kept across operator calls
     int majorant_min_owners = GDX_COL_MAJORANT_MIN_OWNERS;   // smaller swarms keep the plain method
@@ -308,7 +319,7 @@ void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int
 void kd_build(context &c);
 void kd_query(context &c);
 void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng);
-void launch_publish(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb, bool mark);
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix);
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);
@@ -316,6 +327,9 @@ void launch_zero(hipStream_t st, const zero_ranges &z);
 void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, double *rate);
 // ---- majorant.hip ---------------------------------------------------------------------------
 const void *majorant_pack(gdx_col_majorant m);
+unsigned long long majorant_version(gdx_col_majorant m);
+void invalidate_bounds(context &c);
+void build_reverse(context &c);
 int chain_resident_blocks();
 void launch_chain(hipStream_t st, const gdx_wave &w, int *work, int resident);
 void launch_chain_audit(hipStream_t st, const gdx_wave &w, int blocks);

@@ -110,7 +110,14 @@ struct gdx_gref
     // learned majorant (mlp null: the plain method)
     const void *mlp;                          // packed network of the swarm's majorant
     double safety;                            // bound = safety * S * softplus(net)
-    double *lam;                              // per slot: the bound of the current bath
+    double *lam;                              // per slot: S * softplus(net) of the owner's last evaluation
+                                              // (the bound before the safety factors)
+    unsigned char *dirty;                     // per slot: state changed since the slot's last publish
+    unsigned char *stale;                     // per slot: bound inputs changed since its last evaluation
+                                              // (null: not tracked, the swarm has no majorant)
+    const unsigned *rev_beg, *rev_end;        // per slot: its range in rev
+    const int *rev;                           // the owners whose partner rows hold each slot
+    int bound_reuse;                          // keep the bounds of owners whose inputs are unchanged
     float2 *res32;                            // per slot: published {size, number} in FP32 (feature gathers)
     double2 *qstart;                          // per queue position: {start time, 1 = event at the start}
     int *cqueue;                              // thinning candidates of the bath, in the group's slot range

@@ -884,6 +884,7 @@ void geometry_epoch(context &c)
         for (int d = 0; d < LOCAL_GROUPS; ++d)
             if (graph[g] >> d & 1ull) c.edges[size_t(g)*LOCAL_WORDS + d / 32] |= 1u << (d % 32);
     c.geometry_valid = true;
+    c.rev_valid = false;
     load_state(c);
 }
 
@@ -902,6 +903,8 @@ void load_state(context &c)
     c.cqueue.alloc(na);
     c.cstart.alloc(na);
     c.lam.alloc(na);
+    c.dirty.alloc(na);
+    c.stale.alloc(na);
     c.qsum.alloc(na);
     if (c.diag) c.evw.alloc(na);
     c.qevents.alloc(na);
@@ -913,7 +916,64 @@ void load_state(context &c)
                                                                     c.rng.p, c.env.p, c.hgs.p);
         hip_check(hipGetLastError(), "launch gather state");
     }
+    // a new state: no kept bound is valid, no publish is pending
+    hip_check(hipMemsetAsync(c.dirty.p, 0, na, st), "reset dirty flags");
+    invalidate_bounds(c);
     c.state_loaded = true;
+}
+
+void invalidate_bounds(context &c)
+{
+    // on the context's stream, completed before any run (batches launch on the first handle's stream)
+    if (c.stale.p)
+    {
+        hip_check(hipMemsetAsync(c.stale.p, 1, c.stale.n, c.stream), "mark bounds stale");
+        hip_check(hipStreamSynchronize(c.stream), "mark bounds stale");
+    }
+    c.model_version = majorant_version(c.model);
+}
+
+// reverse neighbour lists of the geometry epoch: for every slot j the owners whose rows hold j
+__global__ __launch_bounds__(256) void k_rev_count(const int *nbr, long long entries, unsigned *count)
+{
+    const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= entries) return;
+    const int j = nbr[i];
+    if (j >= 0) atomicAdd(&count[j], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_rev_fill(const int *nbr, long long entries, unsigned *cursor, int *rev)
+{
+    const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= entries) return;
+    const int j = nbr[i];
+    if (j >= 0) rev[atomicAdd(&cursor[j], 1u)] = int(i / N_K);
+}
+
+void build_reverse(context &c)
+{
+    const int n = c.n_active;
+    const size_t na = size_t(std::max(1, n));
+    hipStream_t st = c.stream;
+    c.rev_beg.alloc(na);
+    c.rev_end.alloc(na);
+    c.rev.alloc(na*N_K);
+    if (n > 0)
+    {
+        const long long entries = (long long)n*N_K;
+        const unsigned blocks = unsigned((entries + 255) / 256);
+        hip_check(hipMemsetAsync(c.rev_end.p, 0, sizeof(unsigned)*na, st), "zero reverse counts");
+        k_rev_count<<<blocks, 256, 0, st>>>(c.nbr.p, entries, c.rev_end.p);
+        hip_check(hipGetLastError(), "launch reverse count");
+        scan_exclusive_u32(c.rev_end.p, c.rev_beg.p, n, c.scratch, st);
+        hip_check(hipMemcpyAsync(c.rev_end.p, c.rev_beg.p, sizeof(unsigned)*na, hipMemcpyDeviceToDevice, st),
+                  "reverse cursors");
+        // after the fill each cursor is the end of its slot's list
+        k_rev_fill<<<blocks, 256, 0, st>>>(c.nbr.p, entries, c.rev_end.p, c.rev.p);
+        hip_check(hipGetLastError(), "launch reverse fill");
+    }
+    c.rev_valid = true;
+    invalidate_bounds(c);
 }
 
 void store_state(context &c)

@@ -282,6 +282,7 @@ struct model
     dbuf<pack> P;
     dbuf<int> perm;
     long long steps = 0;
+    unsigned long long version = 0;     // advanced by every change of the parameters or normalization
     std::vector<float> h_norm = std::vector<float>(2*F);
     std::string last_error;
 };
@@ -444,6 +445,7 @@ int gdx_col_majorant_set_params(gdx_col_majorant h, const float *params, const f
         k_mlp_pack<<<1, 256>>>(M.p.p, M.norm.p, M.P.p);
         gdx::hip_check(hipGetLastError(), "majorant pack");
         gdx::hip_check(hipDeviceSynchronize(), "majorant pack");
+        ++M.version;
     });
 }
 
@@ -491,6 +493,7 @@ int gdx_col_majorant_train(gdx_col_majorant h, const float *features, const floa
         check_arg(n <= (1ll << 31) - 1, "training set too large");
         check_arg(std::isfinite(cfg->lr) && cfg->lr > 0.0, "learning rate must be positive");
         model &M = h->m;
+        ++M.version;
         const int B = cfg->batch;
         const long long batches = n / B;               // the last partial minibatch is dropped
         int waves = grid_waves(B);
@@ -542,5 +545,6 @@ int gdx_col_majorant_train(gdx_col_majorant h, const float *features, const floa
 namespace gdx {
 
 const void *majorant_pack(gdx_col_majorant m) { return m ? static_cast<const void *>(m->m.P.p) : nullptr; }
+unsigned long long majorant_version(gdx_col_majorant m) { return m ? m->m.version : 0ull; }
 
 }  // namespace gdx

@@ -185,14 +185,18 @@ __global__ void k_reset_scratch(const gdx_gref *table, int ng)
 }
 
 // col_bath_init + col_comp_zero + col_size_scan (col_bath_init.cu:11-23, col_size_scan.cu:13-25)
-__global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w)
+__global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, int2 *changed, int *nchanged)
 {
     __shared__ unsigned long long red[2][WAVES];
+    __shared__ int list[PUBLISH_OWNERS];
+    __shared__ int count, base;
     int gi, s0, s1;
     if (!due_block(w, PUBLISH_OWNERS, gi, s0, s1)) return;
     const gdx_gref G = w.table[w.ref[gi]];
     const double2 *st = G.st;
     double2 *res = G.res;
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
     unsigned long long lo = ~0ull, hi = 0ull;
     for (int s = s0 + threadIdx.x; s < s1; s += BLOCK)
     {
@@ -201,6 +205,13 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w)
         G.res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
         // positive doubles order like their bit patterns (col_size_scan.cu:20-24)
         if (v.x > 0 && isfinite(v.x)) { lo = min(lo, dbits(v.x)); hi = max(hi, dbits(v.x)); }
+        // a changed record: the bounds of the slot and of every owner whose row holds it go stale
+        // (k_mark_stale, from the list of changed slots)
+        if (G.dirty[s])
+        {
+            G.dirty[s] = 0;
+            if (G.stale) list[atomicAdd(&count, 1)] = s;
+        }
     }
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1)
@@ -219,6 +230,26 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w)
             atomicMin(G.scr_min, lo);
             atomicMax(G.scr_max, hi);
         }
+        base = count ? atomicAdd(nchanged, count) : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += BLOCK) changed[base + i] = make_int2(w.ref[gi], list[i]);
+}
+
+// stale flags of the changed slots of a publish and of the owners whose rows hold them (one K-lane
+// segment per changed slot, the reverse list in strides of K)
+__global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, const int2 *changed, const int *nchanged)
+{
+    const int n = *nchanged;
+    const int kl = threadIdx.x % SEG;
+    constexpr int SEGS = BLOCK / SEG;
+    for (int v = blockIdx.x*SEGS + int(threadIdx.x) / SEG; v < n; v += gridDim.x*SEGS)
+    {
+        const int2 c = changed[v];
+        const gdx_gref &G = table[c.x];
+        if (kl == 0) G.stale[c.y] = 1;
+        const unsigned b = G.rev_beg[c.y], e = G.rev_end[c.y];
+        for (unsigned i = b + kl; i < e; i += SEG) G.stale[G.rev[i]] = 1;
     }
 }
 
@@ -385,7 +416,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         if (sub >= 0)
         {
             const bool check = own && live && V;
-            const bool viol = check && !(R <= GP->lam[s]);
+            const bool viol = check && !(R <= GP->safety*GP->lam[s]);
             const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
             if (lane == 0 && nc) atomicAdd(&checked[0], nc);
             if (lane == 0 && nv) atomicAdd(&checked[1], nv);
@@ -542,25 +573,46 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
     return (COAG_KERNEL == 3 ? double(W) : G.lambda_0*double(W)) / m;
 }
 
-// bound of every due owner of a learned swarm: features per lane, then the network over the
-// wavefront's 64 owners as four 16-owner tiles on the matrix cores
+// bound of every due owner of a learned swarm. An owner whose state and partner records are unchanged
+// since its last evaluation keeps it (stale flags: set by the publishes that change a record, for the
+// slot and for every owner whose row holds it, cleared here). The others are listed in LDS and evaluated
+// 64 per wavefront: features per lane, then the network over the wavefront's 64 owners as four 16-owner
+// tiles on the matrix cores.
 __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
 {
     __shared__ float feat[BLOCK][mlp::F + 1];
     __shared__ float ys[BLOCK];
+    __shared__ int list[BLOCK];
+    __shared__ int count;
     int gi, s0, s1;
     if (!due_block(w, RATE_OWNERS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const int s = s0 + int(threadIdx.x);
-    const bool own = s < s1;
+    const bool stale = s < s1 && (!G.bound_reuse || !G.stale || G.stale[s]);
+    const unsigned long long ball = __ballot(stale);
+    int base = 0;
+    if (lane == 0 && ball) base = atomicAdd(&count, __popcll(ball));
+    base = __shfl(base, 0, WAVE);
+    if (stale) list[base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
+    __syncthreads();
+    const int n = count;
+    // wavefront-uniform: the wavefronts past the list are done, no block barrier follows
+    if (wv*WAVE >= n) return;
+    const bool has = int(threadIdx.x) < n;
+    const int so = has ? list[threadIdx.x] : s0;
     float f[mlp::F];
     double S = 0.0;
-    if (own) S = owner_features(G, s, f);
+    if (has) S = owner_features(G, so, f);
     else for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
     for (int k = 0; k < mlp::F; ++k) feat[threadIdx.x][k] = f[k];
-    __syncthreads();
+    // the tiles read only this wavefront's rows
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const mlp::pack *P = static_cast<const mlp::pack *>(G.mlp);
-    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
 #pragma unroll
     for (int tt = 0; tt < WAVE / 16; ++tt)
     {
@@ -569,8 +621,14 @@ __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
         float z = mlp::tile_forward(P, mlp::input_fragment(P, &feat[row][0], true), fs);
         if (lane < 16) ys[row] = mlp::softplus(z);
     }
-    __syncthreads();
-    if (own) G.lam[s] = G.safety*S*static_cast<double>(ys[threadIdx.x]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (has)
+    {
+        G.lam[so] = S*static_cast<double>(ys[threadIdx.x]);
+        if (G.stale) G.stale[so] = 0;
+    }
 }
 
 // training samples: features, scale and the cached exact rate of every active owner, scattered to
@@ -701,7 +759,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     if (valid && learned && !exact_owner(w, gi, s))
     {
         thin = measure[s] > 0.0;
-        const double L = thin ? G.lam[s] : 0.0;
+        const double L = thin ? G.safety*G.lam[s] : 0.0;
         if (thin && (!isfinite(L) || L < 0.0))
         {
             // unusable bound: the owner takes the plain chain from the bath start
@@ -985,6 +1043,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
                             GP->evw[s].log_mass[cat] += log_mass;
                         }
                         GP->st[s] = make_double2(size_new, numr_new);
+                        GP->dirty[s] = 1;
                         // finished: the wait draw is committed; else the chain continues after the event
                         // and repeats it
                         ctr = done ? k2 : k;
@@ -1389,7 +1448,11 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         {
             if (kl == 0)
             {
-                if (events > 0) G->st[s] = make_double2(t.size_i, numr_i);
+                if (events > 0)
+                {
+                    G->st[s] = make_double2(t.size_i, numr_i);
+                    G->dirty[s] = 1;
+                }
                 G->rng[s] = pool_base + uint64_t(pool_used);
                 if (code) atomicMax(G->err, code);
                 G->qsum[pos] = make_double4(hazard, j1, j2, jmax);
@@ -1510,11 +1573,19 @@ void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng)
     hip_check(hipGetLastError(), "launch reset scratch");
 }
 
-void launch_publish(hipStream_t st, const gdx_wave &w, int blocks)
+// mark: some swarm of the wave tracks stale bounds (has a majorant)
+void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb, bool mark)
 {
     if (!blocks) return;
-    k_publish<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    rb.changed.alloc(size_t(std::max(1, w.total)));
+    hip_check(hipMemsetAsync(rb.nchanged.p, 0, sizeof(int), st), "zero changed count");
+    k_publish<<<xcd_grid(blocks), BLOCK, 0, st>>>(w, rb.changed.p, rb.nchanged.p);
     hip_check(hipGetLastError(), "launch publish");
+    if (!mark) return;
+    constexpr int SEGS = BLOCK / SEG;
+    const int grid = std::max(1, std::min(4096, (w.total + SEGS - 1) / SEGS));
+    k_mark_stale<<<grid, BLOCK, 0, st>>>(w.table, rb.changed.p, rb.nchanged.p);
+    hip_check(hipGetLastError(), "launch mark stale");
 }
 
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks)

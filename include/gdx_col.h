@@ -252,6 +252,11 @@ int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, do
  * outlive its use by h. */
 #define GDX_COL_MAJORANT_MIN_OWNERS 262144
 int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety, int min_owners);
+/* bounds across refresh waves and calls: with reuse on (the default) an owner whose state and partner
+ * records have not changed since its last evaluation keeps its bound instead of running the network
+ * again (a change of the attached model's parameters invalidates every kept bound); off evaluates every
+ * due owner. Results do not depend on it. */
+int gdx_col_set_bound_reuse(gdx_col_handle h, int on);
 
 #ifdef __cplusplus
 }
