@@ -679,7 +679,8 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
             timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024, rb, track); });
-            timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
+            if (pw.blocks_maj > 0)
+                timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
             timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.wsub, pw.blocks_sub); });
             copy_d2h(rb.h_rate.data(), rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st);
             if (pw.wsub.learned)
