@@ -12,8 +12,11 @@ operator (a majorant trained on samples of ten plain steps of the same swarm, Po
 FP64 acceptance); the plain operator on the same swarm is timed first and reported beside it, with the
 batched ensemble (64 realizations, plain and learned) and the clumped batched stress case (64
 realizations totalling 1e7, vortex and ring overdensities) as secondary legs.
-The roofline object is for the kernel with the largest measured time: its algorithmic bytes per
-launch (owners x bytes per owner, listed in KERNEL_BYTES) over its mean HIP-event duration.
+The timed steps run without per-kernel events; each timed leg is followed by --profile-steps calls of
+the same workload with HIP events around every kernel launch (on the library's launch stream), which
+give the per-kernel times. The roofline object is for the kernel with the largest measured time: its
+algorithmic bytes per launch (owners x bytes per owner, listed in KERNEL_BYTES) over its mean HIP-event
+duration.
 The CPU baseline runs the oracle (host restatement of the same operator, OpenMP) on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 3]
@@ -323,6 +326,17 @@ def timed_steps(op, steps: int, horizon: float, label: str) -> tuple[float, dict
     return time.perf_counter() - t0, tot, updates
 
 
+def profiled_window(op, steps: int, horizon: float, label: str) -> tuple[dict, dict, int]:
+    """Kernel durations: `steps` further operator calls of the same workload with HIP events around every
+    kernel launch. The timed steps run without them (the events cost about 0.2 ms per step)."""
+    op.set_profile(True)
+    op.reset_kernel_times()
+    _, tot, _ = timed_steps(op, steps, horizon, f"{label} (kernel timing)")
+    kt = op.kernel_times()
+    op.set_profile(False)
+    return kt, tot, steps
+
+
 def roofline_of(kt: dict, tot: dict, dom: str | None = None) -> dict:
     dom = dom or max(kt, key=lambda k: kt[k]["ms"])
     rec = kt[dom]
@@ -346,6 +360,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="operator calls with per-kernel HIP events after each timed leg (kernel times, roofline)")
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--horizon", type=float, default=0.05)
     ap.add_argument("--cpu-sample", type=int, default=400_000)
@@ -388,7 +404,7 @@ def main() -> int:
     log(f"generating {args.n} representatives")
     st = synth.fiducial_3d_state(args.n, seed=0)
     d = synth.to_device(st, dev)
-    op = CollisionOperator("fiducial_3d", args.n, st["total_mass"], seed=0, defer_writeback=True, profile=True)
+    op = CollisionOperator("fiducial_3d", args.n, st["total_mass"], seed=0, defer_writeback=True)
     op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
     tune = None
     if args.autotune:
@@ -405,11 +421,10 @@ def main() -> int:
     for i in range(args.warmup):
         op.evolve(args.horizon)
     log("warmup done")
-    op.reset_kernel_times()
     elapsed, tot, updates = timed_steps(op, args.steps, args.horizon, "plain")
     plain_value = updates/elapsed
-    kt = op.kernel_times()
     log(f"plain: {plain_value:.4e} updates/s, {1e3*elapsed/args.steps:.3f} ms/step")
+    kt, p_tot, p_steps = profiled_window(op, args.profile_steps, args.horizon, "plain")
 
     # with a geometry rebuild before every operator call (transported runs)
     torch.cuda.synchronize()
@@ -428,11 +443,10 @@ def main() -> int:
         op.use_majorant(model, safety)
         for i in range(args.warmup):
             op.evolve(args.horizon)
-        op.reset_kernel_times()
         l_elapsed, l_tot, l_updates = timed_steps(op, args.steps, args.horizon, "learned")
-        l_kt = op.kernel_times()
         learned_value = l_updates/l_elapsed
         log(f"learned: {learned_value:.4e} updates/s, {1e3*l_elapsed/args.steps:.3f} ms/step")
+        l_kt, lp_tot, lp_steps = profiled_window(op, args.profile_steps, args.horizon, "learned")
         learned = {"value": learned_value, "ms_per_step": 1e3*l_elapsed/args.steps,
                    "learned_over_plain": learned_value/plain_value, "training": train_info,
                    "violation_rate": l_tot["violations"]/max(1, l_tot["exact_checks"]),
@@ -441,8 +455,8 @@ def main() -> int:
                    "exact_pair_evals_per_update": l_tot["pair_evals"]/max(1, l_updates),
                    "events_per_s": l_tot["events"]/l_elapsed, "waves_per_step": l_tot["waves"]/args.steps,
                    "longest_chain_events": l_tot["longest_chain"],
-                   "kernel_ms_per_step": {k: v["ms"]/args.steps for k, v in l_kt.items()},
-                   "roofline": roofline_of(l_kt, l_tot), "totals": l_tot}
+                   "kernel_ms_per_step": {k: v["ms"]/lp_steps for k, v in l_kt.items()},
+                   "roofline": roofline_of(l_kt, lp_tot), "totals": l_tot}
         op.use_majorant(None)
 
     batched = None
@@ -457,13 +471,13 @@ def main() -> int:
         clumped = clumped_leg(args.clumped, args.clumped_total, args.clumped_steps, args.horizon, dev,
                               args.train_samples, args.epochs)
 
-    roofline_plain = roofline_of(kt, tot)
+    roofline_plain = roofline_of(kt, p_tot)
     plain = {"value": plain_value, "ms_per_step": 1e3*elapsed/args.steps, "roofline": roofline_plain,
              "events_per_s": tot["events"]/elapsed, "pair_evals_per_s": tot["pair_evals"]/elapsed,
              "mean_baths_per_owner_step": tot["owner_updates"]/max(1, updates),
              "waves_per_step": tot["waves"]/args.steps, "chain_owners_per_step": tot["chain_owners"]/args.steps,
              "longest_chain_events": tot["longest_chain"],
-             "kernel_ms_per_step": {k: v["ms"]/args.steps for k, v in kt.items()}}
+             "kernel_ms_per_step": {k: v["ms"]/p_steps for k, v in kt.items()}}
     workload = (f"fiducial 3D disk, single swarm of {args.n} representatives, K={K}, coagulation+fragmentation, "
                 f"h_op={args.horizon}, operator-only with the state resident in HBM")
     if learned is not None:

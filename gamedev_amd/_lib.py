@@ -113,6 +113,7 @@ _SIGS = {
     "gdx_col_use_majorant": [P, P, D, I],
     "gdx_col_set_leaf_target": [P, I],
     "gdx_col_set_bound_reuse": [P, I],
+    "gdx_col_set_profile": [P, I],
     "gdx_col_autotune": [P, P, I, I, P, P],
 }
 EXPORTS = tuple(_SIGS) + ("gdx_col_last_error", "gdx_col_majorant_last_error")

@@ -198,6 +198,10 @@ class CollisionOperator:
     def set_leaf_target(self, leaf_target: int) -> None:
         self._call(self.lib.gdx_col_set_leaf_target, int(leaf_target))
 
+    def set_profile(self, on: bool) -> None:
+        """Kernel timing (HIP events per launch) of later operator calls on or off."""
+        self._call(self.lib.gdx_col_set_profile, int(bool(on)))
+
     def set_bound_reuse(self, on: bool) -> None:
         """Learned: keep the bounds of owners whose inputs did not change (default on); results do not
         depend on it."""

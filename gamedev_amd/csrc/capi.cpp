@@ -197,6 +197,15 @@ struct kernel_timer
     struct pending { hipEvent_t a, b; int kind; long long owners; };
     std::vector<pending> open;
     explicit kernel_timer(context &ctx) : c(ctx) {}
+    // events of a call that ended early go back to the pool untimed
+    ~kernel_timer()
+    {
+        for (const pending &p : open)
+        {
+            c.events.push_back(p.a);
+            c.events.push_back(p.b);
+        }
+    }
     bool on() const { return (c.flags & GDX_COL_PROFILE) != 0; }
     hipEvent_t take()
     {
@@ -686,7 +695,6 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             if (pw.wsub.learned)
                 copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st);
             hip_check(hipStreamSynchronize(st), "rating pass");
-            timer.resolve();
             for (op_run *r : now) r->after_rate();
             upload_changed(now);
         }
@@ -715,7 +723,6 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             copy_d2h(rb.h_audit.data(), rb.raw_audit.p, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st);
             copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st);
             hip_check(hipStreamSynchronize(st), "chain pass");
-            timer.resolve();
             for (op_run *r : now) r->after_chain();
             upload_changed(now);
         }
@@ -731,6 +738,9 @@ void run_operator(context &c, diag_state *dg, run_buffers &rb, double duration, 
     kernel_timer timer(c);
     run_swarms(runs, rb, c.stream, timer);
     r.finish(c.stream, rb.h_counters.data(), out);
+    // the call's kernel times, once the stream has drained (not between phases, where the host's
+    // event queries would hold up the next launch)
+    timer.resolve();
 }
 
 }  // namespace gdx
@@ -917,6 +927,7 @@ int gdx_col_operator_batch(const gdx_col_handle *handles, int count, double dura
         kernel_timer timer(h0->c);
         run_swarms(runs, h0->rb, h0->c.stream, timer);
         for (int i = 0; i < count; ++i) runs[i]->finish(h0->c.stream, h0->rb.h_counters.data() + size_t(GDX_CTRS)*i, out ? out + i : nullptr);
+        timer.resolve();
     });
 }
 
@@ -1245,6 +1256,15 @@ int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int rep
         geometry_epoch(c);
         hip_check(hipStreamSynchronize(c.stream), "autotune");
         if (best) *best = pick;
+    });
+}
+
+int gdx_col_set_profile(gdx_col_handle h, int on)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        if (on) h->c.flags |= GDX_COL_PROFILE;
+        else h->c.flags &= ~GDX_COL_PROFILE;
     });
 }
 

@@ -124,6 +124,8 @@ int gdx_col_store_state(gdx_col_handle h);
  * morton_index::build's leaf_target argument, morton_index.cuh:168-229). Neighbour rows, slot order and
  * every operator result do not depend on it; only the search time does. */
 int gdx_col_set_leaf_target(gdx_col_handle h, int leaf_target);
+/* kernel timing of later operator calls on or off (the GDX_COL_PROFILE flag of the configuration) */
+int gdx_col_set_profile(gdx_col_handle h, int on);
 /* autotuner (no reference counterpart; the reference fixes MORTON_LEAF_TARGET per model): times the
  * geometry epoch of the bound state for each candidate leaf target (best of `repeats`, host wall clock
  * around a synchronised epoch), keeps the fastest, and leaves a valid epoch built with it. A deferred
