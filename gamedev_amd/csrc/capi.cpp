@@ -166,9 +166,9 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
     b256[n] = y;
     bsub[n] = z;
     bmaj[n] = u;
-    hip_check(hipMemcpyAsync(rb.d_int.p, rb.h_int, sizeof(int)*(run_buffers::DUE_INTS*size_t(n) + run_buffers::DUE_EXTRA),
+    // ints and bath ends in one copy
+    hip_check(hipMemcpyAsync(rb.due_d.p, rb.due_h.p, rb.due_ints_bytes + sizeof(double)*std::max(1, n),
                              hipMemcpyHostToDevice, st), "due list");
-    hip_check(hipMemcpyAsync(rb.d_dt.p, rb.h_dt, sizeof(double)*std::max(1, n), hipMemcpyHostToDevice, st), "due list");
     phase_wave pw;
     const int *d = rb.d_int.p;
     auto wave = [&](const int *bstart) {
@@ -691,9 +691,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             if (pw.blocks_maj > 0)
                 timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
             timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.wsub, pw.blocks_sub); });
-            copy_d2h(rb.h_rate.data(), rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng, st);
-            if (pw.wsub.learned)
-                copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st);
+            rb.read_back(rb.off_rate, rb.off_err, st);             // rate bins and wave counters
             hip_check(hipStreamSynchronize(st), "rating pass");
             for (op_run *r : now) r->after_rate();
             upload_changed(now);
@@ -719,9 +717,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256, rb.qprefix.p); });
             timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.work.p, rb.resident); });
             timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w256, pw.blocks256); });
-            copy_d2h(rb.h_err.data(), rb.err.p, sizeof(int)*nr, st);
-            copy_d2h(rb.h_audit.data(), rb.raw_audit.p, sizeof(gdx_raw_audit)*COL_BIN_S*ng, st);
-            copy_d2h(rb.h_wave.data(), rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr, st);
+            rb.read_back(rb.off_wave, rb.back_bytes, st);          // wave counters, errors, audit bins
             hip_check(hipStreamSynchronize(st), "chain pass");
             for (op_run *r : now) r->after_chain();
             upload_changed(now);

@@ -127,70 +127,94 @@ struct kd_tree
 
 // buffers of operator runs: the group table, pinned staging and device copies of the due lists,
 // the chain's queue prefix and work counter
+// a typed view into one of run_buffers' shared allocations
+template <typename T>
+struct view
+{
+    T *p = nullptr;
+    T *data() { return p; }
+    const T *data() const { return p; }
+};
+
 struct run_buffers
 {
-    // due-list layout per phase: ref, beg, cnt, sub (n each), then four block-start arrays (n + 1 each)
+    // due-list layout per phase: ref, beg, cnt, sub (n each), then four block-start arrays (n + 1 each),
+    // then the bath ends (doubles): one page-locked block, one device block, one copy per phase
     static constexpr int DUE_INTS = 8, DUE_EXTRA = 4;
     dbuf<gdx_gref> table;
     std::vector<gdx_gref> h_table;
+    pinned<unsigned char> due_h;
+    dbuf<unsigned char> due_d;
     int *h_int = nullptr;
     double *h_dt = nullptr;
+    view<int> d_int;
+    view<double> d_dt;
+    size_t due_ints_bytes = 0;
     size_t cap = 0;
-    dbuf<int> d_int;
-    dbuf<double> d_dt;
     dbuf<int> qprefix;
     dbuf<int> work;                   // chain pull counters: heavy queue, light queue
     dbuf<int2> changed;               // a publish's changed slots {table index, slot} (stale marking)
     dbuf<int> nchanged;
     int resident = 0;                 // resident blocks of the persistent chain kernel
-    // controller sums, queue lengths, counters and errors of every group and swarm of the run, so that
-    // each phase clears and reads them back with one call each
-    dbuf<gdx_raw_rate> raw_rate;
-    dbuf<gdx_raw_audit> raw_audit;
+    // controller sums, queue lengths, counters and errors of every group and swarm of the run. The
+    // read-backs share one device block and one page-locked mirror laid out [raw_rate][wave_ctr][err]
+    // [raw_audit], so that the rate phase reads [raw_rate, wave_ctr] and the chain phase [wave_ctr,
+    // raw_audit] with one copy each
+    dbuf<unsigned char> back_d;
+    pinned<unsigned char> back_h;
+    size_t off_rate = 0, off_wave = 0, off_err = 0, off_audit = 0, back_bytes = 0;
+    view<gdx_raw_rate> raw_rate, h_rate;
+    view<gdx_raw_audit> raw_audit, h_audit;
+    view<int> err, h_err;
+    view<unsigned long long> wave_ctr, h_wave;
     dbuf<int> qcount, hcount, ccount;
-    dbuf<int> err;
-    dbuf<unsigned long long> counters, wave_ctr;
-    pinned<gdx_raw_rate> h_rate;
-    pinned<gdx_raw_audit> h_audit;
-    pinned<int> h_err;
-    pinned<unsigned long long> h_counters, h_wave;
+    dbuf<unsigned long long> counters;
+    pinned<unsigned long long> h_counters;
     void size_run(int groups, int swarms)
     {
-        raw_rate.alloc(size_t(groups)*COL_BIN_S);
-        raw_audit.alloc(size_t(groups)*COL_BIN_S);
+        auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
+        off_rate = 0;
+        off_wave = align(off_rate + sizeof(gdx_raw_rate)*size_t(groups)*COL_BIN_S);
+        off_err = align(off_wave + sizeof(unsigned long long)*size_t(swarms)*GDX_WAVE_CTRS);
+        off_audit = align(off_err + sizeof(int)*size_t(swarms));
+        back_bytes = align(off_audit + sizeof(gdx_raw_audit)*size_t(groups)*COL_BIN_S);
+        back_d.alloc(back_bytes);
+        back_h.resize(back_bytes);
+        raw_rate.p = reinterpret_cast<gdx_raw_rate *>(back_d.p + off_rate);
+        wave_ctr.p = reinterpret_cast<unsigned long long *>(back_d.p + off_wave);
+        err.p = reinterpret_cast<int *>(back_d.p + off_err);
+        raw_audit.p = reinterpret_cast<gdx_raw_audit *>(back_d.p + off_audit);
+        h_rate.p = reinterpret_cast<gdx_raw_rate *>(back_h.p + off_rate);
+        h_wave.p = reinterpret_cast<unsigned long long *>(back_h.p + off_wave);
+        h_err.p = reinterpret_cast<int *>(back_h.p + off_err);
+        h_audit.p = reinterpret_cast<gdx_raw_audit *>(back_h.p + off_audit);
         qcount.alloc(groups);
         hcount.alloc(groups);
         ccount.alloc(groups);
-        err.alloc(swarms);
         counters.alloc(size_t(swarms)*GDX_CTRS);
-        wave_ctr.alloc(size_t(swarms)*GDX_WAVE_CTRS);
-        h_rate.resize(size_t(groups)*COL_BIN_S);
-        h_audit.resize(size_t(groups)*COL_BIN_S);
-        h_err.resize(swarms);
         h_counters.resize(size_t(swarms)*GDX_CTRS);
-        h_wave.resize(size_t(swarms)*GDX_WAVE_CTRS);
+    }
+    // device-to-host copy of the read-back bytes [from, to) (offsets above)
+    void read_back(size_t from, size_t to, hipStream_t st)
+    {
+        hip_check(hipMemcpyAsync(back_h.p + from, back_d.p + from, to - from, hipMemcpyDeviceToHost, st), "read back");
     }
     run_buffers() = default;
     run_buffers(const run_buffers &) = delete;
     run_buffers &operator=(const run_buffers &) = delete;
-    ~run_buffers()
-    {
-        if (h_int) (void)hipHostFree(h_int);
-        if (h_dt) (void)hipHostFree(h_dt);
-    }
     // room for n due entries
     void reserve(size_t n)
     {
         if (n <= cap && h_int) return;
         size_t m = std::max<size_t>(n, 64);
-        if (h_int) (void)hipHostFree(h_int);
-        if (h_dt) (void)hipHostFree(h_dt);
-        h_int = nullptr;
-        h_dt = nullptr;
-        hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_int), sizeof(int)*(DUE_INTS*m + DUE_EXTRA)), "pinned due list");
-        hip_check(hipHostMalloc(reinterpret_cast<void **>(&h_dt), sizeof(double)*m), "pinned due list");
-        d_int.alloc(DUE_INTS*m + DUE_EXTRA);
-        d_dt.alloc(m);
+        due_ints_bytes = (sizeof(int)*(DUE_INTS*m + DUE_EXTRA) + 15) & ~size_t(15);
+        const size_t bytes = due_ints_bytes + sizeof(double)*m;
+        due_h.resize(bytes);
+        due_d.alloc(bytes);
+        h_int = reinterpret_cast<int *>(due_h.p);
+        h_dt = reinterpret_cast<double *>(due_h.p + due_ints_bytes);
+        d_int.p = reinterpret_cast<int *>(due_d.p);
+        d_dt.p = reinterpret_cast<double *>(due_d.p + due_ints_bytes);
         qprefix.alloc(2*(m + 1));
         work.alloc(2);
         nchanged.alloc(1);
