@@ -1348,9 +1348,14 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         const pair_term p = moments ? eval_pair_terms(t, cm, q, lambda_0) : eval_pair_rate(t, q, lambda_0);
         const bool bad_rate = !isfinite(p.rate) || p.rate < 0.0;
         double R = seg_fold<false>(p.rate);
-        double J1 = seg_fold<false>(p.j1);
-        double J2 = seg_fold<false>(p.j2);
-        double JM = seg_fold<true>(p.jmax);
+        // the jump moments only where a segment of the wavefront keeps compensators
+        double J1 = 0.0, J2 = 0.0, JM = 0.0;
+        if (__ballot(moments))
+        {
+            J1 = seg_fold<false>(p.j1);
+            J2 = seg_fold<false>(p.j2);
+            JM = seg_fold<true>(p.jmax);
+        }
         // last slot with a positive rate (-1: none), -2 when any rate of the segment is invalid
         int LAST = seg_max_u(p.rate > 0.0 ? slot_k + 1 : 0) - 1;
         if (__ballot(bad_rate) & seg_mask) LAST = -2;
