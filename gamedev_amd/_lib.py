@@ -85,6 +85,7 @@ _SIGS = {
     "gdx_col_reset_kernel_times": [P],
     "gdx_col_active_count": [P, C.POINTER(I)],
     "gdx_col_get_neighbors": [P, P, P],
+    "gdx_col_record_info": [P, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_float)],
     "gdx_col_get_sites": [P, P, P, P],
     "gdx_col_get_morton": [P, P, P],
     "gdx_col_get_kdtree": [P, P, P, P],

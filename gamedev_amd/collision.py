@@ -161,9 +161,17 @@ class CollisionOperator:
         self._call(self.lib.gdx_col_get_sites, _np_ptr(pts), _np_ptr(act), _np_ptr(grp))
         return pts, act, grp
 
+    def record_info(self) -> dict:
+        """Indexed records of the last Morton epoch (n plus periodic ghosts in a wedge), whether images of
+        one particle are disjoint, and the largest search cap."""
+        r, u, q = C.c_int(0), C.c_int(0), C.c_float(0.0)
+        self._call(self.lib.gdx_col_record_info, C.byref(r), C.byref(u), C.byref(q))
+        return {"records": r.value, "unique_ids": bool(u.value), "q_max": q.value}
+
     def morton(self) -> tuple[np.ndarray, np.ndarray]:
-        keys = np.empty(self.n, dtype=np.uint64)
-        idx = np.empty(self.n, dtype=np.int32)
+        m = self.record_info()["records"]
+        keys = np.empty(m, dtype=np.uint64)
+        idx = np.empty(m, dtype=np.int32)
         self._call(self.lib.gdx_col_get_morton, _np_ptr(keys), _np_ptr(idx))
         return keys, idx
 

@@ -783,6 +783,11 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         g_last_error = "search must be GDX_COL_SEARCH_MORTON or GDX_COL_SEARCH_KDTREE";
         return GDX_COL_ERR_ARG;
     }
+    if (X_WEDGE && cfg->search == GDX_COL_SEARCH_KDTREE)
+    {
+        g_last_error = "an azimuthal wedge model needs periodic ghosts: only GDX_COL_SEARCH_MORTON is built for it";
+        return GDX_COL_ERR_ARG;
+    }
     gdx_col_ctx *h = nullptr;
     int rc = guard(nullptr, [&] {
         h = new gdx_col_ctx();
@@ -1014,6 +1019,16 @@ int gdx_col_active_count(gdx_col_handle h, int *n_active)
     return GDX_COL_OK;
 }
 
+int gdx_col_record_info(gdx_col_handle h, int *records, int *unique_ids, float *q_max)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    if (!h->c.geometry_valid) { h->c.last_error = "collision geometry has not been built"; return GDX_COL_ERR_STATE; }
+    if (records) *records = h->c.index.n;
+    if (unique_ids) *unique_ids = h->c.unique_ids ? 1 : 0;
+    if (q_max) *q_max = h->c.q_max;
+    return GDX_COL_OK;
+}
+
 int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure)
 {
     if (!h) return GDX_COL_ERR_ARG;
@@ -1022,11 +1037,13 @@ int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure)
         require(c.geometry_valid, GDX_COL_ERR_STATE, "collision geometry has not been built");
         const size_t na = size_t(c.n_active);
         std::vector<int> nbr(na*N_K), slot_pidx(na), pidx_slot(c.n);
+        std::vector<unsigned char> image(X_WEDGE ? na*N_K : 0);
         std::vector<double> meas(na);
         hipStream_t st = c.stream;
         if (na)
         {
             copy_d2h(nbr.data(), c.nbr.p, sizeof(int)*na*N_K, st);
+            if (X_WEDGE) copy_d2h(image.data(), c.nbr_image.p, na*N_K, st);
             copy_d2h(meas.data(), c.measure.p, sizeof(double)*na, st);
             copy_d2h(slot_pidx.data(), c.slot_pidx.p, sizeof(int)*na, st);
         }
@@ -1040,8 +1057,9 @@ int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure)
                 int v = -1;
                 if (s >= 0)
                 {
-                    int j = nbr[size_t(s)*N_K + bitrev_slot(k)];
-                    v = j < 0 ? -1 : 3*slot_pidx[j];
+                    const size_t e = size_t(s)*N_K + bitrev_slot(k);
+                    int j = nbr[e];
+                    v = j < 0 ? -1 : 3*slot_pidx[j] + (X_WEDGE ? int(image[e]) : 0);
                 }
                 if (codes) codes[size_t(p)*N_K + k] = v;
             }
@@ -1069,8 +1087,8 @@ int gdx_col_get_morton(gdx_col_handle h, unsigned long long *keys, int *idx)
     return guard(&h->c, [&] {
         context &c = h->c;
         require(c.geometry_valid, GDX_COL_ERR_STATE, "collision geometry has not been built");
-        if (keys) copy_d2h(keys, c.index.keys.p, sizeof(uint64_t)*c.n, c.stream);
-        if (idx) copy_d2h(idx, c.index.idx.p, sizeof(int)*c.n, c.stream);
+        if (keys) copy_d2h(keys, c.index.keys.p, sizeof(uint64_t)*c.index.n, c.stream);
+        if (idx) copy_d2h(idx, c.index.idx.p, sizeof(int)*c.index.n, c.stream);
         hip_check(hipStreamSynchronize(c.stream), "read morton");
     });
 }

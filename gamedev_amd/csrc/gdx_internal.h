@@ -267,6 +267,14 @@ struct context
     dbuf<int> slot_pidx;       // slot -> particle
     dbuf<unsigned long long> graph;   // per group: 64-bit dependency mask
     dbuf<unsigned int> overflow;
+    // azimuthal wedge: physical records then seam ghosts, their codes 3*i + image, ghost counts and offsets
+    dbuf<float4> wedge_rec;
+    dbuf<int> wedge_code;
+    dbuf<unsigned int> ghost_count, ghost_off;
+    dbuf<unsigned char> nbr_image;  // n_active x N_K image of each neighbour (slot order)
+    int n_records = 0;              // indexed records: n, plus ghosts in a wedge
+    float q_max = 0.0f;             // largest search cap of the epoch (wedge seam test)
+    bool unique_ids = true;         // images of one particle can never share a search ball
     int n_active = 0;
     std::vector<int> group_beg, group_cnt, group_all;   // slot ranges of active owners; all particles per group
     std::vector<uint32_t> edges;                        // LOCAL_GROUPS x LOCAL_WORDS
@@ -331,7 +339,8 @@ struct context
 void scan_exclusive_u32(const unsigned int *in, unsigned int *out, int n, sort_scratch &s, hipStream_t st);
 void radix_sort_pairs(uint64_t *keys, int *vals, int n, int begin_bit, int end_bit, sort_scratch &s, hipStream_t st);
 void morton_build(morton_index &ix, const float4 *points, const unsigned char *active, int n, float ox, float oy,
-                  float oz, float width, int dim, int leaf_target, int max_level, sort_scratch &s, hipStream_t st);
+                  float oz, float width, int dim, int leaf_target, int max_level, sort_scratch &s, hipStream_t st,
+                  const int *codes = nullptr);
 void geometry_epoch(context &c);
 void load_state(context &c);
 void store_state(context &c);

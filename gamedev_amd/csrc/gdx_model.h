@@ -11,10 +11,10 @@ namespace gdx {
 
 constexpr double PI = 3.14159265358979323846;
 constexpr int DIM = (N_Z > 1) ? 3 : 2;
-// an azimuthal wedge needs periodic images (inc/swarm/const_defs.cuh:56-57); only full disks are built
+// an azimuthal wedge needs periodic images (inc/swarm/const_defs.cuh:56-57): records near either seam
+// get a rotated ghost whose code 3*i + image keeps the physical identifier (inc/swarm/_col_image.cuh:11-26)
 constexpr bool X_WEDGE = N_X > 1
     && static_cast<float>(X_MAX) - static_cast<float>(X_MIN) < 6.28318530717958647692f - 1.0e-06f;
-static_assert(!X_WEDGE, "azimuthal wedges (periodic Morton ghosts) are not built by this library");
 // spatial controller groups (inc/swarm/_col_types.cuh:123-125)
 constexpr int LOCAL_GROUPS = ((N_X > 1) ? COL_BIN_X : 1)*COL_BIN_Y*((N_Z > 1) ? COL_BIN_Z : 1);
 constexpr int LOCAL_WORDS = (LOCAL_GROUPS + 31) / 32;

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 
 #pragma clang fp contract(off)
@@ -272,16 +273,79 @@ __global__ __launch_bounds__(256) void k_morton_keys(const float4 *pts, int n, f
     vals[i] = int(i);
 }
 
+// sorted record m carries its code 3*i + image; the activity of physical particle i marks it
 __global__ __launch_bounds__(256) void k_records(const int *sidx, const float4 *pts, const unsigned char *active,
-                                                int n, float4 *rec)
+                                                const int *codes, int n, float4 *rec)
 {
     long long m = (long long)blockIdx.x*blockDim.x + threadIdx.x;
     if (m >= n) return;
     int p = sidx[m];
     float4 q = pts[p];
-    int code = 3*p;
-    if (active && !active[p]) code |= int(0x80000000u);
+    int code = codes ? codes[p] : 3*p;
+    if (active && !active[code / 3]) code |= int(0x80000000u);
     rec[m] = make_float4(q.x, q.y, q.z, __int_as_float(code));
+}
+
+// ============================================================================================
+// azimuthal wedge: periodic ghosts (inc/swarm/morton/morton_ghost.cuh:28-86)
+// ============================================================================================
+// Cartesian distance from a record to the radial half-plane at azimuth offset `off` from it
+// (_get_morton_seam_dist, morton_ghost.cuh:28-35)
+__device__ __forceinline__ float seam_dist(float px, float py, float off)
+{
+    float R = hypotf(px, py);
+    return (cosf(off) >= 0.0f) ? R*fabsf(sinf(off)) : R;
+}
+
+// largest search cap of the epoch (the max_element of swarm_runtime.cu:413-414); caps are >= 0
+__global__ __launch_bounds__(256) void k_cap_max(const float4 *pts, int n, unsigned int *out)
+{
+    long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    float w = i < n ? pts[i].w : 0.0f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) w = fmaxf(w, __shfl_xor(w, o, WAVE));
+    if (threadIdx.x % WAVE == 0) atomicMax(out, __float_as_uint(w));
+}
+
+// ghosts per particle: one for each seam whose distance is within the largest cap (morton_gcount)
+__global__ __launch_bounds__(256) void k_ghost_count(bound_state u, const float4 *pts, int n, float q_max,
+                                                    float x_min, float x_max, unsigned int *count)
+{
+    long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pts[i];
+    float posx = static_cast<float>(ld(u.x, i));
+    unsigned c = 0;
+    if (seam_dist(p.x, p.y, posx - x_min) <= q_max) ++c;
+    if (seam_dist(p.x, p.y, x_max - posx) <= q_max) ++c;
+    count[i] = c;
+}
+
+// physical records first (code 3i), then each particle's ghosts at n + offset: image 2 rotated by
+// +width near x_min, image 1 rotated by -width near x_max (morton_gwrite); the rotation's sine and
+// cosine are computed once on the host
+__global__ __launch_bounds__(256) void k_ghost_write(bound_state u, const float4 *pts, int n, const unsigned int *off,
+                                                    float q_max, float x_min, float x_max, float cw, float sw,
+                                                    float4 *rec, int *code)
+{
+    long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pts[i];
+    rec[i] = p;
+    code[i] = 3*int(i);
+    float posx = static_cast<float>(ld(u.x, i));
+    long long g = n + off[i];
+    if (seam_dist(p.x, p.y, posx - x_min) <= q_max)
+    {
+        rec[g] = make_float4(cw*p.x - sw*p.y, sw*p.x + cw*p.y, p.z, 0.0f);
+        code[g] = 3*int(i) + 2;
+        ++g;
+    }
+    if (seam_dist(p.x, p.y, x_max - posx) <= q_max)
+    {
+        rec[g] = make_float4(cw*p.x + sw*p.y, -sw*p.x + cw*p.y, p.z, 0.0f);
+        code[g] = 3*int(i) + 1;
+    }
 }
 
 __device__ __forceinline__ int child_end(const uint64_t *keys, int begin, int end, int shift, int code)
@@ -492,11 +556,35 @@ __device__ __forceinline__ void wave_centre(float qx, float qy, float qz, bool q
 }
 
 // col_cache_get (inc/swarm/_col_cache.cuh:73-159) over Morton-ordered queries
+// drop every later image of a physical particle already listed, keep the first K (the id sort,
+// duplicate mask and re-sort of _morton_ghost_topk, morton_query.cuh:62-140: in (d2, code) order the
+// first image of a particle is its nearest one)
+template <int KC>
+__device__ __forceinline__ void dedup_images(const float (&dc)[KC], const int (&cc)[KC], float (&d)[N_K], int (&c)[N_K])
+{
+#pragma unroll
+    for (int k = 0; k < N_K; ++k) { d[k] = __builtin_inff(); c[k] = INT_MAX; }
+    int ph[KC];
+#pragma unroll
+    for (int i = 0; i < KC; ++i) ph[i] = cc[i] / 3;
+#pragma unroll
+    for (int i = 0; i < KC; ++i)
+    {
+        bool dup = cc[i] == INT_MAX;
+#pragma unroll
+        for (int j = 0; j < i; ++j) dup = dup || (ph[j] == ph[i]);
+        if (!dup && nb_less(dc[i], cc[i], d[N_K - 1], c[N_K - 1])) topk_insert<N_K>(d, c, dc[i], cc[i]);
+    }
+}
+
+// one query per physical record; KC = N_K (disjoint images) or 3 N_K (a narrow wedge, then deduplicated)
+template <int KC>
 __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, const gdx_node *nodes, int n, int dim,
                                                         int max_level, const int *sidx, const float4 *pts,
                                                         const int *pidx_slot, const unsigned char *group,
                                                         bound_state u, int *nbr, double *measure,
-                                                        unsigned long long *graph, unsigned int *overflow)
+                                                        unsigned long long *graph, unsigned int *overflow,
+                                                        unsigned char *nbr_image)
 {
     __shared__ int stacks[TOPK_BLOCK / WAVE][TOPK_STACK];
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
@@ -504,7 +592,7 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
     bool valid = m < n;
     float4 me = valid ? rec[m] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
     int code = __float_as_int(me.w);
-    bool qactive = valid && code >= 0;
+    bool qactive = valid && code >= 0 && (!X_WEDGE || code % 3 == 0);
     int p = valid ? sidx[m] : 0;
     float q = qactive ? pts[p].w : 0.0f;
     float q2 = q*q;
@@ -512,7 +600,19 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
     wave_centre(me.x, me.y, me.z, qactive, cx, cy, cz);
     float d[N_K];
     int c[N_K];
-    topk_wave<N_K>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, d, c, stacks[wave], overflow);
+    if constexpr (KC == N_K)
+    {
+        topk_wave<N_K>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, d, c, stacks[wave],
+                       overflow);
+    }
+    else
+    {
+        float dc[KC];
+        int cc[KC];
+        topk_wave<KC>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, dc, cc, stacks[wave],
+                      overflow);
+        dedup_images<KC>(dc, cc, d, c);
+    }
     unsigned long long mask = 0ull;
     int g = qactive ? int(group[p]) : -1;
     if (qactive)
@@ -540,6 +640,15 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
         for (int t = 0; t < N_K; t += 4)
             dst[t / 4] = make_int4(row[bitrev_slot(t)], row[bitrev_slot(t + 1)], row[bitrev_slot(t + 2)],
                                    row[bitrev_slot(t + 3)]);
+        if constexpr (X_WEDGE)
+        {
+#pragma unroll
+            for (int t = 0; t < N_K; ++t)
+            {
+                const int k = bitrev_slot(t);
+                nbr_image[(long long)s*N_K + t] = static_cast<unsigned char>(c[k] != INT_MAX ? c[k] % 3 : 0);
+            }
+        }
         double radius = sqrt(static_cast<double>(max_d2));
         measure[s] = ball_measure(ld(u.y, p), ld(u.z, p), radius);
     }
@@ -587,12 +696,12 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_knn(const float4 *rec, const gdx
 // slot order and per-slot state
 // ============================================================================================
 __global__ __launch_bounds__(256) void k_slot_keys(const int *sidx, const unsigned char *active,
-                                                  const unsigned char *group, int n, uint64_t *keys, int *vals)
+                                                  const unsigned char *group, int nr, int n, uint64_t *keys, int *vals)
 {
     long long m = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (m >= n) return;
+    if (m >= nr) return;
     int p = sidx[m];
-    keys[m] = active[p] ? uint64_t(group[p]) : 255ull;
+    keys[m] = p >= n ? 256ull : active[p] ? uint64_t(group[p]) : 255ull;
     vals[m] = p;
 }
 
@@ -734,7 +843,8 @@ void radix_sort_pairs(uint64_t *keys, int *vals, int n, int begin_bit, int end_b
 
 // morton_index::build (inc/swarm/morton/morton_index.cuh:168-229)
 void morton_build(morton_index &ix, const float4 *points, const unsigned char *active, int n, float ox, float oy,
-                  float oz, float width, int dim, int leaf_target, int max_level, sort_scratch &s, hipStream_t st)
+                  float oz, float width, int dim, int leaf_target, int max_level, sort_scratch &s, hipStream_t st,
+                  const int *codes)
 {
     if (n <= 0 || !(width > 0.0f)) throw status_error(12, "invalid adaptive Morton size");
     if (dim != 2 && dim != 3) throw status_error(12, "adaptive Morton dimension must be 2 or 3");
@@ -748,7 +858,7 @@ void morton_build(morton_index &ix, const float4 *points, const unsigned char *a
     k_morton_keys<<<blocks_for(n, 256), 256, 0, st>>>(points, n, ox, oy, oz, width, max_level, dim, ix.keys.p, ix.idx.p);
     hip_check(hipGetLastError(), "launch morton keys");
     radix_sort_pairs(ix.keys.p, ix.idx.p, n, 0, 3*max_level, s, st);
-    k_records<<<blocks_for(n, 256), 256, 0, st>>>(ix.idx.p, points, active, n, ix.rec.p);
+    k_records<<<blocks_for(n, 256), 256, 0, st>>>(ix.idx.p, points, active, codes, n, ix.rec.p);
     hip_check(hipGetLastError(), "launch morton records");
 
     size_t cap = std::max<size_t>(1024, size_t(n) / size_t(std::max(1, leaf_target))*4 + 64);
@@ -800,6 +910,52 @@ void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int
     hip_check(hipGetLastError(), "launch knn");
 }
 
+// morton_ghost_index::build with ghosts (morton_ghost.cuh:96-168) and the unique-identifier test of
+// swarm_runtime.cu:331-340, 413-415: count, scan, write physical records then ghosts
+static void wedge_records(context &c)
+{
+    const int n = c.n;
+    hipStream_t st = c.stream;
+    const float x_min = static_cast<float>(X_MIN), x_max = static_cast<float>(X_MAX);
+    dbuf<unsigned int> cap;
+    cap.alloc(1);
+    hip_check(hipMemsetAsync(cap.p, 0, sizeof(unsigned int), st), "zero cap maximum");
+    k_cap_max<<<blocks_for(n, 256), 256, 0, st>>>(c.pts.p, n, cap.p);
+    c.ghost_count.alloc(size_t(n));
+    c.ghost_off.alloc(size_t(n));
+    unsigned int bits = 0;
+    hip_check(hipMemcpyAsync(&bits, cap.p, sizeof bits, hipMemcpyDeviceToHost, st), "read cap maximum");
+    hip_check(hipStreamSynchronize(st), "cap maximum");
+    std::memcpy(&c.q_max, &bits, sizeof bits);
+    // _get_image_dist_min (inc/swarm/param_grid.cuh:86-89): the closest any two images of one particle get
+    // (each sine rounded once from double to float, as in the ghost rotation below)
+    auto sinf1 = [](float a) { return static_cast<float>(std::sin(static_cast<double>(a))); };
+    const float image_dist_min = 2.0f*static_cast<float>(Y_MIN)
+        *std::fmin(sinf1(static_cast<float>(Z_MIN)), sinf1(static_cast<float>(Z_MAX)))
+        *std::fabs(sinf1(0.5f*(x_max - x_min)));
+    c.unique_ids = image_dist_min < 0.0f || image_dist_min > 2.0f*c.q_max;
+    k_ghost_count<<<blocks_for(n, 256), 256, 0, st>>>(c.user, c.pts.p, n, c.q_max, x_min, x_max, c.ghost_count.p);
+    hip_check(hipGetLastError(), "launch ghost count");
+    scan_exclusive_u32(c.ghost_count.p, c.ghost_off.p, n, c.scratch, st);
+    unsigned int last[2] = {0u, 0u};
+    hip_check(hipMemcpyAsync(&last[0], c.ghost_count.p + (n - 1), sizeof(unsigned int), hipMemcpyDeviceToHost, st),
+              "read last ghost count");
+    hip_check(hipMemcpyAsync(&last[1], c.ghost_off.p + (n - 1), sizeof(unsigned int), hipMemcpyDeviceToHost, st),
+              "read last ghost offset");
+    hip_check(hipStreamSynchronize(st), "ghost offsets");
+    const long long total = (long long)n + last[0] + last[1];
+    if (total > INT_MAX / 3) throw status_error(12, "wedge record count exceeds the index limit");
+    c.n_records = int(total);
+    c.wedge_rec.alloc(size_t(total));
+    c.wedge_code.alloc(size_t(total));
+    const float width = x_max - x_min;
+    const float sw = static_cast<float>(std::sin(static_cast<double>(width)));
+    const float cw = static_cast<float>(std::cos(static_cast<double>(width)));
+    k_ghost_write<<<blocks_for(n, 256), 256, 0, st>>>(c.user, c.pts.p, n, c.ghost_off.p, c.q_max, x_min, x_max, cw,
+                                                     sw, c.wedge_rec.p, c.wedge_code.p);
+    hip_check(hipGetLastError(), "launch ghost write");
+}
+
 // the evolve_collisions geometry branch (src/swarm/swarm_runtime.cu:384-448) for bound particle state
 void geometry_epoch(context &c)
 {
@@ -828,19 +984,32 @@ void geometry_epoch(context &c)
     // root cube of morton_ghost_index::build_index: width 2.0002*Y_MAX centred on the origin
     float root_width = 2.0002f*static_cast<float>(Y_MAX);
     float o = -0.5f*root_width;
-    morton_build(c.index, c.pts.p, c.active.p, n, o, o, o, root_width, DIM, c.leaf_target, MORTON_MAX_LEVEL,
-                 c.scratch, st);
+    c.n_records = n;
+    c.unique_ids = true;
+    if constexpr (X_WEDGE)
+    {
+        wedge_records(c);
+        morton_build(c.index, c.wedge_rec.p, c.active.p, c.n_records, o, o, o, root_width, DIM, c.leaf_target,
+                     MORTON_MAX_LEVEL, c.scratch, st, c.wedge_code.p);
+    }
+    else
+    {
+        morton_build(c.index, c.pts.p, c.active.p, n, o, o, o, root_width, DIM, c.leaf_target, MORTON_MAX_LEVEL,
+                     c.scratch, st);
+    }
+    const int nr = c.n_records;
 
-    // slot order: active owners grouped by controller group, Morton order inside each group
+    // slot order: active owners grouped by controller group, Morton order inside each group; ghost
+    // records sort last (key 256) and are cut off
     dbuf<uint64_t> gkeys;
     dbuf<int> gvals;
     dbuf<int> gcount;
-    gkeys.alloc(n);
-    gvals.alloc(n);
+    gkeys.alloc(nr);
+    gvals.alloc(nr);
     gcount.alloc(2*LOCAL_GROUPS);
     hip_check(hipMemsetAsync(gcount.p, 0, sizeof(int)*2*LOCAL_GROUPS, st), "zero group counts");
-    k_slot_keys<<<blocks_for(n, 256), 256, 0, st>>>(c.index.idx.p, c.active.p, c.group.p, n, gkeys.p, gvals.p);
-    radix_sort_pairs(gkeys.p, gvals.p, n, 0, 8, c.scratch, st);
+    k_slot_keys<<<blocks_for(nr, 256), 256, 0, st>>>(c.index.idx.p, c.active.p, c.group.p, nr, n, gkeys.p, gvals.p);
+    radix_sort_pairs(gkeys.p, gvals.p, nr, 0, X_WEDGE ? 16 : 8, c.scratch, st);
     k_slot_maps<<<blocks_for(n, 256), 256, 0, st>>>(gvals.p, c.group.p, c.active.p, n, c.slot_pidx.p, c.pidx_slot.p,
                                                    gcount.p, gcount.p + LOCAL_GROUPS);
     hip_check(hipGetLastError(), "launch slot maps");
@@ -867,10 +1036,17 @@ void geometry_epoch(context &c)
     }
     else
     {
-        int waves = blocks_for(n, WAVE);
-        k_col_topk<<<xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE)), TOPK_BLOCK, 0, st>>>(
-            c.index.rec.p, c.index.nodes.p, n, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
-            c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p);
+        int waves = blocks_for(nr, WAVE);
+        dim3 grid = xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE));
+        if (X_WEDGE) c.nbr_image.alloc(na*N_K);
+        if (c.unique_ids)
+            k_col_topk<N_K><<<grid, TOPK_BLOCK, 0, st>>>(
+                c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
+                c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p);
+        else
+            k_col_topk<X_WEDGE ? 3*N_K : N_K><<<grid, TOPK_BLOCK, 0, st>>>(
+                c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
+                c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p);
         hip_check(hipGetLastError(), "launch top-K");
     }
     unsigned int ovf = 0;

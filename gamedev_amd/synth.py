@@ -38,6 +38,21 @@ def plain_2d_state(n: int = 65536, seed: int = 0, total_mass: float = 1e-4, size
     return {"x": phi, "y": r, "z": z, "size": s, "numr": numr, "total_mass": total_mass}
 
 
+def wedge_2d_state(n: int = 65536, seed: int = 0, r_min: float = 0.5, r_max: float = 1.5, x_min: float = -0.1,
+                   x_max: float = 0.1, s_min: float = 0.05, s_max: float = 6.4) -> dict:
+    """Vertically integrated azimuthal wedge: r uniform, azimuth uniform over the wedge, log-uniform sizes,
+    equal-mass representatives carrying 1% of a unit-surface-density gas wedge."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = rng.uniform(r_min, r_max, n)
+    phi = rng.uniform(x_min, x_max, n)
+    z = np.full(n, 0.5*math.pi)
+    s = np.exp(rng.uniform(math.log(s_min), math.log(s_max), n))
+    total_mass = 0.01*0.5*(x_max - x_min)*(r_max**2 - r_min**2)
+    w = total_mass/n
+    numr = w/grain_mass(s)
+    return {"x": phi, "y": r, "z": z, "size": s, "numr": numr, "total_mass": total_mass}
+
+
 def fiducial_3d_state(n: int = 10_000_000, seed: int = 0, r_min: float = 0.3, r_max: float = 3.0,
                       sigma_ln: float = 1.0) -> dict:
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -164,15 +164,23 @@ int gdx_col_reset_kernel_times(gdx_col_handle h);
 
 /* ---- inspection (host buffers, particle order) ---- */
 int gdx_col_active_count(gdx_col_handle h, int *n_active);
-/* neighbour codes 3*j (-1 empty), n x N_K in (d^2, code) order, and the KNN measure (0 if inactive) */
+/* neighbour codes 3*j + image (-1 empty; image 0 unless the model is an azimuthal wedge: 2 = the
+ * partner rotated by +width, 1 = by -width), n x N_K in (d^2, code) order, and the KNN measure (0 if
+ * inactive); the col_cache_get output (inc/swarm/_col_cache.cuh:135-155) */
 int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure);
+/* indexed records of the last Morton epoch (n, plus periodic ghosts in a wedge), whether the images of
+ * one particle are disjoint (no deduplication needed) and the largest search cap; the record count,
+ * unique_ids and max_search_dist of morton_ghost_index::build (inc/swarm/morton/morton_ghost.cuh:96-168,
+ * src/swarm/swarm_runtime.cu:413-415) */
+int gdx_col_record_info(gdx_col_handle h, int *records, int *unique_ids, float *q_max);
 /* collision sites: float {x, y, z, search cap} x n, active flags, spatial group */
 int gdx_col_get_sites(gdx_col_handle h, float *pts4, unsigned char *active, unsigned char *group);
 /* k-d tree of the last geometry epoch (search GDX_COL_SEARCH_KDTREE): particle index and split
  * dimension per node (n each) and the world box {lo xyz, hi xyz}; test access to
  * kdtree::buildTree (inc/swarm/kdtree/builder_thrust.h:418-526) */
 int gdx_col_get_kdtree(gdx_col_handle h, int *node_idx, int *node_dim, float *world6);
-/* Morton keys and record order after the stable sort (n each) */
+/* Morton keys and record order after the stable sort (one per indexed record: n, or the record count
+ * of gdx_col_record_info in a wedge; record i >= n is a ghost) */
 int gdx_col_get_morton(gdx_col_handle h, unsigned long long *sorted_keys, int *sorted_idx);
 /* bath-start rate moments {rate (-1 invalid), first, second, maximum} per particle (0 for inactive) */
 int gdx_col_get_cached(gdx_col_handle h, double *cached4);

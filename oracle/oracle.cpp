@@ -1286,6 +1286,96 @@ void orc_topk_brute(int n, const float *pts, const uint8_t *active, int nq, cons
     }
 }
 
+// ---- azimuthal wedge (periodic images) ----
+// _get_image_dist_min (inc/swarm/param_grid.cuh:86-89): the closest two images of one particle get
+float orc_image_dist_min()
+{
+    const float x_min = float(X_MIN), x_max = float(X_MAX);
+    auto sinf1 = [](float a) { return float(std::sin(double(a))); };
+    return 2.0f*float(Y_MIN)*std::fmin(sinf1(float(Z_MIN)), sinf1(float(Z_MAX)))*std::fabs(sinf1(0.5f*(x_max - x_min)));
+}
+
+// _get_morton_seam_dist (inc/swarm/morton/morton_ghost.cuh:28-35)
+static float seam_dist(const float *p, float off)
+{
+    float R = std::hypot(p[0], p[1]);
+    return (std::cos(off) >= 0.0f) ? R*std::fabs(std::sin(off)) : R;
+}
+
+// morton_gcount + exclusive scan + morton_gwrite (morton_ghost.cuh:38-86, 122-153): physical records
+// 0..n-1 with codes 3i, then per particle in index order its image 2 (rotated by +width, near x_min)
+// and image 1 (rotated by -width, near x_max). rec and code hold room for 3n; returns the count.
+int orc_wedge_records(int n, const float *pts, const float *posx, float q_max, float *rec, int *code)
+{
+    const float x_min = float(X_MIN), x_max = float(X_MAX);
+    const float width = x_max - x_min;
+    // sin and cos of the float width, each rounded once to float
+    const float sw = float(std::sin(double(width))), cw = float(std::cos(double(width)));
+    for (int i = 0; i < n; i++)
+    {
+        rec[3*i] = pts[3*i]; rec[3*i + 1] = pts[3*i + 1]; rec[3*i + 2] = pts[3*i + 2];
+        code[i] = 3*i;
+    }
+    int g = n;
+    for (int i = 0; i < n; i++)
+    {
+        const float *p = pts + 3*size_t(i);
+        if (seam_dist(p, posx[i] - x_min) <= q_max)
+        {
+            rec[3*g] = cw*p[0] - sw*p[1]; rec[3*g + 1] = sw*p[0] + cw*p[1]; rec[3*g + 2] = p[2];
+            code[g++] = 3*i + 2;
+        }
+        if (seam_dist(p, x_max - posx[i]) <= q_max)
+        {
+            rec[3*g] = cw*p[0] + sw*p[1]; rec[3*g + 1] = -sw*p[0] + cw*p[1]; rec[3*g + 2] = p[2];
+            code[g++] = 3*i + 1;
+        }
+    }
+    return g;
+}
+
+// exact top-K over coded records by brute force; with dedup, _morton_ghost_topk (morton_query.cuh:62-140):
+// the 3K nearest records, each physical particle kept at its nearest image, the first K of those
+void orc_topk_records(int nr, const float *rec, const int *code, const uint8_t *active, int nq, const float *query_pts,
+                      const float *query_q, int k, int dedup, int *out_codes, float *out_d2, int threads)
+{
+    #pragma omp parallel for schedule(dynamic, 16) num_threads(thread_count(threads))
+    for (int q = 0; q < nq; q++)
+    {
+        std::vector<std::pair<float, int>> cand;
+        float qd = query_q[q];
+        float qd2 = qd*qd;
+        for (int r = 0; r < nr; r++)
+        {
+            if (!active[code[r] / 3]) continue;
+            float d2 = point_d2(query_pts + 3*size_t(q), rec + 3*size_t(r));
+            if (d2 <= qd2) cand.emplace_back(d2, code[r]);
+        }
+        auto less = [](const std::pair<float, int> &a, const std::pair<float, int> &b) {
+            return nb_less(a.first, a.second, b.first, b.second);
+        };
+        std::sort(cand.begin(), cand.end(), less);
+        if (dedup)
+        {
+            if (cand.size() > size_t(3*k)) cand.resize(size_t(3*k));
+            std::vector<std::pair<float, int>> kept;
+            for (size_t a = 0; a < cand.size(); a++)
+            {
+                bool dup = false;
+                for (size_t b = 0; b < a; b++) dup = dup || cand[b].second / 3 == cand[a].second / 3;
+                if (!dup) kept.push_back(cand[a]);
+            }
+            cand.swap(kept);
+        }
+        int take = std::min<int>(k, int(cand.size()));
+        for (int s = 0; s < k; s++)
+        {
+            out_codes[size_t(q)*k + s] = s < take ? cand[s].second : -1;
+            out_d2[size_t(q)*k + s] = s < take ? cand[s].first : std::numeric_limits<float>::infinity();
+        }
+    }
+}
+
 // exact top-K through the Morton hierarchy (traversal contract of _morton_topk, morton_index.cuh:493-679)
 void orc_topk_tree(int n, const float *pts, const uint8_t *active, const int *sorted_idx, int node_count,
                    const float *lower, const float *nwidth, const int *begin, const int *count, const int *child,

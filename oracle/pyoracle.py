@@ -58,6 +58,11 @@ class Oracle:
         L.orc_hierarchy_build.restype = i
         L.orc_hierarchy_build.argtypes = [i, P, C.c_float, C.c_float, C.c_float, C.c_float, i, i, i, i] + [P]*6
         L.orc_topk_brute.argtypes = [i, P, P, i, P, P, P, i, P, P, i]
+        L.orc_image_dist_min.restype = C.c_float
+        L.orc_image_dist_min.argtypes = []
+        L.orc_wedge_records.restype = i
+        L.orc_wedge_records.argtypes = [i, P, P, C.c_float, P, P]
+        L.orc_topk_records.argtypes = [i, P, P, P, i, P, P, i, i, P, P, i]
         L.orc_topk_tree.argtypes = [i, P, P, P, i, P, P, P, P, P, P, i, i, i, P, i, P, P, P, i]
         L.orc_space_bin.argtypes = [i, P, P, P, P]
         L.orc_kd_build.argtypes = [i, P, P, P, P]
@@ -169,6 +174,32 @@ class Oracle:
         d2 = np.empty((nq, k), np.float32)
         qi = np.zeros(nq, np.int32)
         self.lib.orc_topk_brute(len(pts), _p(pts), _p(act), nq, _p(qi), _p(qp), _p(qq), k, _p(codes), _p(d2), threads)
+        return codes, d2
+
+    def image_dist_min(self) -> np.float32:
+        return np.float32(self.lib.orc_image_dist_min())
+
+    def wedge_records(self, pts, posx, q_max):
+        """Physical records then seam ghosts (float xyz) and their codes 3i + image."""
+        pts = np.ascontiguousarray(pts, dtype=np.float32)
+        posx = np.ascontiguousarray(posx, dtype=np.float32)
+        n = len(pts)
+        rec = np.empty((3*n, 3), np.float32)
+        code = np.empty(3*n, np.int32)
+        m = self.lib.orc_wedge_records(n, _p(pts), _p(posx), C.c_float(q_max), _p(rec), _p(code))
+        return rec[:m].copy(), code[:m].copy()
+
+    def topk_records(self, rec, code, active, query_pts, query_q, k, dedup, threads=0):
+        rec = np.ascontiguousarray(rec, dtype=np.float32)
+        code = np.ascontiguousarray(code, dtype=np.int32)
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+        qp = np.ascontiguousarray(query_pts, dtype=np.float32)
+        qq = np.ascontiguousarray(query_q, dtype=np.float32)
+        nq = len(qp)
+        codes = np.empty((nq, k), np.int32)
+        d2 = np.empty((nq, k), np.float32)
+        self.lib.orc_topk_records(len(rec), _p(rec), _p(code), _p(act), nq, _p(qp), _p(qq), k, int(bool(dedup)),
+                                  _p(codes), _p(d2), threads)
         return codes, d2
 
     def topk_tree(self, pts, active, sorted_idx, nodes, dim, max_level, query_idx, k, search_dist, threads=0):

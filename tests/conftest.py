@@ -7,7 +7,7 @@ ROOT = Path(__file__).resolve().parent.parent
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
-MODELS = ("plain_2d", "fiducial_3d")
+MODELS = ("plain_2d", "fiducial_3d", "wedge_2d")
 
 
 def pytest_configure(config):
