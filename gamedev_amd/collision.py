@@ -177,8 +177,9 @@ class CollisionOperator:
 
     def kdtree(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """k-d tree of the last geometry epoch: particle index and split dimension per node, world box."""
-        idx = np.empty(self.n, dtype=np.int32)
-        dim = np.empty(self.n, dtype=np.int32)
+        m = self.record_info()["records"]
+        idx = np.empty(m, dtype=np.int32)
+        dim = np.empty(m, dtype=np.int32)
         world = np.empty(6, dtype=np.float32)
         self._call(self.lib.gdx_col_get_kdtree, _np_ptr(idx), _np_ptr(dim), _np_ptr(world))
         return idx, dim, world

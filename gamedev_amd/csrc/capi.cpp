@@ -783,11 +783,6 @@ int gdx_col_create(const gdx_col_config *cfg, gdx_col_handle *out)
         g_last_error = "search must be GDX_COL_SEARCH_MORTON or GDX_COL_SEARCH_KDTREE";
         return GDX_COL_ERR_ARG;
     }
-    if (X_WEDGE && cfg->search == GDX_COL_SEARCH_KDTREE)
-    {
-        g_last_error = "an azimuthal wedge model needs periodic ghosts: only GDX_COL_SEARCH_MORTON is built for it";
-        return GDX_COL_ERR_ARG;
-    }
     gdx_col_ctx *h = nullptr;
     int rc = guard(nullptr, [&] {
         h = new gdx_col_ctx();
@@ -1023,7 +1018,7 @@ int gdx_col_record_info(gdx_col_handle h, int *records, int *unique_ids, float *
 {
     if (!h) return GDX_COL_ERR_ARG;
     if (!h->c.geometry_valid) { h->c.last_error = "collision geometry has not been built"; return GDX_COL_ERR_STATE; }
-    if (records) *records = h->c.index.n;
+    if (records) *records = h->c.search == GDX_COL_SEARCH_KDTREE ? h->c.kd.n : h->c.index.n;
     if (unique_ids) *unique_ids = h->c.unique_ids ? 1 : 0;
     if (q_max) *q_max = h->c.q_max;
     return GDX_COL_OK;

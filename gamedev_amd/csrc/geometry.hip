@@ -910,9 +910,8 @@ void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int
     hip_check(hipGetLastError(), "launch knn");
 }
 
-// morton_ghost_index::build with ghosts (morton_ghost.cuh:96-168) and the unique-identifier test of
-// swarm_runtime.cu:331-340, 413-415: count, scan, write physical records then ghosts
-static void wedge_records(context &c)
+// the largest search cap and the unique-identifier test of swarm_runtime.cu:331-340, 413-415
+static void wedge_cap(context &c)
 {
     const int n = c.n;
     hipStream_t st = c.stream;
@@ -921,8 +920,6 @@ static void wedge_records(context &c)
     cap.alloc(1);
     hip_check(hipMemsetAsync(cap.p, 0, sizeof(unsigned int), st), "zero cap maximum");
     k_cap_max<<<blocks_for(n, 256), 256, 0, st>>>(c.pts.p, n, cap.p);
-    c.ghost_count.alloc(size_t(n));
-    c.ghost_off.alloc(size_t(n));
     unsigned int bits = 0;
     hip_check(hipMemcpyAsync(&bits, cap.p, sizeof bits, hipMemcpyDeviceToHost, st), "read cap maximum");
     hip_check(hipStreamSynchronize(st), "cap maximum");
@@ -934,6 +931,17 @@ static void wedge_records(context &c)
         *std::fmin(sinf1(static_cast<float>(Z_MIN)), sinf1(static_cast<float>(Z_MAX)))
         *std::fabs(sinf1(0.5f*(x_max - x_min)));
     c.unique_ids = image_dist_min < 0.0f || image_dist_min > 2.0f*c.q_max;
+}
+
+// morton_ghost_index::build with ghosts (morton_ghost.cuh:96-168): count, scan, write physical records
+// then ghosts
+static void wedge_records(context &c)
+{
+    const int n = c.n;
+    hipStream_t st = c.stream;
+    const float x_min = static_cast<float>(X_MIN), x_max = static_cast<float>(X_MAX);
+    c.ghost_count.alloc(size_t(n));
+    c.ghost_off.alloc(size_t(n));
     k_ghost_count<<<blocks_for(n, 256), 256, 0, st>>>(c.user, c.pts.p, n, c.q_max, x_min, x_max, c.ghost_count.p);
     hip_check(hipGetLastError(), "launch ghost count");
     scan_exclusive_u32(c.ghost_count.p, c.ghost_off.p, n, c.scratch, st);
@@ -986,7 +994,9 @@ void geometry_epoch(context &c)
     float o = -0.5f*root_width;
     c.n_records = n;
     c.unique_ids = true;
-    if constexpr (X_WEDGE)
+    // the Morton search of a wedge indexes seam ghosts; the k-d search adds its own images (kd_build)
+    if (X_WEDGE) wedge_cap(c);
+    if (X_WEDGE && c.search == GDX_COL_SEARCH_MORTON)
     {
         wedge_records(c);
         morton_build(c.index, c.wedge_rec.p, c.active.p, c.n_records, o, o, o, root_width, DIM, c.leaf_target,

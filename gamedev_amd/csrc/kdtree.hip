@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 
 #pragma clang fp contract(off)
 
@@ -222,17 +223,37 @@ __device__ __forceinline__ float d2_of(float ax, float ay, float az, float bx, f
 
 __device__ __forceinline__ float key_d2(uint64_t k) { return __uint_as_float(uint32_t(k >> 32)); }
 
-__global__ __launch_bounds__(KD_QUERY_BLOCK) void k_kd_topk(const float4 *node, const int *node_idx, int n,
+// col_site_init's wedge images (src/swarm/col_site_init.cu:42-62): physical points at [0, n), every point
+// rotated by -width (image 1) at [n, 2n) and by +width (image 2) at [2n, 3n)
+__global__ __launch_bounds__(KD_BLOCK) void k_kd_images(const float4 *pts, int n, float cw, float sw, float4 *out)
+{
+    const long long i = (long long)blockIdx.x*KD_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pts[i];
+    out[i] = p;
+#pragma unroll
+    for (int image = 0; image < 2; ++image)
+    {
+        const float sa = image == 0 ? -sw : sw;
+        out[i + (long long)(image + 1)*n] = make_float4(cw*p.x - sa*p.y, sa*p.x + cw*p.y, p.z, p.w);
+    }
+}
+
+// queries: the nq physical particles in Morton order; the tree holds nt records, record r being particle
+// r % n_phys at image r / n_phys when n_phys > 0 (a wedge), else particle r
+__global__ __launch_bounds__(KD_QUERY_BLOCK) void k_kd_topk(const float4 *node, const int *node_idx, int nt, int nq,
+                                                           int n_phys, float image_dist_min,
                                                            const float *world, const int *sidx, const float4 *pts,
                                                            const unsigned char *active, const int *pidx_slot,
                                                            const unsigned char *group, bound_state u, int *nbr,
                                                            double *measure, unsigned long long *graph,
-                                                           unsigned int *overflow)
+                                                           unsigned int *overflow, unsigned char *nbr_image)
 {
     const int lane = threadIdx.x % WAVE;
     const int m = blockIdx.x*KD_QUERY_BLOCK + threadIdx.x;
-    const int p = m < n ? sidx[m] : 0;
-    const bool qactive = m < n && active[p];
+    const int n = nt;
+    const int p = m < nq ? sidx[m] : 0;
+    const bool qactive = m < nq && active[p];
     unsigned long long mask = 0ull;
     int g = qactive ? int(group[p]) : -1;
     if (qactive)
@@ -240,6 +261,8 @@ __global__ __launch_bounds__(KD_QUERY_BLOCK) void k_kd_topk(const float4 *node, 
         const float4 qp = pts[p];
         uint64_t heap[N_K];
         const uint64_t empty = (uint64_t(__float_as_uint(qp.w*qp.w)) << 32) | 0xffffffffull;
+        // one entry per particle when two of its images can share this ball (_col_cache.cuh:38-41)
+        const bool dedup = X_WEDGE && !(image_dist_min < 0.0f || image_dist_min > 2.0f*qp.w);
 #pragma unroll
         for (int k = 0; k < N_K; ++k) heap[k] = empty;
         int st_node[KD_STACK];
@@ -268,14 +291,34 @@ __global__ __launch_bounds__(KD_QUERY_BLOCK) void k_kd_topk(const float4 *node, 
                 if (!found) break;
             }
             const float4 np = node[id];
-            const int j = node_idx[id];
+            const int r = node_idx[id];
+            const int image = X_WEDGE ? r / n_phys : 0;
+            const int j = X_WEDGE ? r - image*n_phys : r;
             const float d2 = d2_of(np.x, np.y, np.z, qp.x, qp.y, qp.z);
             if (active[j])
             {
-                uint64_t cand = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(3*j);
-                if (cand < heap[0])
+                uint64_t cand = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(3*j + image);
+                // a particle already in the heap is replaced only by a nearer image of itself
+                // (idx_old_heap::processCandidate, inc/swarm/kdtree/index_heap.cuh:112-175)
+                int slot = 0;
+                bool take = cand < heap[0];
+                if (X_WEDGE && dedup)
                 {
-                    int slot = 0;
+                    int found = -1;
+#pragma unroll
+                    for (int k = 0; k < N_K; ++k)
+                    {
+                        const uint32_t ck = uint32_t(heap[k]);
+                        if (found < 0 && ck != 0xffffffffu && int(ck / 3u) == j) found = k;
+                    }
+                    if (found >= 0)
+                    {
+                        slot = found;
+                        take = cand < heap[found];
+                    }
+                }
+                if (take)
+                {
                     while (true)
                     {
                         int child = 2*slot + 1;
@@ -329,6 +372,15 @@ __global__ __launch_bounds__(KD_QUERY_BLOCK) void k_kd_topk(const float4 *node, 
         for (int t = 0; t < N_K; t += 4)
             dst[t / 4] = make_int4(row[bitrev_slot(t)], row[bitrev_slot(t + 1)], row[bitrev_slot(t + 2)],
                                    row[bitrev_slot(t + 3)]);
+        if constexpr (X_WEDGE)
+        {
+#pragma unroll
+            for (int t = 0; t < N_K; ++t)
+            {
+                const uint32_t code = uint32_t(heap[bitrev_slot(t)]);
+                nbr_image[(long long)s*N_K + t] = static_cast<unsigned char>(code != 0xffffffffu ? code % 3u : 0u);
+            }
+        }
         measure[s] = ball_measure(ld(u.y, p), ld(u.z, p), sqrt(static_cast<double>(max_d2)));
     }
     // one atomic per distinct owner group of this wavefront (col_dep_graph.cu:14-60)
@@ -349,8 +401,21 @@ __global__ __launch_bounds__(KD_QUERY_BLOCK) void k_kd_topk(const float4 *node, 
 
 void kd_build(context &c)
 {
-    const int n = c.n;
+    const float4 *src = c.pts.p;
+    int n = c.n;
     hipStream_t st = c.stream;
+    if constexpr (X_WEDGE)
+    {
+        if (3ll*n > INT_MAX / 3) throw status_error(12, "wedge k-d node count exceeds the index limit");
+        const float width = static_cast<float>(X_MAX - X_MIN);
+        const float sw = static_cast<float>(std::sin(static_cast<double>(width)));
+        const float cw = static_cast<float>(std::cos(static_cast<double>(width)));
+        c.wedge_rec.alloc(size_t(3)*n);
+        k_kd_images<<<blocks(n, KD_BLOCK), KD_BLOCK, 0, st>>>(c.pts.p, n, cw, sw, c.wedge_rec.p);
+        hip_check(hipGetLastError(), "launch k-d images");
+        src = c.wedge_rec.p;
+        n *= 3;
+    }
     kd_tree &t = c.kd;
     t.n = n;
     t.node.alloc(n);
@@ -374,7 +439,7 @@ void kd_build(context &c)
     const unsigned int init[6] = {~0u, ~0u, ~0u, 0u, 0u, 0u};
     hip_check(hipMemcpyAsync(wbits.p, init, sizeof(init), hipMemcpyHostToDevice, st), "k-d world init");
     hip_check(hipMemsetAsync(tag.p, 0, sizeof(uint32_t)*n, st), "k-d tags");
-    k_kd_init<<<blocks(n, KD_BLOCK), KD_BLOCK, 0, st>>>(c.pts.p, n, t.node.p, t.idx.p, wbits.p);
+    k_kd_init<<<blocks(n, KD_BLOCK), KD_BLOCK, 0, st>>>(src, n, t.node.p, t.idx.p, wbits.p);
     k_kd_world<<<1, 64, 0, st>>>(wbits.p, t.world.p, dim0.p);
     k_kd_dim0<<<blocks(n, KD_BLOCK), KD_BLOCK, 0, st>>>(t.node.p, n, dim0.p);
     hip_check(hipGetLastError(), "launch k-d init");
@@ -406,9 +471,21 @@ void kd_build(context &c)
 void kd_query(context &c)
 {
     const int n = c.n;
+    float image_dist_min = -1.0f;
+    if constexpr (X_WEDGE)
+    {
+        // _get_image_dist_min (inc/swarm/param_grid.cuh:86-89), each sine rounded once from double
+        auto sinf1 = [](float a) { return static_cast<float>(std::sin(static_cast<double>(a))); };
+        const float x_min = static_cast<float>(X_MIN), x_max = static_cast<float>(X_MAX);
+        image_dist_min = 2.0f*static_cast<float>(Y_MIN)
+            *std::fmin(sinf1(static_cast<float>(Z_MIN)), sinf1(static_cast<float>(Z_MAX)))
+            *std::fabs(sinf1(0.5f*(x_max - x_min)));
+        c.nbr_image.alloc(size_t(std::max(1, c.n_active))*N_K);
+    }
     k_kd_topk<<<blocks(n, KD_QUERY_BLOCK), KD_QUERY_BLOCK, 0, c.stream>>>(
-        c.kd.node.p, c.kd.idx.p, n, c.kd.world.p, c.index.idx.p, c.pts.p, c.active.p, c.pidx_slot.p, c.group.p,
-        c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p);
+        c.kd.node.p, c.kd.idx.p, c.kd.n, n, X_WEDGE ? n : 0, image_dist_min, c.kd.world.p, c.index.idx.p, c.pts.p,
+        c.active.p, c.pidx_slot.p, c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p,
+        c.nbr_image.p);
     hip_check(hipGetLastError(), "launch k-d top-K");
 }
 

@@ -168,16 +168,18 @@ int gdx_col_active_count(gdx_col_handle h, int *n_active);
  * partner rotated by +width, 1 = by -width), n x N_K in (d^2, code) order, and the KNN measure (0 if
  * inactive); the col_cache_get output (inc/swarm/_col_cache.cuh:135-155) */
 int gdx_col_get_neighbors(gdx_col_handle h, int *codes, double *measure);
-/* indexed records of the last Morton epoch (n, plus periodic ghosts in a wedge), whether the images of
- * one particle are disjoint (no deduplication needed) and the largest search cap; the record count,
- * unique_ids and max_search_dist of morton_ghost_index::build (inc/swarm/morton/morton_ghost.cuh:96-168,
- * src/swarm/swarm_runtime.cu:413-415) */
+/* indexed records of the last epoch (Morton search: n, plus periodic ghosts in a wedge; k-d search: n,
+ * or 3n in a wedge), whether the images of one particle are disjoint under the largest search cap (no
+ * deduplication needed) and that cap; the record count, unique_ids and max_search_dist of
+ * morton_ghost_index::build (inc/swarm/morton/morton_ghost.cuh:96-168, src/swarm/swarm_runtime.cu:
+ * 413-415) and N_T (val/swarm/src/const_defs.cuh:240) */
 int gdx_col_record_info(gdx_col_handle h, int *records, int *unique_ids, float *q_max);
 /* collision sites: float {x, y, z, search cap} x n, active flags, spatial group */
 int gdx_col_get_sites(gdx_col_handle h, float *pts4, unsigned char *active, unsigned char *group);
-/* k-d tree of the last geometry epoch (search GDX_COL_SEARCH_KDTREE): particle index and split
- * dimension per node (n each) and the world box {lo xyz, hi xyz}; test access to
- * kdtree::buildTree (inc/swarm/kdtree/builder_thrust.h:418-526) */
+/* k-d tree of the last geometry epoch (search GDX_COL_SEARCH_KDTREE): record index and split dimension
+ * per node (one per record of gdx_col_record_info: in a wedge record r is particle r % n at image
+ * r / n) and the world box {lo xyz, hi xyz}; test access to kdtree::buildTree
+ * (inc/swarm/kdtree/builder_thrust.h:418-526) */
 int gdx_col_get_kdtree(gdx_col_handle h, int *node_idx, int *node_dim, float *world6);
 /* Morton keys and record order after the stable sort (one per indexed record: n, or the record count
  * of gdx_col_record_info in a wedge; record i >= n is a ghost) */

@@ -1501,9 +1501,15 @@ void orc_kd_build(int n, const float *pts, int *out_idx, int *out_dim, float *wo
 // top-K of the queries over the tree: a max-heap of (d^2 bits << 32 | code) keys initialised to
 // (q^2, 0xffffffff), culled at nextafter(root d^2), children visited near side first with the
 // far side's closest corner tracked on a stack. Output rows are in heap order; codes 3*j, -1 empty.
-void orc_kd_query(int n, const float *pts, const uint8_t *active, const int *node_idx, const int *node_dim,
-                  const float *world6, int nq, const int *query_idx, int k, const float *search_dist,
-                  int *out_codes, float *out_d2, int threads)
+// With n_phys > 0 (an azimuthal wedge) the tree holds n_phys physical points and their images 1 and 2
+// at [n_phys, 2 n_phys) and [2 n_phys, 3 n_phys): node record r is particle r % n_phys at image
+// r / n_phys, code 3*(r % n_phys) + image, active by its particle. A query whose two images can share
+// its ball (image_dist_min <= 2 q, col_cache_get in inc/swarm/_col_cache.cuh:38-41) keeps one entry
+// per particle: a candidate of a particle already in the heap replaces that entry only when smaller
+// (idx_old_heap::processCandidate, inc/swarm/kdtree/index_heap.cuh:112-175).
+static void kd_query(int n, const float *pts, const uint8_t *active, const int *node_idx, const int *node_dim,
+                     const float *world6, int nq, const int *query_idx, int k, const float *search_dist,
+                     int *out_codes, float *out_d2, int threads, int n_phys, float image_dist_min)
 {
     kd_box world;
     for (int d = 0; d < 3; d++) { world.lo[d] = world6[d]; world.hi[d] = world6[3 + d]; }
@@ -1525,13 +1531,30 @@ void orc_kd_query(int n, const float *pts, const uint8_t *active, const int *nod
             return f;
         };
         std::vector<uint64_t> heap(k, enc(qd*qd, 0xffffffffu));
+        const bool dedup = n_phys > 0 && !(image_dist_min < 0.0f || image_dist_min > 2.0f*qd);
         auto cull = [&]() { return std::nextafter(dec(heap[0]), std::numeric_limits<float>::infinity()); };
         auto process = [&](int node, float d2) {
-            int j = node_idx[node];
+            int r = node_idx[node];
+            int j = n_phys > 0 ? r % n_phys : r;
+            int image = n_phys > 0 ? r / n_phys : 0;
             if (!active[j]) return;
-            uint64_t cand = enc(d2, uint32_t(3*j));
-            if (cand >= heap[0]) return;
-            int slot = 0;
+            uint64_t cand = enc(d2, uint32_t(3*j + image));
+            int slot = -1;
+            if (dedup)
+                for (int s = 0; s < k; s++)
+                {
+                    uint32_t c = uint32_t(heap[s]);
+                    if (c != 0xffffffffu && int(c / 3) == j) { slot = s; break; }
+                }
+            if (slot >= 0)
+            {
+                if (cand >= heap[slot]) return;
+            }
+            else
+            {
+                if (cand >= heap[0]) return;
+                slot = 0;
+            }
             while (true)
             {
                 int child = 2*slot + 1;
@@ -1588,6 +1611,46 @@ void orc_kd_query(int n, const float *pts, const uint8_t *active, const int *nod
             uint32_t code = uint32_t(heap[s2]);
             out_codes[size_t(qi)*k + s2] = code == 0xffffffffu ? -1 : int(code);
             out_d2[size_t(qi)*k + s2] = dec(heap[s2]);
+        }
+    }
+}
+
+void orc_kd_query(int n, const float *pts, const uint8_t *active, const int *node_idx, const int *node_dim,
+                  const float *world6, int nq, const int *query_idx, int k, const float *search_dist,
+                  int *out_codes, float *out_d2, int threads)
+{
+    kd_query(n, pts, active, node_idx, node_dim, world6, nq, query_idx, k, search_dist, out_codes, out_d2, threads,
+             0, -1.0f);
+}
+
+// the k-d search of an azimuthal wedge: pts holds 3 n_phys points (orc_kd_images), query_idx and
+// search_dist index physical particles
+void orc_kd_query_wedge(int n_phys, const float *pts, const uint8_t *active, const int *node_idx,
+                        const int *node_dim, const float *world6, int nq, const int *query_idx, int k,
+                        const float *search_dist, int *out_codes, float *out_d2, int threads)
+{
+    kd_query(3*n_phys, pts, active, node_idx, node_dim, world6, nq, query_idx, k, search_dist, out_codes, out_d2,
+             threads, n_phys, orc_image_dist_min());
+}
+
+// col_site_init's wedge images for the k-d tree (src/swarm/col_site_init.cu:42-62): out holds 3n points,
+// the physical ones, then every point rotated by -width (image 1), then by +width (image 2); the width
+// is float(X_MAX - X_MIN), its sine and cosine each rounded once from double
+void orc_kd_images(int n, const float *pts, float *out)
+{
+    const float width = float(X_MAX - X_MIN);
+    const float sw = float(std::sin(double(width))), cw = float(std::cos(double(width)));
+    for (int i = 0; i < n; i++)
+    {
+        const float *p = pts + 3*size_t(i);
+        for (int d = 0; d < 3; d++) out[3*size_t(i) + d] = p[d];
+        for (int image = 0; image < 2; image++)
+        {
+            const float sa = image == 0 ? -sw : sw;
+            float *o = out + 3*(size_t(i) + size_t(image + 1)*n);
+            o[0] = cw*p[0] - sa*p[1];
+            o[1] = sa*p[0] + cw*p[1];
+            o[2] = p[2];
         }
     }
 }

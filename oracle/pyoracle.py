@@ -67,6 +67,8 @@ class Oracle:
         L.orc_space_bin.argtypes = [i, P, P, P, P]
         L.orc_kd_build.argtypes = [i, P, P, P, P]
         L.orc_kd_query.argtypes = [i, P, P, P, P, P, i, P, i, P, P, P, i]
+        L.orc_kd_query_wedge.argtypes = [i, P, P, P, P, P, i, P, i, P, P, P, i]
+        L.orc_kd_images.argtypes = [i, P, P]
         L.orc_change_limit.restype = i
         L.orc_change_limit.argtypes = [d, d, d, d, C.POINTER(d), C.POINTER(i)]
         L.orc_build_binmap.restype = i
@@ -239,6 +241,28 @@ class Oracle:
         d2 = np.empty((nq, k), np.float32)
         self.lib.orc_kd_query(len(pts), _p(pts), _p(act), _p(idx), _p(dim), _p(world), nq, _p(qi), k, _p(sd),
                               _p(codes), _p(d2), threads)
+        return codes, d2
+
+    def kd_images(self, pts):
+        """Physical points, then all rotated by -width (image 1), then by +width (image 2)."""
+        pts = np.ascontiguousarray(pts, dtype=np.float32)
+        out = np.empty((3*len(pts), 3), np.float32)
+        self.lib.orc_kd_images(len(pts), _p(pts), _p(out))
+        return out
+
+    def kd_query_wedge(self, pts3, active, tree, query_idx, k, search_dist, threads=0):
+        """Heap-ordered rows (codes 3*j + image) over a kd_build tree of kd_images points; the
+        queries, activity and search caps index physical particles."""
+        pts3 = np.ascontiguousarray(pts3, dtype=np.float32)
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+        idx, dim, world = (np.ascontiguousarray(a) for a in tree)
+        qi = np.ascontiguousarray(query_idx, dtype=np.int32)
+        sd = np.ascontiguousarray(search_dist, dtype=np.float32)
+        nq = len(qi)
+        codes = np.empty((nq, k), np.int32)
+        d2 = np.empty((nq, k), np.float32)
+        self.lib.orc_kd_query_wedge(len(pts3) // 3, _p(pts3), _p(act), _p(idx), _p(dim), _p(world), nq, _p(qi), k,
+                                    _p(sd), _p(codes), _p(d2), threads)
         return codes, d2
 
     def space_bin(self, x, y, z):

@@ -43,18 +43,6 @@ def test_argument_errors_without_gpu():
     assert lib.gdx_col_operator(None, C.c_double(1.0), None) == 15
 
 
-def test_wedge_model_rejects_kdtree_without_gpu():
-    """A wedge needs periodic ghosts, built for the Morton search only: creation fails before any GPU call."""
-    import ctypes as C
-    from gamedev_amd import _lib
-    lib = _lib.load("wedge_2d")
-    h = C.c_void_p()
-    cfg = _lib.Config(1024, 1.0, 0, 0, _lib.SEARCH_KDTREE)
-    assert lib.gdx_col_create(C.byref(cfg), C.byref(h)) == 15
-    assert b"wedge" in lib.gdx_col_last_error(None)
-    assert not h.value
-
-
 def test_kernel_timer_ids_match_header():
     """The host mirror sizes its kernel-time buffers by the header's GDX_COL_KERNELS."""
     from gamedev_amd import _lib
