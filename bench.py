@@ -183,6 +183,50 @@ def ensemble_moments(ds) -> "np.ndarray":
     return M
 
 
+def wedge_leg(n: int, steps: int, dev) -> dict:
+    """Azimuthal wedge model (x in [-0.1, 0.1], physical kernel, K=32): geometry epoch with periodic seam
+    ghosts for the Morton search and with 3n image nodes for the k-d search, then plain operator steps
+    at a horizon of a quarter of the controller's refresh bound."""
+    import torch
+    from gamedev_amd import CollisionOperator, synth
+    st = synth.wedge_2d_state(n, seed=0)
+    out = {"representatives": n, "steps": steps}
+    for search in ("kdtree", "morton"):
+        d = synth.to_device(st, dev)
+        op = CollisionOperator("wedge_2d", n, st["total_mass"], seed=0, defer_writeback=True, search=search)
+        op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
+        op.geometry()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.geometry()
+        torch.cuda.synchronize()
+        out[f"geometry_epoch_ms_{search}"] = 1e3*(time.perf_counter() - t0)
+        info = op.record_info()
+        out[f"records_{search}"] = info["records"]
+        out["unique_ids"] = info["unique_ids"]
+    op.rate_pass()
+    pts, act, _ = op.sites()
+    c = op.cached()
+    J2 = float(c[act == 1, 2].mean())
+    h = 0.25*(op.info["col_bath_eps"]**2)/J2
+    op.evolve(h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev = 0
+    for _ in range(steps):
+        ev += op.evolve(h)["events"]
+    op.store_state()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    active = op.active_count()
+    out.update({"horizon": h, "active_owners": active, "updates_per_s": steps*active/el,
+                "ms_per_step": 1e3*el/steps, "events_per_step": ev/steps})
+    log(f"wedge leg: {n} representatives, {out['records_morton']} Morton records "
+        f"({out['geometry_epoch_ms_morton']:.1f} ms epoch), {out['records_kdtree']} k-d nodes "
+        f"({out['geometry_epoch_ms_kdtree']:.1f} ms), {out['updates_per_s']:.3e} updates/s")
+    return out
+
+
 def clumped_leg(count: int, total: int, steps: int, horizon: float, dev, samples: int, epochs: int) -> dict:
     """The clumped stress case, batched: `count` realizations totalling `total` representatives (sizes
     floor(2^U[16,20]) rescaled), vortex and ring overdensities, sigma_ln = 1.5. The majorant is trained on
@@ -380,6 +424,9 @@ def main() -> int:
                     help="realizations of the clumped batched leg (0 skips it), totalling --clumped-total")
     ap.add_argument("--clumped-total", type=int, default=10_000_000)
     ap.add_argument("--clumped-steps", type=int, default=5)
+    ap.add_argument("--wedge", type=int, default=1_000_000,
+                    help="representatives of the azimuthal-wedge leg (0: skip)")
+    ap.add_argument("--wedge-steps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size dispatch that calibrates "
                          "hardware byte counters)")
@@ -471,6 +518,11 @@ def main() -> int:
         clumped = clumped_leg(args.clumped, args.clumped_total, args.clumped_steps, args.horizon, dev,
                               args.train_samples, args.epochs)
 
+    wedge = None
+    if args.wedge > 0:
+        torch.cuda.empty_cache()
+        wedge = wedge_leg(args.wedge, args.wedge_steps, dev)
+
     roofline_plain = roofline_of(kt, p_tot)
     plain = {"value": plain_value, "ms_per_step": 1e3*elapsed/args.steps, "roofline": roofline_plain,
              "events_per_s": tot["events"]/elapsed, "pair_evals_per_s": tot["pair_evals"]/elapsed,
@@ -513,6 +565,7 @@ def main() -> int:
             "active_owners": n_active,
             "batched_ensemble": batched,
             "clumped_batched": clumped,
+            "wedge": wedge,
         },
     }
     print(json.dumps(out))

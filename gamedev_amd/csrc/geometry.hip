@@ -460,10 +460,46 @@ __device__ __forceinline__ void topk_insert(float (&d)[K], int (&c)[K], float nd
     if (carry) { d[0] = nd; c[0] = nc; }
 }
 
+// insert (nd, nc) keeping one entry per physical particle (code / 3): an entry of the same particle is
+// replaced when the candidate precedes it, else the candidate is dropped. The list stays the K first of
+// the candidates seen so far with each particle collapsed to its nearest image, which is what the
+// reference's 3K candidates with duplicate removal (morton_query.cuh:62-140) select: every record up
+// to the K-th particle's distance belongs to one of those K particles, so at most 3K records precede
+// it. Precondition as topk_insert.
+template <int K>
+__device__ __forceinline__ void topk_insert_dedup(float (&d)[K], int (&c)[K], float nd, int nc)
+{
+    const unsigned pn = static_cast<unsigned>(nc) / 3u;
+    int e = -1;
+    float de = 0.0f;
+    int ce = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        const bool same = e < 0 && c[k] != INT_MAX && static_cast<unsigned>(c[k]) / 3u == pn;
+        e = same ? k : e;
+        de = same ? d[k] : de;
+        ce = same ? c[k] : ce;
+    }
+    if (e >= 0)
+    {
+        if (!nb_less(nd, nc, de, ce)) return;
+#pragma unroll
+        for (int k = 0; k < K - 1; ++k)
+        {
+            d[k] = k >= e ? d[k + 1] : d[k];
+            c[k] = k >= e ? c[k + 1] : c[k];
+        }
+        d[K - 1] = __builtin_inff();
+        c[K - 1] = INT_MAX;
+    }
+    topk_insert<K>(d, c, nd, nc);
+}
+
 __device__ __forceinline__ float readlane_f(float v, int l)
 { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 
-template <int K>
+template <int K, bool DEDUP = false>
 __device__ void topk_wave(const float4 *rec, const gdx_node *nodes, int dim, int max_level, float qx, float qy,
                           float qz, float q2, bool qactive, float cx, float cy, float cz, float (&d)[K], int (&c)[K],
                           int *stack, unsigned int *overflow)
@@ -498,7 +534,11 @@ __device__ void topk_wave(const float4 *rec, const gdx_node *nodes, int dim, int
                     float dx = qx - px, dy = qy - py, dz = qz - pz;
                     float d2 = dx*dx + dy*dy + dz*dz;
                     bool e = need && code >= 0 && d2 <= q2 && nb_less(d2, code, d[K - 1], c[K - 1]);
-                    if (e) topk_insert<K>(d, c, d2, code);
+                    if (e)
+                    {
+                        if constexpr (DEDUP) topk_insert_dedup<K>(d, c, d2, code);
+                        else topk_insert<K>(d, c, d2, code);
+                    }
                 }
             }
             continue;
@@ -556,29 +596,8 @@ __device__ __forceinline__ void wave_centre(float qx, float qy, float qz, bool q
 }
 
 // col_cache_get (inc/swarm/_col_cache.cuh:73-159) over Morton-ordered queries
-// drop every later image of a physical particle already listed, keep the first K (the id sort,
-// duplicate mask and re-sort of _morton_ghost_topk, morton_query.cuh:62-140: in (d2, code) order the
-// first image of a particle is its nearest one)
-template <int KC>
-__device__ __forceinline__ void dedup_images(const float (&dc)[KC], const int (&cc)[KC], float (&d)[N_K], int (&c)[N_K])
-{
-#pragma unroll
-    for (int k = 0; k < N_K; ++k) { d[k] = __builtin_inff(); c[k] = INT_MAX; }
-    int ph[KC];
-#pragma unroll
-    for (int i = 0; i < KC; ++i) ph[i] = cc[i] / 3;
-#pragma unroll
-    for (int i = 0; i < KC; ++i)
-    {
-        bool dup = cc[i] == INT_MAX;
-#pragma unroll
-        for (int j = 0; j < i; ++j) dup = dup || (ph[j] == ph[i]);
-        if (!dup && nb_less(dc[i], cc[i], d[N_K - 1], c[N_K - 1])) topk_insert<N_K>(d, c, dc[i], cc[i]);
-    }
-}
-
-// one query per physical record; KC = N_K (disjoint images) or 3 N_K (a narrow wedge, then deduplicated)
-template <int KC>
+// one query per physical record; DEDUP when two images of one particle can share a search ball
+template <bool DEDUP>
 __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, const gdx_node *nodes, int n, int dim,
                                                         int max_level, const int *sidx, const float4 *pts,
                                                         const int *pidx_slot, const unsigned char *group,
@@ -600,19 +619,8 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
     wave_centre(me.x, me.y, me.z, qactive, cx, cy, cz);
     float d[N_K];
     int c[N_K];
-    if constexpr (KC == N_K)
-    {
-        topk_wave<N_K>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, d, c, stacks[wave],
-                       overflow);
-    }
-    else
-    {
-        float dc[KC];
-        int cc[KC];
-        topk_wave<KC>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, dc, cc, stacks[wave],
-                      overflow);
-        dedup_images<KC>(dc, cc, d, c);
-    }
+    topk_wave<N_K, DEDUP>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, d, c, stacks[wave],
+                          overflow);
     unsigned long long mask = 0ull;
     int g = qactive ? int(group[p]) : -1;
     if (qactive)
@@ -1050,11 +1058,11 @@ void geometry_epoch(context &c)
         dim3 grid = xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE));
         if (X_WEDGE) c.nbr_image.alloc(na*N_K);
         if (c.unique_ids)
-            k_col_topk<N_K><<<grid, TOPK_BLOCK, 0, st>>>(
+            k_col_topk<false><<<grid, TOPK_BLOCK, 0, st>>>(
                 c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
                 c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p);
         else
-            k_col_topk<X_WEDGE ? 3*N_K : N_K><<<grid, TOPK_BLOCK, 0, st>>>(
+            k_col_topk<X_WEDGE><<<grid, TOPK_BLOCK, 0, st>>>(
                 c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
                 c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p);
         hip_check(hipGetLastError(), "launch top-K");
