@@ -19,6 +19,13 @@
 //                           coordinate <= the split and every right one >= it) and split
 //                           dimensions equal to the widest extent of each subtree's domain
 //   orc_kd_query            neighbour sets equal to orc_topk_brute, rows a valid max-heap
+//   orc_image_dist_min      2 y_min sin(z) |sin(width/2)| evaluated in the test
+//   orc_wedge_records       hand-computed seam distances, ghost codes and rotation angles (atan2/hypot)
+//   orc_topk_records        chord lengths across the seam, the nearest-image rule on a narrow wedge, and
+//                           a numpy brute force over all three images of every particle (no seam test,
+//                           no candidate cap) at disjoint and shared-ball search caps
+//   orc_kd_images           rotation angles +-width in the test
+//   orc_kd_query_wedge      sorted rows equal to the same all-images brute force, rows a valid max-heap
 //   orc_ball_measure        closed-form interior balls, Monte-Carlo cap volumes
 //   orc_vrel_t / pair rate  Ormel & Cuzzi (2007) equations 17/18/26-29 evaluated independently in tests
 //   orc_outcome_moments     numerical expectation of orc_sample_outcome over u, closed-form sticking moments
@@ -74,6 +81,17 @@ void orc_topk_brute(int n, const float *pts, const uint8_t *active, int nq, cons
                     const float *query_pts, const float *query_q, int k, int *out_codes, float *out_d2,
                     int threads);
 // same contract through the Morton hierarchy (the CPU-baseline search)
+// azimuthal wedge: seam ghosts (rec, code hold 3n; returns the record count), brute-force top-K over
+// coded records (dedup: 3K candidates, each particle at its nearest image), k-d image nodes (3n) and the
+// k-d query over them with the per-query physical-id heap
+float orc_image_dist_min(void);
+int orc_wedge_records(int n, const float *pts, const float *posx, float q_max, float *rec, int *code);
+void orc_topk_records(int nr, const float *rec, const int *code, const uint8_t *active, int nq, const float *query_pts,
+                      const float *query_q, int k, int dedup, int *out_codes, float *out_d2, int threads);
+void orc_kd_images(int n, const float *pts, float *out);
+void orc_kd_query_wedge(int n_phys, const float *pts, const uint8_t *active, const int *node_idx,
+                        const int *node_dim, const float *world6, int nq, const int *query_idx, int k,
+                        const float *search_dist, int *out_codes, float *out_d2, int threads);
 void orc_kd_build(int n, const float *pts, int *out_idx, int *out_dim, float *world6);
 void orc_kd_query(int n, const float *pts, const uint8_t *active, const int *node_idx, const int *node_dim,
                   const float *world6, int nq, const int *query_idx, int k, const float *search_dist,
