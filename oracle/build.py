@@ -40,6 +40,19 @@ def _one(model: str) -> Path:
     return out
 
 
+def build_planted(model: str, plant: int, out_dir: Path) -> Path:
+    """The oracle of `model` with planted mistake number `plant` (see ORC_PLANT in oracle.cpp), for the
+    tests that show the pins reject it; -O1 and no OpenMP keep the build to a few seconds."""
+    out_dir.mkdir(parents=True, exist_ok=True)
+    out = out_dir / f"liboracle_{model}_plant{plant}.so"
+    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", f"-DORC_PLANT={plant}",
+           "-Wno-unknown-pragmas", f"-I{MOD / model}", str(HERE / "oracle.cpp"), "-o", str(out)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"planted oracle build failed for {model}/{plant}:\n{proc.stdout}{proc.stderr}")
+    return out
+
+
 def build(targets: list[str] | None = None) -> list[Path]:
     targets = targets or models()
     with cf.ThreadPoolExecutor(min(8, len(targets))) as ex:

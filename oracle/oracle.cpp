@@ -28,6 +28,13 @@ using namespace gdx;
 
 namespace {
 
+// Planted mistakes for the pin tests (tests/test_oracle_pins.py builds the oracle with -DORC_PLANT=n
+// and checks that the pins reject every variant); the ordinary build defines no ORC_PLANT.
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+constexpr int PLANT = ORC_PLANT;
+
 constexpr double PI = 3.14159265358979323846;
 constexpr int DIM = (N_Z > 1) ? 3 : 2;
 constexpr bool X_WEDGE = N_X > 1
@@ -89,7 +96,7 @@ inline double grain_mass(double size) { return PI*RHO_0*size*size*size / 6.0; } 
 inline double omegaK(double R) { return std::sqrt(G*M_S / R / R / R); }                  // param_phys.cuh:66-68
 inline double hg(double R) { return ASPR_0*std::pow(R / R_0, 0.5*(IDX_Q + 1.0)); }       // param_phys.cuh:70-72
 inline double eta(double R, double Z, double h)                                          // param_phys.cuh:75-77
-{ return -0.5*((IDX_P + 0.5*IDX_Q - 1.5)*h*h + IDX_Q*(1.0 - R / std::sqrt(R*R + Z*Z))); }
+{ return -0.5*((IDX_P + 0.5*IDX_Q - 1.5)*h*h + (PLANT == 7 ? -IDX_Q : IDX_Q)*(1.0 - R / std::sqrt(R*R + Z*Z))); }
 inline double gas_strat(double R, double Z, double h)                                    // param_phys.cuh:80-82
 { return std::exp((R / std::sqrt(R*R + Z*Z) - 1.0) / (h*h)); }
 inline double sigma_g(double R) { return SIGMA_0*std::pow(R / R_0, IDX_P); }             // param_phys.cuh:86-87
@@ -107,7 +114,8 @@ inline double stokes(double R, double Z, double h, double size)                 
 {
     double st = STOKES_0*(size / S_0);
 #ifndef CONST_ST
-    st /= std::pow(R / R_0, IDX_P);
+    if (PLANT == 4) st *= std::pow(R / R_0, IDX_P);
+    else st /= std::pow(R / R_0, IDX_P);
     st /= gas_strat(R, Z, h);
 #else
     (void)R; (void)Z; (void)h;
@@ -208,9 +216,10 @@ double vrel_pair(double y, double z, double size_i, double size_j)
     double vn = -eta(R, Z, h)*R*omega;
     double fi = 1.0 / (1.0 + si*si);
     double fj = 1.0 / (1.0 + sj*sj);
-    double dvr = 2.0*vn*(si*fi - sj*fj);
+    double dvr = (PLANT == 1 ? 1.0 : 2.0)*vn*(si*fi - sj*fj);
     double dvphi = vn*(fi - fj);
-    double dvz = Z*omega*(std::fmin(si, 0.5) - std::fmin(sj, 0.5));
+    double cap = PLANT == 2 ? 1.0 : 0.5;
+    double dvz = Z*omega*(std::fmin(si, cap) - std::fmin(sj, cap));
     double sigma_local = sigma_g(R)*gas_strat(R, Z, h);
     double vt = vrel_t(R, si, sj, h, sigma_local);
     double vb = 0.0;
@@ -369,7 +378,7 @@ double chain_rate(double y_i, double z_i, double y_j, double z_j, double size_i,
         double R_i = cyl_R(y_i, z_i);
         double R_j = cyl_R(y_j, z_j);
         double H_gi = R_i*hg(R_i);
-        double H_gj = R_j*hg(R_j);
+        double H_gj = PLANT == 3 ? R_i*hg(R_i) : R_j*hg(R_j);
         rate /= std::sqrt(2.0*PI*(H_gi*H_gi + H_gj*H_gj));
     }
     return rate;
@@ -777,6 +786,23 @@ owner_totals reduce_pairs(pair_work &w, std::vector<double> &tmp)
     return t;
 }
 
+// partner of an event (col_chain_run.cu:299-320): the first slot in slot order whose running sum of
+// positive pair rates exceeds target = U * total; on roundoff (no slot passes) the last positive slot
+int partner_walk(const double *rate, int k_count, double target, int last_positive)
+{
+    double cumulative = 0.0;
+    for (int k = 0; k < k_count; k++)
+    {
+        if (rate[k] <= 0.0) continue;
+        cumulative += rate[k];
+        if (PLANT == 5 ? cumulative >= target : cumulative > target) return k;
+    }
+    if (PLANT == 6)
+        for (int k = 0; k < k_count; k++)
+            if (rate[k] > 0.0) return k;
+    return last_positive;
+}
+
 struct owner_bath
 {
     double time = 0, hazard = 0, j1 = 0, j2 = 0, jmax = 0;
@@ -848,15 +874,7 @@ void run_owner(int i, double end, const double *y, const double *z, double *size
         b.j2 += t.j2*wait;
         b.jmax = std::fmax(b.jmax, t.jmax);
         double target = col_uniform(pidx, seed, ctr)*t.rate;
-        double cumulative = 0.0;
-        int slot = -1;
-        for (int k = 0; k < N_K; k++)
-        {
-            if (w.rate[k] <= 0.0) continue;
-            cumulative += w.rate[k];
-            if (cumulative > target) { slot = k; break; }
-        }
-        if (slot < 0) slot = t.last;
+        int slot = partner_walk(w.rate.data(), N_K, target, t.last);
         if (slot < 0) { b.error = 5; break; }
         int j = nbr[size_t(i)*N_K + slot] / 3;
         double size_j = size_old[j];
@@ -1134,6 +1152,22 @@ double orc_sample_outcome(double si, double sj, int high, double u, int *categor
 double orc_chain_rate(double y_i, double z_i, double y_j, double z_j, double size_i, double size_j,
                       double numr_j, double lambda_0, double *vrel)
 { return chain_rate(y_i, z_i, y_j, z_j, size_i, size_j, numr_j, lambda_0, *vrel); }
+double orc_stokes(double y, double z, double size)
+{
+    double R = cyl_R(y, z), Z = cyl_Z(y, z);
+    return stokes(R, Z, hg(R), size);
+}
+// the chain's partner pick over K rates; last_positive < 0 computes it (-1 if no rate is positive)
+int orc_partner_walk(const double *rate, int k_count, double target, int last_positive)
+{
+    if (last_positive == -3)
+    {
+        last_positive = -1;
+        for (int k = 0; k < k_count; k++)
+            if (rate[k] > 0.0) last_positive = k;
+    }
+    return partner_walk(rate, k_count, target, last_positive);
+}
 
 // col_site_init, Morton variant (src/swarm/col_site_init.cu:79-118)
 int orc_site_init(int n, const double *x, const double *y, const double *z, const double *size,

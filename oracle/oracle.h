@@ -27,7 +27,16 @@
 //   orc_kd_images           rotation angles +-width in the test
 //   orc_kd_query_wedge      sorted rows equal to the same all-images brute force, rows a valid max-heap
 //   orc_ball_measure        closed-form interior balls, Monte-Carlo cap volumes
-//   orc_vrel_t / pair rate  Ormel & Cuzzi (2007) equations 17/18/26-29 evaluated independently in tests
+//   orc_vrel_t              Ormel & Cuzzi (2007) equations 17/18/26-29 evaluated independently in tests
+//   orc_vrel_pair, orc_stokes, orc_chain_rate
+//                           drift, settling (St capped at 0.5), Stokes number, rotation support and the
+//                           vertically integrated overlap sqrt(2 pi (H_gi^2 + H_gj^2)) restated in numpy
+//                           from the closed forms of doc/guide_swarm.md 8.4.1/8.4.3 and doc/guide_basis.md
+//                           3-4, at the colphys probe points with unequal sizes and St on both sides of 0.5
+//   orc_partner_walk        known answers on crafted rates and targets (zero slots, exact cumulative ties,
+//                           roundoff fallback to the last positive slot) and numpy searchsorted
+//   planted mistakes        -DORC_PLANT=1..7 plants a factor, cap, sign, index or comparison error in the
+//                           functions above; the pin tests build each variant and require a failure
 //   orc_outcome_moments     numerical expectation of orc_sample_outcome over u, closed-form sticking moments
 //   controller/scheduler    crafted inputs with hand-derived outputs; tick tiling sums to the horizon
 //   orc_operator            mass conservation, repeat determinism, Smoluchowski-type rate checks
@@ -62,6 +71,10 @@ double orc_sample_outcome(double si, double sj, int high_speed, double u, int *c
 // one pair's chain rate numerator (_get_col_chain_rate) and relative speed
 double orc_chain_rate(double y_i, double z_i, double y_j, double z_j, double size_i, double size_j,
                       double numr_j, double lambda_0, double *vrel);
+// Stokes number of a grain of `size` at (y, z) (_get_stokes, param_phys.cuh:163-210)
+double orc_stokes(double y, double z, double size);
+// the chain's partner pick (col_chain_run.cu:299-320); last_positive = -3 derives it from the rates
+int orc_partner_walk(const double *rate, int k_count, double target, int last_positive);
 
 // ---- geometry epoch -------------------------------------------------------------------------
 // returns the first bad particle index + 1 (0 when every state is finite)

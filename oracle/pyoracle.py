@@ -31,8 +31,8 @@ def _p(a: np.ndarray):
 
 
 class Oracle:
-    def __init__(self, model: str):
-        path = HERE / "lib" / f"liboracle_{model}.so"
+    def __init__(self, model: str, path: Path | None = None):
+        path = path or HERE / "lib" / f"liboracle_{model}.so"
         if not path.exists():
             raise RuntimeError(f"oracle for {model} is not built: {path} (python oracle/build.py)")
         self.lib = L = C.CDLL(str(path))
@@ -51,6 +51,10 @@ class Oracle:
         L.orc_sample_outcome.argtypes = [d, d, i, d, C.POINTER(i), C.POINTER(d)]
         L.orc_chain_rate.restype = d
         L.orc_chain_rate.argtypes = [d]*8 + [C.POINTER(d)]
+        L.orc_stokes.restype = d
+        L.orc_stokes.argtypes = [d]*3
+        L.orc_partner_walk.restype = i
+        L.orc_partner_walk.argtypes = [P, i, d, i]
         L.orc_site_init.restype = i
         L.orc_site_init.argtypes = [i] + [P]*9
         L.orc_morton_keys.argtypes = [i, P, C.c_float, C.c_float, C.c_float, C.c_float, i, i, P]
@@ -127,6 +131,13 @@ class Oracle:
         v = C.c_double()
         r = self.lib.orc_chain_rate(yi, zi, yj, zj, si, sj, nj, lam, C.byref(v))
         return r, v.value
+
+    def stokes(self, y, z, size):
+        return self.lib.orc_stokes(y, z, size)
+
+    def partner_walk(self, rate, target, last_positive=-3):
+        r = np.ascontiguousarray(rate, dtype=np.float64)
+        return self.lib.orc_partner_walk(_p(r), len(r), float(target), int(last_positive))
 
     # ---- geometry ----
     def site_init(self, x, y, z, size, numr):

@@ -4,6 +4,8 @@ import math
 import numpy as np
 import pytest
 
+from physics_ref import oc07_speed_sq
+
 
 # ---- KNN ball measure: closed forms and Monte-Carlo cap fractions ----
 def test_ball_measure_interior_closed_forms(oracles):
@@ -31,29 +33,6 @@ def test_ball_measure_single_face_monte_carlo(oracles, model, dim):
 
 
 # ---- Ormel & Cuzzi (2007) regimes, evaluated independently in stopping-time form ----
-def _oc07(t1, t2, tL, teta, vg2):
-    """Eqs. 26-29 of Ormel & Cuzzi (2007), written with stopping times t1 >= t2, t_L and t_eta = Re^-1/2 t_L."""
-    ya = 1.6
-    St1, St2 = t1/tL, t2/tL
-    if t1 < 0.2*teta:                                      # eq. 27: both tightly coupled
-        return vg2*(t1 - t2)**2/(tL*teta)
-    if t1 < teta/ya:                                       # eq. 26 with the Kolmogorov time
-        return vg2*(St1 - St2)/(St1 + St2)*(St1**2/(St1 + teta/tL) - St2**2/(St2 + teta/tL))
-    if t1 < 5.0*teta:                                      # eqs. 17 + 18 at y* = ya
-        r = teta/tL
-        c1 = (St1 - St2)/(St1 + St2)*(St1/(1 + ya) - St2**2/(St2 + ya*St1))
-        c2 = 2*(ya*St1 - r) + St1/(1 + ya) - St1**2/(St1 + r) + St2**2/(ya*St1 + St2) - St2**2/(St2 + r)
-        return vg2*(c1 + c2)
-    eps = t2/t1
-    if St1 < 0.2:                                          # eq. 28, intermediate regime
-        y = ya
-    elif St1 < 1.0:                                        # eq. 28 with the DustPy y* fit
-        y = 1.6015125 - 0.63119577*St1 + 0.32938936*St1**2 - 0.29847604*St1**3
-    else:                                                  # eq. 29, heavy particles
-        return vg2*(1.0/(1 + St1) + 1.0/(1 + St2))
-    return vg2*St1*(2*y - (1 + eps) + 2.0/(1 + eps)*(1.0/(1 + y) + eps**3/(y + eps)))
-
-
 def test_vrel_t_regimes_and_boundaries(oracles):
     """The colphys probe points (val/swarm/src/colphys_main.cuh:40-63): six regimes and ten straddles."""
     o = oracles["fiducial_3d"]
@@ -69,7 +48,7 @@ def test_vrel_t_regimes_and_boundaries(oracles):
              [b*f for b in bounds for f in (1 - 1e-6, 1 + 1e-6)]
     for sl in probes:
         got = o.vrel_t(R, sl, 0.3*sl, h, sig)
-        ref = math.sqrt(_oc07(sl, 0.3*sl, 1.0, ris, vg2))
+        ref = math.sqrt(oc07_speed_sq(sl, 0.3*sl, 1.0, ris, vg2))
         assert got == pytest.approx(ref, rel=1e-11), sl
 
 
