@@ -37,31 +37,67 @@ sys.path.insert(0, str(ROOT))
 METRIC = "collision updates/s at 1e7 reps, K=32: learned+batched vs plain, same GPU"
 HBM_PEAK_GBS = 8000.0
 K = 32
-# algorithmic bytes (K = 32), per owner of the launch (all owners of the due groups) unless noted
-#   publish:     owner (s, N) read 16 + reservoir publish 16
-#   bath_rate:   codes 4K + partner (s, N) 16K + measure 8 + owner (y, z) 16 + group/active 5
-#   screen:      cached rate 8 + measure 8 + RNG counter read + write 16 + particle index 4 + published (s, N) 16
-#   chain:       per chained owner 140 (state, clock, RNG, sums), plus codes 4 + partner (s, N) 16 per pair term
-#   chain_audit: per chained owner: queue entry 4 + chain sums 36 + published and final state 24
-#   majorant:    codes 4K + partner (s, N) 16K + owner (s, N) 16 + measure 8 + environment 64 + bound write 8
-KERNEL_BYTES = {"publish": 32, "bath_rate": 20*K + 29, "screen": 52, "majorant": 12*K + 72}
+# algorithmic bytes (K = 32) of each timed kernel, and what limits it (measured: SQ counters and traces)
+#   publish:      per due owner: state (s, N) read 16 + reservoir publish 16
+#   mark_stale:   per due owner: its stale flag 1 (a lower bound: the reverse rows read are not counted)
+#   bath_rate:    per exactly rated owner: codes 4K + partner (s, N) 16K + measure 8 + owner (y, z) 16 + group/active 5
+#   screen:       per due owner: cached rate 8 + measure 8 + draw counter read + write 16 + particle index 4
+#                 + published (s, N) 16
+#   candidates:   per thinning candidate: candidate record 16 + owner state 16 + measure 8; per pair term of its
+#                 K-sums: code 4 + partner (s, N) 16
+#   queue_prefix: one block per phase over the group counters (not counted)
+#   chain:        per chained owner 140 (state, clock, draw counter, sums), per pair term: code 4 + partner (s, N) 16
+#   chain_audit:  per chained owner: queue entry 4 + chain sums 36 + published and final state 24
+#   majorant:     per due owner: stale flag 1; per evaluated owner: codes 4K + FP32 partner records 8K + owner
+#                 state 16 + measure 8 + environment 40 + bound 8
+PER_OWNER = {"publish": 32, "mark_stale": 1, "screen": 52}
+RATED_BYTES = 20*K + 29
+CAND_BYTES, CAND_PAIR_BYTES = 40, 20
 CHAIN_OWNER_BYTES, CHAIN_PAIR_BYTES, AUDIT_OWNER_BYTES = 140, 20, 64
+MAJORANT_DUE_BYTES, MAJORANT_EVAL_BYTES = 1, 12*K + 72
+LIMIT = {"publish": "hbm", "mark_stale": "latency", "bath_rate": "fp64-valu", "screen": "latency",
+         "candidates": "fp64-valu", "queue_prefix": "latency", "chain": "fp64-valu + serial event latency",
+         "chain_audit": "latency", "majorant": "valu (features) + gather latency"}
 
 
 def kernel_bytes(kernel: str, rec: dict, tot: dict) -> float:
-    """Algorithmic bytes moved by all timed launches of one kernel (bath_rate: the owners it rated)."""
+    """Algorithmic bytes moved by all timed launches of one kernel, from the same calls' counters."""
+    if kernel in PER_OWNER:
+        return rec["owners"]*PER_OWNER[kernel]
     if kernel == "bath_rate":
-        return tot["rated_owners"]*KERNEL_BYTES[kernel]
-    if kernel in KERNEL_BYTES:
-        return rec["owners"]*KERNEL_BYTES[kernel]
-    chain_pairs = tot["pair_evals"] - K*tot["rated_owners"] - tot.get("candidate_pairs", 0)
+        return tot["rated_owners"]*RATED_BYTES
+    if kernel == "candidates":
+        return tot["candidates"]*CAND_BYTES + tot.get("candidate_pairs", 0)*CAND_PAIR_BYTES
+    if kernel == "majorant":
+        return rec["owners"]*MAJORANT_DUE_BYTES + tot.get("bound_evals", 0)*MAJORANT_EVAL_BYTES
     if kernel == "chain":
+        chain_pairs = tot["pair_evals"] - K*tot["rated_owners"] - tot.get("candidate_pairs", 0)
         return tot["chain_owners"]*CHAIN_OWNER_BYTES + chain_pairs*CHAIN_PAIR_BYTES
-    return tot["chain_owners"]*AUDIT_OWNER_BYTES
+    if kernel == "chain_audit":
+        return tot["chain_owners"]*AUDIT_OWNER_BYTES
+    return 0.0
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores() -> dict:
+    """The host share this process runs on: nproc, the affinity set, OMP_NUM_THREADS and the CPU model."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
+    except OSError:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = int(omp) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": omp,
+            "cpu_model": model}
 
 
 def cpu_baseline(n_sample: int, steps: int, horizon: float, threads: int) -> dict:
@@ -351,7 +387,7 @@ def timed_steps(op, steps: int, horizon: float, label: str) -> tuple[float, dict
     """Run `steps` operator calls; returns wall seconds, summed stats and active owners per step."""
     import torch
     keys = ("owner_updates", "events", "waves", "baths", "pair_evals", "chain_owners", "thinned", "candidates",
-            "accepted", "exact_checks", "violations", "rated_owners", "candidate_pairs")
+            "accepted", "exact_checks", "violations", "rated_owners", "candidate_pairs", "bound_evals")
     tot = {k: 0 for k in keys}
     tot["longest_chain"] = 0
     updates = 0
@@ -381,6 +417,20 @@ def profiled_window(op, steps: int, horizon: float, label: str) -> tuple[dict, d
     return kt, tot, steps
 
 
+def kernel_table(kt: dict, tot: dict, steps: int) -> dict:
+    """Per kernel: ms per step and per launch, algorithmic GB/s and its fraction of the HBM peak, the limiter."""
+    out = {}
+    for name, rec in kt.items():
+        if rec["launches"] == 0:
+            continue
+        b = kernel_bytes(name, rec, tot)
+        avg = rec["ms"]/rec["launches"]
+        gbs = b/rec["launches"]/(avg*1e-3)/1e9 if avg > 0 else 0.0
+        out[name] = {"ms_per_step": rec["ms"]/steps, "avg_launch_ms": avg, "launches": rec["launches"],
+                     "algorithmic_GBps": gbs, "hbm_frac": gbs/HBM_PEAK_GBS, "limit": LIMIT.get(name, "")}
+    return out
+
+
 def roofline_of(kt: dict, tot: dict, dom: str | None = None) -> dict:
     dom = dom or max(kt, key=lambda k: kt[k]["ms"])
     rec = kt[dom]
@@ -395,7 +445,7 @@ def roofline_of(kt: dict, tot: dict, dom: str | None = None) -> dict:
         if per_owner:
             traffic = per_owner*rec["owners"]/max(1, rec["launches"])
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+            "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom, "limit": LIMIT.get(dom, ""),
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}
 
 
@@ -503,6 +553,7 @@ def main() -> int:
                    "events_per_s": l_tot["events"]/l_elapsed, "waves_per_step": l_tot["waves"]/args.steps,
                    "longest_chain_events": l_tot["longest_chain"],
                    "kernel_ms_per_step": {k: v["ms"]/lp_steps for k, v in l_kt.items()},
+                   "kernels": kernel_table(l_kt, lp_tot, lp_steps),
                    "roofline": roofline_of(l_kt, lp_tot), "totals": l_tot}
         op.use_majorant(None)
 
@@ -529,7 +580,8 @@ def main() -> int:
              "mean_baths_per_owner_step": tot["owner_updates"]/max(1, updates),
              "waves_per_step": tot["waves"]/args.steps, "chain_owners_per_step": tot["chain_owners"]/args.steps,
              "longest_chain_events": tot["longest_chain"],
-             "kernel_ms_per_step": {k: v["ms"]/p_steps for k, v in kt.items()}}
+             "kernel_ms_per_step": {k: v["ms"]/p_steps for k, v in kt.items()},
+             "kernels": kernel_table(kt, p_tot, p_steps)}
     workload = (f"fiducial 3D disk, single swarm of {args.n} representatives, K={K}, coagulation+fragmentation, "
                 f"h_op={args.horizon}, operator-only with the state resident in HBM")
     if learned is not None:
@@ -543,9 +595,11 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(args.cpu_sample, args.cpu_steps, args.horizon, threads)
+        host = host_cores()
+        log(f"cpu baseline on {host['threads']} threads ({host['cpu_model']}, nproc {host['nproc']}, "
+            f"affinity {host['affinity']}, OMP_NUM_THREADS {host['omp_num_threads']})")
+        cpu = cpu_baseline(args.cpu_sample, args.cpu_steps, args.horizon, host["threads"])
+        cpu["host"] = host
 
     out = {
         "metric": METRIC, "value": value, "unit": "collision updates/s", "n_gpus": 1, "steps": args.steps,

@@ -18,7 +18,8 @@ DEFER_WRITEBACK = 1
 PROFILE = 2
 DIAGNOSTICS = 4
 SEARCH_MORTON, SEARCH_KDTREE = 0, 1
-KERNELS = ("publish", "bath_rate", "screen", "chain", "chain_audit", "majorant")
+KERNELS = ("publish", "bath_rate", "screen", "chain", "chain_audit", "majorant", "candidates", "mark_stale",
+           "queue_prefix")
 MAJORANT_FEATURES = 8
 MAJORANT_PARAMS = 4801
 
@@ -53,7 +54,7 @@ class Stats(C.Structure):
                 ("pair_evals", C.c_longlong), ("dt_col", C.c_double), ("clock_dyn", C.c_double),
                 ("max_chain_events", C.c_longlong), ("thinned", C.c_longlong), ("candidates", C.c_longlong),
                 ("accepted", C.c_longlong), ("exact_checks", C.c_longlong), ("violations", C.c_longlong),
-                ("rated_owners", C.c_longlong), ("candidate_pairs", C.c_longlong)]
+                ("rated_owners", C.c_longlong), ("candidate_pairs", C.c_longlong), ("bound_evals", C.c_longlong)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -114,6 +115,7 @@ _SIGS = {
     "gdx_col_use_majorant": [P, P, D, I],
     "gdx_col_set_leaf_target": [P, I],
     "gdx_col_set_bound_reuse": [P, I],
+    "gdx_col_set_fixed_bath": [P, D],
     "gdx_col_set_profile": [P, I],
     "gdx_col_autotune": [P, P, I, I, P, P],
 }

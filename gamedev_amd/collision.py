@@ -216,6 +216,10 @@ class CollisionOperator:
         depend on it."""
         self._call(self.lib.gdx_col_set_bound_reuse, int(bool(on)))
 
+    def set_fixed_bath(self, duration: float) -> None:
+        """Test access: every bath of later calls lasts min(duration, horizon); 0 restores the controller."""
+        self._call(self.lib.gdx_col_set_fixed_bath, float(duration))
+
     def autotune(self, candidates=(16, 32, 64, 128, 256), repeats: int = 2) -> dict:
         """Time the geometry epoch per Morton leaf target, keep the fastest (a valid epoch is left built
         with it); returns {"best": leaf, "ms": {leaf: ms}}."""

@@ -335,6 +335,11 @@ struct op_run
     int bin_end(int g) const { return g + 1 < LOCAL_GROUPS ? binmap[(g + 1)*COL_BIN_S] : merged; }
     double requested_step(int g, int *binding = nullptr) const
     {
+        if (c.fixed_bath > 0.0)
+        {
+            if (binding) *binding = 0;
+            return std::min(duration, c.fixed_bath);
+        }
         int first = binmap[g*COL_BIN_S];
         return choose_bath(ratebin.data() + first, bin_end(g) - first, duration, c.state[g].limit_scale, binding);
     }
@@ -563,7 +568,7 @@ struct op_run
                 stats.thinned += cnt - thin_rated(cnt, sub);
                 sub_phase[g]++;
             }
-            // candidates above their bound (each restarted on the plain chain) also raise the scale
+            // candidates above their bound (each taken exactly by superposition) also raise the scale
             const double checks = double(wave_ctr[GDX_WAVE_CHECKS]), viol = double(wave_ctr[GDX_WAVE_VIOLATIONS]);
             if (viol > 0.01*checks + 3.0) adapt_bound(1.25);
         }
@@ -595,6 +600,7 @@ struct op_run
         stats.exact_checks = (long long)counters[GDX_CTR_CHECKS];
         stats.violations = (long long)counters[GDX_CTR_VIOLATIONS];
         stats.candidate_pairs = (long long)counters[GDX_CTR_CAND_PAIRS];
+        stats.bound_evals = (long long)counters[GDX_CTR_BOUND_EVALS];
         stats.dt_col = dt_col;
         stats.clock_dyn = duration;
         if (diag && dg->log.is_open())
@@ -687,7 +693,9 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
             launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
-            timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024, rb, track); });
+            timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024, rb); });
+            if (track && pw.blocks1024)
+                timer.time(GDX_COL_KERNEL_STALE, pw.w256.total, [&] { launch_mark_stale(st, pw.w1024, rb); });
             if (pw.blocks_maj > 0)
                 timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
             timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.wsub, pw.blocks_sub); });
@@ -714,7 +722,11 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.work.p, 2*sizeof(int));
             launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
-            timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256, rb.qprefix.p); });
+            timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256); });
+            if (pw.w256.learned && pw.blocks256)
+                timer.time(GDX_COL_KERNEL_CANDIDATES, pw.w256.total, [&] { launch_candidates(st, pw.w256, pw.blocks256); });
+            if (pw.blocks256)
+                timer.time(GDX_COL_KERNEL_PREFIX, pw.w256.total, [&] { launch_queue_prefix(st, pw.w256, rb.qprefix.p); });
             timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.work.p, rb.resident); });
             timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w256, pw.blocks256); });
             rb.read_back(rb.off_wave, rb.back_bytes, st);          // wave counters, errors, audit bins
@@ -931,6 +943,9 @@ int gdx_col_reset_controller(gdx_col_handle h)
 {
     if (!h) return GDX_COL_ERR_ARG;
     h->c.state.assign(LOCAL_GROUPS, context::bath_state{});
+    // the learned bound's adaptive scale is controller memory too: a restart from a checkpoint written
+    // at this boundary continues with the same scale
+    h->c.safety_scale = 1.0;
     h->diag.summary = controller_summary{};
     return GDX_COL_OK;
 }
@@ -1198,7 +1213,8 @@ int gdx_col_rate_pass(gdx_col_handle h)
         launch_reset_scratch(c.stream, rb.table.p, LOCAL_GROUPS);
         hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, c.stream), "zero rate bins");
         phase_wave pw = upload_phase(rb, e, c.stream);
-        launch_publish(c.stream, pw.w1024, pw.blocks1024, rb, c.model && c.rev_valid);
+        launch_publish(c.stream, pw.w1024, pw.blocks1024, rb);
+        if (c.model && c.rev_valid && pw.blocks1024) launch_mark_stale(c.stream, pw.w1024, rb);
         launch_bath_rate(c.stream, pw.wsub, pw.blocks_sub);
         hip_check(hipStreamSynchronize(c.stream), "rating pass");
     });
@@ -1283,6 +1299,15 @@ int gdx_col_set_bound_reuse(gdx_col_handle h, int on)
     return guard(&h->c, [&] {
         h->c.bound_reuse = on ? 1 : 0;
         invalidate_bounds(h->c);
+    });
+}
+
+int gdx_col_set_fixed_bath(gdx_col_handle h, double duration)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(std::isfinite(duration) && duration >= 0.0, GDX_COL_ERR_ARG, "fixed bath duration must be >= 0");
+        h->c.fixed_bath = duration;
     });
 }
 

@@ -576,6 +576,54 @@ GDX_FN double vrel_pair_terms(const owner_terms &t, double size_j)
     return sqrt(dvr*dvr + dvphi*dvphi + dvz*dvz + vt*vt + vb*vb);
 }
 
+// partner-only factors of one slot, held for the whole of an owner's event chain: the owner's position
+// factors and the partner's published record do not change during the bath, so these are the values
+// vrel_pair_terms would otherwise recompute after every event, by the same operations (the vertical
+// overlap of a vertically integrated disk stays per event: holding it too costs the chain kernel a
+// wavefront per SIMD)
+struct partner_terms
+{
+    double sj, fj, aj, mj;
+};
+// the chain holds them where its registers allow two wavefronts per SIMD (the 3D physical kernel; a
+// vertically integrated one needs the partner's scale height besides)
+constexpr bool HOLD_PARTNER = COAG_KERNEL == 3 && N_Z > 1;
+
+GDX_FN partner_terms make_partner_terms(const owner_terms &t, double size_j)
+{
+    partner_terms p{0.0, 0.0, 0.0, 0.0};
+    if constexpr (COAG_KERNEL == 3)
+    {
+        p.sj = stokes_partner(t, size_j);
+        p.fj = 1.0 / (1.0 + p.sj*p.sj);
+        p.aj = p.sj*p.fj;
+        p.mj = fmin(p.sj, 0.5);
+    }
+    else
+    {
+        (void)t; (void)size_j;
+    }
+    return p;
+}
+
+// vrel_pair_terms from held partner factors: bit for bit the same speed
+GDX_FN double vrel_pair_held(const owner_terms &t, const partner_terms &p, double size_j)
+{
+    double dvr = 2.0*t.vn*(t.ai - p.aj);
+    double dvphi = t.vn*(t.fi - p.fj);
+    double dvz = t.zo*(t.mi - p.mj);
+    double vt = vrel_t_merged(t.ris, t.vg_sq, t.si, p.sj);
+    double vb = 0.0;
+#ifndef CODE_UNIT
+    double mi = grain_mass(t.size_i);
+    double mj = grain_mass(size_j);
+    vb = fmin(sqrt(8.0*t.cs*t.cs*M_MOL*(mi + mj) / (PI*mi*mj)), t.cs);
+#else
+    (void)size_j;
+#endif
+    return sqrt(dvr*dvr + dvphi*dvphi + dvz*dvz + vt*vt + vb*vb);
+}
+
 // catastrophic-fragmentation moments (_col_event.cuh:62-73): they depend on the owner size alone
 struct cat_moments
 {
@@ -672,8 +720,9 @@ GDX_FN partner load_partner(int j, const double2 *res, const double *hgs)
     return q;
 }
 
-template <class CM>
-GDX_FN pair_term eval_pair_terms(const owner_terms &t, const CM &cm, const partner &q, double lambda_0)
+template <class CM, bool HELD = false>
+GDX_FN pair_term eval_pair_terms(const owner_terms &t, const CM &cm, const partner &q, double lambda_0,
+                                 const partner_terms &pt = partner_terms{})
 {
     pair_term p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (!q.valid) return p;
@@ -694,7 +743,7 @@ GDX_FN pair_term eval_pair_terms(const owner_terms &t, const CM &cm, const partn
     }
     else
     {
-        vrel = vrel_pair_terms(t, size_j);
+        vrel = HELD ? vrel_pair_held(t, pt, size_j) : vrel_pair_terms(t, size_j);
         rate = numr_j*vrel*PI*(size_i + size_j)*(size_i + size_j) / 4.0;
         if constexpr (N_Z == 1) rate /= sqrt(2.0*PI*(t.hg*t.hg + q.hg_j*q.hg_j));
     }
@@ -728,7 +777,9 @@ GDX_FN double outcome_factor(const owner_terms &t, double sj, bool high_speed)
 }
 
 // eval_pair_terms without the jump moments: the same rate bit for bit
-GDX_FN pair_term eval_pair_rate(const owner_terms &t, const partner &q, double lambda_0)
+template <bool HELD = false>
+GDX_FN pair_term eval_pair_rate(const owner_terms &t, const partner &q, double lambda_0,
+                                const partner_terms &pt = partner_terms{})
 {
     pair_term p{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (!q.valid) return p;
@@ -749,7 +800,7 @@ GDX_FN pair_term eval_pair_rate(const owner_terms &t, const partner &q, double l
     }
     else
     {
-        vrel = vrel_pair_terms(t, size_j);
+        vrel = HELD ? vrel_pair_held(t, pt, size_j) : vrel_pair_terms(t, size_j);
         rate = numr_j*vrel*PI*(size_i + size_j)*(size_i + size_j) / 4.0;
         if constexpr (N_Z == 1) rate /= sqrt(2.0*PI*(t.hg*t.hg + q.hg_j*q.hg_j));
     }

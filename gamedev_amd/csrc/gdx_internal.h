@@ -315,6 +315,7 @@ struct context
     double safety = 1.2;
     double safety_scale = 1.0; // learned: adaptive factor on the bound, kept across operator calls
     int majorant_min_owners = GDX_COL_MAJORANT_MIN_OWNERS;   // smaller swarms keep the plain method
+    double fixed_bath = 0.0;   // > 0: every bath lasts this long (test access; the controller still audits)
     bool diag = false;         // GDX_COL_DIAGNOSTICS
     dbuf<gdx_event_work> evw;  // per-slot event statistics of the current call
     dbuf<gdx_event_work> ev_sum;
@@ -352,9 +353,12 @@ void knn_query(const morton_index &ix, const float4 *queries, int nq, int k, int
 void kd_build(context &c);
 void kd_query(context &c);
 void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng);
-void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb, bool mark);
+void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb);
+void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb);
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
-void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix);
+void launch_screen(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_queue_prefix(hipStream_t st, const gdx_wave &w, int *qprefix);
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_zero(hipStream_t st, const zero_ranges &z);
 void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, double *rate);

@@ -129,7 +129,7 @@ struct gdx_gref
 enum
 {
     GDX_CTR_EVENTS, GDX_CTR_CHAINED, GDX_CTR_PAIRS, GDX_CTR_LONGEST, GDX_CTR_THINNED, GDX_CTR_CANDIDATES,
-    GDX_CTR_ACCEPTED, GDX_CTR_CHECKS, GDX_CTR_VIOLATIONS, GDX_CTR_CAND_PAIRS, GDX_CTRS
+    GDX_CTR_ACCEPTED, GDX_CTR_CHECKS, GDX_CTR_VIOLATIONS, GDX_CTR_CAND_PAIRS, GDX_CTR_BOUND_EVALS, GDX_CTRS
 };
 // per-wave counters per swarm
 enum

@@ -599,6 +599,7 @@ __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
     if (stale) list[base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
     __syncthreads();
     const int n = count;
+    if (threadIdx.x == 0 && n > 0) atomicAdd(G.counters + GDX_CTR_BOUND_EVALS, (unsigned long long)n);
     // wavefront-uniform: the wavefronts past the list are done, no block barrier follows
     if (wv*WAVE >= n) return;
     const bool has = int(threadIdx.x) < n;
@@ -872,9 +873,10 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 
 // thinning candidates of learned swarms, one lane per candidate: the exact K-sum at the candidate time
 // (the owner and its frozen partners are unchanged since the bath start, so this is the bath-start
-// rate). Above the bound: the owner restarts the plain chain from the bath start on fresh draws.
-// Otherwise the candidate is an event with probability R / bound; a rejected candidate continues on R
-// from the candidate time. The owner's first event (at the candidate time when accepted, else at the
+// rate). Below the bound the candidate is an event with probability R / bound, and a rejected candidate
+// continues on R from the candidate time. Above the bound the exact process is the bound's process plus
+// an independent one of rate R - bound: the first event is the earlier of the candidate time and that
+// process's first point, so the owner's event time is still exactly distributed. The owner's first event (at the candidate time when accepted, else at the
 // end of its next wait if that falls inside the bath) is taken here, with the draws and operations of
 // the chain: the partner walk over the same rates in slot order, the outcome, then the K-sum at the new
 // size and the next wait. Owners whose next event still falls inside the bath enter the chain after
@@ -922,21 +924,31 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
         {
             chain = true;                                  // the chain reports the invalid rate
         }
-        else if (R > c.y)
-        {
-            check = viol = chain = true;
-        }
         else
         {
             check = true;
-            accepted = col_uniform(pidx, seed, ctr)*c.y < R;
-            // the first event: at the candidate time, or after the next wait on R
-            bool event = accepted;
+            bool event;
             double t_event = c.x;
-            uint64_t k = ctr;                              // draws taken here, committed at the end
-            if (accepted)
+            if (R > c.y)
             {
-                qs = make_double2(c.x, 1.0);
+                // the bound is below the exact rate: the exact process is the bound's process plus an
+                // independent one of rate R - bound, so the owner's first event is the earlier of the
+                // candidate time and that process's first point (both inside the bath)
+                viol = true;
+                const double wx = -log(col_uniform(pidx, seed, ctr)) / (R - c.y);
+                if (wx < c.x) t_event = wx;
+                event = true;
+            }
+            else
+            {
+                accepted = col_uniform(pidx, seed, ctr)*c.y < R;
+                event = accepted;
+            }
+            // the first event: at t_event, or after the next wait on R from the candidate time
+            uint64_t k = ctr;                              // draws taken here, committed at the end
+            if (event)
+            {
+                qs = make_double2(t_event, 1.0);
             }
             else if (R > 0.0)
             {
@@ -1213,6 +1225,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     owner_terms t{};
     cat_moments cm{0.0, 0.0, 0.0};
     partner q{0.0, 0.0, 0.0, false};
+    partner_terms pt{0.0, 0.0, 0.0, 0.0};
     // the owner's next SEG draws, one per lane of the segment (lane kl: draw pool_base + kl, and its
     // -log for waits), computed side by side and taken in stream order: the same numbers as drawing
     // one at a time, for one Philox block and one logarithm per SEG draws
@@ -1312,8 +1325,10 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 moments = exact_owner(w, gi, s);
                 // the catastrophic moments enter only the jump moments
                 if (moments) cm = catastrophic_moments(cur.x);
-                // the partners are the published states: fixed for the whole chain
+                // the partners are the published states: fixed for the whole chain, and so are their
+                // partner-only factors
                 q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
+                if constexpr (HOLD_PARTNER) pt = make_partner_terms(t, q.size_j);
                 time = hazard = j1 = j2 = jmax = 0.0;
                 first_event = false;
                 // a learned group's queue entries carry their start (plain groups of a mixed wave do not)
@@ -1345,7 +1360,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         // an event takes at most three draws
         if (pool_used > SEG - 3) refill(pool_base + uint64_t(pool_used));
         // owners outside a learned swarm's audited subsample keep no compensators: rates alone
-        const pair_term p = moments ? eval_pair_terms(t, cm, q, lambda_0) : eval_pair_rate(t, q, lambda_0);
+        const pair_term p = moments ? eval_pair_terms<cat_moments, HOLD_PARTNER>(t, cm, q, lambda_0, pt)
+                                    : eval_pair_rate<HOLD_PARTNER>(t, q, lambda_0, pt);
         const bool bad_rate = !isfinite(p.rate) || p.rate < 0.0;
         double R = seg_fold<false>(p.rate);
         // the jump moments only where a segment of the wavefront keeps compensators
@@ -1579,14 +1595,19 @@ void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng)
 }
 
 // mark: some swarm of the wave tracks stale bounds (has a majorant)
-void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb, bool mark)
+void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb)
 {
     if (!blocks) return;
     rb.changed.alloc(size_t(std::max(1, w.total)));
     hip_check(hipMemsetAsync(rb.nchanged.p, 0, sizeof(int), st), "zero changed count");
     k_publish<<<xcd_grid(blocks), BLOCK, 0, st>>>(w, rb.changed.p, rb.nchanged.p);
     hip_check(hipGetLastError(), "launch publish");
-    if (!mark) return;
+}
+
+// after launch_publish of the same wave: the slots it listed mark their readers stale
+void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb)
+{
+    if (!w.total) return;
     constexpr int SEGS = BLOCK / SEG;
     const int grid = std::max(1, std::min(4096, (w.total + SEGS - 1) / SEGS));
     k_mark_stale<<<grid, BLOCK, 0, st>>>(w.table, rb.changed.p, rb.nchanged.p);
@@ -1666,16 +1687,23 @@ void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, do
     hip_check(hipGetLastError(), "launch sample");
 }
 
-void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, int *qprefix)
+void launch_screen(hipStream_t st, const gdx_wave &w, int blocks)
 {
     if (!blocks) return;
     k_screen<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch screen");
-    if (w.learned)
-    {
-        k_candidates<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
-        hip_check(hipGetLastError(), "launch candidates");
-    }
+}
+
+// learned waves only, after the screen of the same wave
+void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks)
+{
+    if (!blocks || !w.learned) return;
+    k_candidates<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    hip_check(hipGetLastError(), "launch candidates");
+}
+
+void launch_queue_prefix(hipStream_t st, const gdx_wave &w, int *qprefix)
+{
     k_queue_prefix<<<1, 256, 0, st>>>(w, qprefix);
     hip_check(hipGetLastError(), "launch queue prefix");
 }

@@ -129,7 +129,7 @@ int main(int argc, char **argv)
     }
     long long trials = argc > 1 ? std::atoll(argv[1]) : 1000000;
     lcg g{argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1ull};
-    long long bad[6] = {0, 0, 0, 0, 0, 0}, reg[6] = {0, 0, 0, 0, 0, 0}, branch[3] = {0, 0, 0};
+    long long bad[7] = {0, 0, 0, 0, 0, 0, 0}, reg[6] = {0, 0, 0, 0, 0, 0}, branch[3] = {0, 0, 0};
     for (long long n = 0; n < trials; ++n)
     {
         double y = Y_MIN + (Y_MAX - Y_MIN)*g.next();
@@ -158,6 +158,21 @@ int main(int argc, char **argv)
             bad[5]++;
         }
 
+        // the chain's held partner factors, taken at the bath start (another owner size) and kept across
+        // the owner's resize after an event: the same terms bit for bit
+        owner_terms t0 = make_owner(e, size_i*(0.5 + g.next()), m, hg_i);
+        const partner_terms held = make_partner_terms(t0, size_j);
+        owner_resize(t0, size_i);
+        pair_term h = eval_pair_terms<cat_moments, true>(t0, cm, q, lambda_0, held);
+        pair_term hr = eval_pair_rate<true>(t0, q, lambda_0, held);
+        const double hv[6] = {h.rate, h.j1, h.j2, h.jmax, h.vrel, hr.rate}, bw[6] = {b.rate, b.j1, b.j2, b.jmax, b.vrel, b.rate};
+        for (int f = 0; f < 6; ++f)
+            if (bits(hv[f]) != bits(bw[f]) && !(std::isnan(hv[f]) && std::isnan(bw[f])))
+            {
+                if (bad[6] < 3) std::printf("held-partner mismatch field %d: size_i %.17g size_j %.17g\n", f, size_i, size_j);
+                bad[6]++;
+            }
+
         if constexpr (COAG_KERNEL == 3) reg[regime(e, size_i, size_j)]++;
         double ratio = size_j / size_i;
         bool high = a.vrel >= V_FRAG;
@@ -177,6 +192,7 @@ int main(int argc, char **argv)
     std::printf("model %s trials %lld\n", GDX_MODEL_NAME, trials);
     std::printf("regimes %lld %lld %lld %lld %lld %lld\n", reg[0], reg[1], reg[2], reg[3], reg[4], reg[5]);
     std::printf("branches %lld %lld %lld\n", branch[0], branch[1], branch[2]);
-    std::printf("mismatches %lld %lld %lld %lld %lld %lld\n", bad[0], bad[1], bad[2], bad[3], bad[4], bad[5]);
-    return (bad[0] | bad[1] | bad[2] | bad[3] | bad[4] | bad[5]) ? 1 : 0;
+    std::printf("mismatches %lld %lld %lld %lld %lld %lld %lld\n", bad[0], bad[1], bad[2], bad[3], bad[4], bad[5],
+                bad[6]);
+    return (bad[0] | bad[1] | bad[2] | bad[3] | bad[4] | bad[5] | bad[6]) ? 1 : 0;
 }

@@ -7,6 +7,8 @@ fiducial_3d_state:  1e7 representatives by default, full 3D, r uniform on [0.3, 
                     Gaussian about pi/2 with h/r = 0.05 r^0.25, log-normal sizes (median 1e-4,
                     sigma_ln = 1, clipped to [1e-5, 1e-1]), equal-mass representatives, total dust
                     mass = 0.01 x the gas mass 2 pi Sigma_0 (3.0 - 0.3).
+smoluchowski_state: the constant-kernel Smoluchowski benchmark: unit monomers of unit mass on an
+                    annulus, unit-volume measure (model smol_const).
 clumped_3d_state:   the clumped stress case: the fiducial background with a vortex overdensity (x10,
                     width 0.05 r, at r = 1, phi = pi) and a pressure-bump ring (x5, width 0.03, at
                     r = 1.5), sizes log-normal with sigma_ln = 1.5.
@@ -100,6 +102,20 @@ def clumped_3d_state(n: int, seed: int = 0, r_min: float = 0.3, r_max: float = 3
     w = total_mass/n
     numr = w/grain_mass(s)
     return {"x": phi, "y": r, "z": theta, "size": s, "numr": numr, "total_mass": total_mass}
+
+
+def smoluchowski_state(n: int = 65536, seed: int = 0, total_mass: float = 1e30) -> dict:
+    """Constant-kernel Smoluchowski benchmark (model smol_const): unit monomers (diameter 1, density 6/pi, so
+    mass 1) on a full-azimuth annulus 0.5 <= r < 1.5, equal represented mass total_mass / n
+    (val/paper/smoluchowski/README.md: BENCHMARK_MASS = 1e30; positions uniform instead of the campaign's
+    jittered ring grid, which the unit-volume measure makes irrelevant to the rates)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = rng.uniform(0.5, 1.5, n)
+    phi = rng.uniform(-math.pi, math.pi, n)
+    z = np.full(n, 0.5*math.pi)
+    s = np.ones(n)
+    numr = np.full(n, total_mass/n)
+    return {"x": phi, "y": r, "z": z, "size": s, "numr": numr, "total_mass": total_mass}
 
 
 def ensemble_sizes(count: int, lo: float, hi: float, seed: int = 12345, total: int | None = None) -> list[int]:

@@ -53,7 +53,10 @@ extern "C" {
 #define GDX_COL_KERNEL_CHAIN     3
 #define GDX_COL_KERNEL_AUDIT     4
 #define GDX_COL_KERNEL_MAJORANT  5   /* learned: features and network per due owner */
-#define GDX_COL_KERNELS          6
+#define GDX_COL_KERNEL_CANDIDATES 6  /* learned: exact K-sum and first event of each thinning candidate */
+#define GDX_COL_KERNEL_STALE     7   /* learned: stale marking of the owners whose inputs changed */
+#define GDX_COL_KERNEL_PREFIX    8   /* per-group queue prefix of a chain phase */
+#define GDX_COL_KERNELS          9
 
 typedef struct gdx_col_ctx *gdx_col_handle;
 
@@ -93,6 +96,7 @@ typedef struct {
     long long violations;           /* of those, exact rate above the majorant */
     long long rated_owners;         /* owner baths rated with the exact K-sum by rate passes */
     long long candidate_pairs;      /* of pair_evals, those of the thinning-candidate pass */
+    long long bound_evals;          /* owner bounds evaluated by the network (kept bounds excluded) */
 } gdx_col_stats;
 
 /* library and model */
@@ -140,8 +144,13 @@ int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);
  * reference runs one swarm per process); for ensembles of small realizations. Launches go to the
  * first handle's stream; out holds `count` entries. */
 int gdx_col_operator_batch(const gdx_col_handle *handles, int count, double duration, gdx_col_stats *out);
-/* controller memory reset at output boundaries (src/swarm/swarm_runtime.cu:491-496) */
+/* controller memory reset at output boundaries (src/swarm/swarm_runtime.cu:491-496); also resets the
+ * adaptive scale of an attached majorant's bound */
 int gdx_col_reset_controller(gdx_col_handle h);
+/* test access, no reference counterpart: > 0 makes every bath of later operator calls last
+ * min(duration, horizon) instead of the controller's choice (audits still run); 0 restores the
+ * controller. Two operators with the same fixed bath follow the same schedule whatever their rates. */
+int gdx_col_set_fixed_bath(gdx_col_handle h, double duration);
 int gdx_col_synchronize(gdx_col_handle h);
 
 /* ---- diagnostics (handles created with GDX_COL_DIAGNOSTICS) ---- */

@@ -1,6 +1,8 @@
 """The learned majorant inside the operator, on the GPU through the C ABI: training samples against a
 restatement of the features and the oracle's exact rates, ensemble agreement of the thinned operator
 with the plain one, the fallback when the bound fails, and what the trained network gains."""
+import math
+
 import numpy as np
 import pytest
 
@@ -161,6 +163,46 @@ def test_learned_operator_matches_plain_ensemble(gpu):
     print(f"[learned] checks {checks} violations {viol} candidates {cand} accepted {acc}", flush=True)
     assert checks > 0 and viol/checks < 0.01
     assert cand > 0 and 0 < acc <= cand
+
+
+def test_learned_events_match_plain_on_fixed_schedule(gpu):
+    """With every bath fixed at the same duration both operators follow one schedule, so the learned
+    operator's event process must be the plain one in distribution: the mean event count and the mean
+    changes of M0, M2 and M3 agree within 4 standard errors of the seed ensembles, with no further
+    allowance. (With the adaptive controller the learned swarm's 1/16-subsample controller picks other
+    baths, and the frozen-bath approximation then differs at the order of COL_BATH_EPS.)"""
+    from gamedev_amd import synth
+    n, h, dt, steps, seeds = 262144, 0.05, 0.05/8, 3, 12
+    st = synth.fiducial_3d_state(n, seed=41)
+    op, _ = _operator("fiducial_3d", st, gpu, seed=600)
+    m, _, safety = _train("fiducial_3d", op, 0.002, 3, gpu)
+    M0 = _moments(synth.to_device(st, gpu))
+    res = {"plain": [], "learned": []}
+    for mode in res:
+        for k in range(seeds):
+            o, d = _operator("fiducial_3d", st, gpu, seed=2000 + k)
+            o.set_fixed_bath(dt)
+            if mode == "learned":
+                o.use_majorant(m, safety, min_owners=1)
+            stats = [o.evolve(h) for _ in range(steps)]
+            assert all(s["waves"] == 8 for s in stats)
+            res[mode].append((sum(s["events"] for s in stats), _moments(d) - M0, stats))
+        print(f"[learned] fixed schedule {mode}: events {[r[0] for r in res[mode]]}", flush=True)
+    ev = {k: np.array([r[0] for r in v], float) for k, v in res.items()}
+    se = math.sqrt(ev["plain"].var(ddof=1)/seeds + ev["learned"].var(ddof=1)/seeds)
+    print(f"[learned] fixed schedule: mean events plain {ev['plain'].mean():.1f} learned "
+          f"{ev['learned'].mean():.1f} (se of the difference {se:.1f})", flush=True)
+    assert ev["plain"].mean() > 10000
+    assert abs(ev["learned"].mean() - ev["plain"].mean()) <= 4*se
+    for j in (0, 2, 3):
+        dp = np.array([r[1][j] for r in res["plain"]])
+        dl = np.array([r[1][j] for r in res["learned"]])
+        sj = math.sqrt(dp.var(ddof=1)/seeds + dl.var(ddof=1)/seeds)
+        print(f"[learned] fixed schedule: change of M{j} plain {dp.mean():.4e} learned {dl.mean():.4e} "
+              f"(se {sj:.2e})", flush=True)
+        assert abs(dl.mean() - dp.mean()) <= 4*sj
+    st_l = [s for r in res["learned"] for s in r[2]]
+    assert sum(s["thinned"] for s in st_l) > 0 and sum(s["candidates"] for s in st_l) > 0
 
 
 def test_failed_bound_falls_back_to_plain(gpu):

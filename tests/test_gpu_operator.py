@@ -166,8 +166,8 @@ def test_profile_mode_same_result(gpu):
             kt = op.kernel_times()
             assert set(kt) == set(_lib.KERNELS)
             for name, rec in kt.items():
-                if name == "majorant":
-                    # no majorant attached: the bound kernel never runs
+                if name in ("majorant", "candidates", "mark_stale"):
+                    # no majorant attached: the learned kernels never run
                     assert rec["launches"] == 0, name
                     continue
                 assert rec["launches"] > 0 and rec["ms"] > 0.0 and rec["owners"] > 0, name
