@@ -1316,12 +1316,21 @@ int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety, in
     if (!h) return GDX_COL_ERR_ARG;
     return guard(&h->c, [&] {
         require(!m || (std::isfinite(safety) && safety >= 1.0), GDX_COL_ERR_ARG, "majorant safety factor must be >= 1");
+        require(!m || (long long)h->c.n <= MAJORANT_MAX_OWNERS, GDX_COL_ERR_ARG,
+                "a swarm with a majorant holds at most 2^32 / N_K representatives");
         h->c.mlp = majorant_pack(m);
         h->c.model = m;
         invalidate_bounds(h->c);
         h->c.safety = m ? safety : 1.0;
         h->c.majorant_min_owners = min_owners > 0 ? min_owners : GDX_COL_MAJORANT_MIN_OWNERS;
     });
+}
+
+int gdx_col_majorant_max_owners(long long *out)
+{
+    if (!out) return GDX_COL_ERR_ARG;
+    *out = MAJORANT_MAX_OWNERS;
+    return GDX_COL_OK;
 }
 
 // ---- host controller and scheduler ----
@@ -1393,6 +1402,43 @@ int gdx_col_host_schedule(int n, double horizon, const double *requested, const 
             if (tick != 0) s.adapt(tick, groups, req, passed);
             ticks[w] = tick;
             due_counts[w] = int(groups.size());
+            ++w;
+            s.advance(groups);
+        }
+        *waves = w;
+    });
+}
+
+int gdx_col_host_schedule_adaptive(int n, double horizon, const double *requested, const unsigned int *edges,
+                                   int max_waves, const double *wave_requested, const unsigned char *wave_passed,
+                                   unsigned long long *ticks, int *due_counts, int *wave_levels, int *waves)
+{
+    if (n <= 0 || !requested || !edges || !wave_requested || !wave_passed || !ticks || !due_counts || !wave_levels
+        || !waves)
+        return GDX_COL_ERR_ARG;
+    return guard(nullptr, [&] {
+        std::vector<double> req(requested, requested + n);
+        std::vector<uint32_t> e(edges, edges + size_t(n)*((n + 31) / 32));
+        schedule s(horizon, req, e);
+        std::vector<bool> passed(n, true);
+        int w = 0;
+        while (s.time() < s.end)
+        {
+            if (w >= max_waves) throw status_error(GDX_COL_ERR_ARG, "schedule exceeds max_waves");
+            uint64_t tick = s.time();
+            std::vector<int> groups = s.due(tick);
+            if (tick != 0)
+            {
+                for (int c = 0; c < n; ++c)
+                {
+                    req[c] = wave_requested[size_t(w)*n + c];
+                    passed[c] = wave_passed[size_t(w)*n + c] != 0;
+                }
+                s.adapt(tick, groups, req, passed);
+            }
+            ticks[w] = tick;
+            due_counts[w] = int(groups.size());
+            for (int c = 0; c < n; ++c) wave_levels[size_t(w)*n + c] = s.level[c];
             ++w;
             s.advance(groups);
         }

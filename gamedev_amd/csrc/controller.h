@@ -1,7 +1,11 @@
 // Host bath controller and power-of-two scheduler of the collision operator.
 //
-// Follows the reference's host logic: change_limit, _build_col_binmap, _choose_col_bath and
-// _finish_col_bath (inc/swarm/_col_bound.cuh:22-216) and local_schedule (inc/swarm/_col_sched.cuh:17-183).
+// The controller formulas (change_limit, choose_bath, the Bernstein envelopes of finish_bath) keep the
+// floating-point expression order of the reference's _col_bound.cuh:22-216, because bath durations and
+// audit decisions must coincide with it. The integer parts (merged size bins, the safety-factor streaks,
+// the scheduler of _col_sched.cuh:17-183) are written from the rules the reference documents
+// (doc/guide_swarm.md 8.7.1, 8.7.3, 8.7.5); their results are fixed by those rules, so they coincide too
+// (the host tests compare them with the oracle's restatement on hand-made and random inputs).
 // Failures raise status_error with the library status of the reference's std::runtime_error.
 #ifndef GDX_CONTROLLER_H
 #define GDX_CONTROLLER_H
@@ -45,42 +49,44 @@ inline std::pair<double, int> change_limit(double A, double B, double epsilon, d
     return r;
 }
 
-// merge adjacent sparse size bins inside every spatial group until each holds COL_BIN_MIN owners
+// Merged size bins (the rule of the reference's controller, doc/guide_swarm.md 8.7.1): inside each
+// spatial group, adjacent raw size bins are merged from small to large sizes, a merged bin being closed
+// as soon as it holds COL_BIN_MIN owners; a tail that stays below COL_BIN_MIN joins the merged bin before
+// it (if there is one); the bins above the largest occupied one form one further, empty merged bin, and
+// a group with no owners is one empty merged bin. Merged bins are numbered consecutively over groups.
+// Returns the number of merged bins; map[r] is the merged bin of raw bin r.
 inline int build_binmap(const std::vector<int> &raw_count, std::vector<int> &map)
 {
     const int groups = int(raw_count.size()) / COL_BIN_S;
     map.assign(raw_count.size(), -1);
-    int merged = 0;
+    int total = 0;
+    int local[COL_BIN_S];
     for (int gidx = 0; gidx < groups; ++gidx)
     {
-        const int *cnt = raw_count.data() + gidx*COL_BIN_S;
-        int *out = map.data() + gidx*COL_BIN_S;
-        int last = -1;
-        for (int b = 0; b < COL_BIN_S; ++b) if (cnt[b] > 0) last = b;
-        if (last < 0)
+        const int *cnt = raw_count.data() + size_t(gidx)*COL_BIN_S;
+        int top = COL_BIN_S - 1;
+        while (top >= 0 && cnt[top] <= 0) --top;
+        // group-local merged indices of the occupied range [0, top]
+        int closed = 0, fill = 0;
+        for (int b = 0; b <= top; ++b)
         {
-            int id = merged++;
-            for (int b = 0; b < COL_BIN_S; ++b) out[b] = id;
-            continue;
+            local[b] = closed;
+            fill += cnt[b];
+            if (fill >= COL_BIN_MIN && b < top) { ++closed; fill = 0; }
         }
-        int b0 = 0, prev = -1;
-        while (b0 <= last)
+        int used = top >= 0 ? closed + 1 : 0;
+        if (top >= 0 && fill < COL_BIN_MIN && closed > 0)
         {
-            int b1 = b0, sum = 0;
-            while (b1 <= last && sum < COL_BIN_MIN) sum += cnt[b1++];
-            bool tail = b1 > last && sum < COL_BIN_MIN && prev >= 0;
-            int id = tail ? prev : merged++;
-            for (int b = b0; b < b1; ++b) out[b] = id;
-            prev = id;
-            b0 = b1;
+            for (int b = 0; b <= top; ++b) if (local[b] == closed) local[b] = closed - 1;
+            used = closed;
         }
-        if (last + 1 < COL_BIN_S)
-        {
-            int id = merged++;
-            for (int b = last + 1; b < COL_BIN_S; ++b) out[b] = id;
-        }
+        for (int b = top + 1; b < COL_BIN_S; ++b) local[b] = used;   // the empty bins above
+        if (top + 1 < COL_BIN_S) ++used;
+        int *out = map.data() + size_t(gidx)*COL_BIN_S;
+        for (int b = 0; b < COL_BIN_S; ++b) out[b] = total + local[b];
+        total += used;
     }
-    return merged;
+    return total;
 }
 
 // binding: 0 horizon or COL_BATH_MAX, 1 mean-change bound, 2 variance bound (the last bin that shortened it)
@@ -104,6 +110,30 @@ inline double choose_bath(const rate_bin *bins, int count, double remaining, dou
     }
     if (binding) *binding = reason;
     return duration;
+}
+
+// The group's safety factor after an audited bath (doc/guide_swarm.md 8.7.5): halved (floor 0.25) after a
+// distribution overshoot or the second overshooting bath in a row of either kind; raised by 1.25 (to at
+// most 1) after the third quiet bath in a row; each change restarts the streak that caused it. Returns
+// whether this bath is the second overshooting bath in a row at the floor (a persistent overshoot).
+inline bool update_safety(bath_state &state, bool distribution, bool activity)
+{
+    const bool overshoot = distribution || activity;
+    const bool at_floor = state.limit_scale <= 0.25*(1.0 + 8.0*std::numeric_limits<double>::epsilon());
+    state.floor_streak = (overshoot && at_floor) ? state.floor_streak + 1 : 0;
+    state.activity_streak = overshoot ? state.activity_streak + 1 : 0;
+    state.quiet_streak = overshoot ? 0 : state.quiet_streak + 1;
+    if (distribution || state.activity_streak == 2)
+    {
+        state.limit_scale = std::max(0.25, 0.5*state.limit_scale);
+        state.activity_streak = 0;
+    }
+    else if (state.quiet_streak == 3)
+    {
+        state.limit_scale = std::min(1.0, 1.25*state.limit_scale);
+        state.quiet_streak = 0;
+    }
+    return state.floor_streak >= 2;
 }
 
 // realized activity and size change against Bernstein envelopes; adapt the group's safety factor
@@ -144,81 +174,84 @@ inline bath_result finish_bath(const audit_bin *bins, int count, bath_state &sta
     }
     r.d_bath = mass_difference / (2.0*total);
     r.distribution_overshoot = r.distribution_overshoot || r.d_bath > COL_BATH_EPS;
-    const bool issue = r.activity_overshoot || r.distribution_overshoot;
-    const bool at_floor = state.limit_scale <= 0.25*(1.0 + 8.0*std::numeric_limits<double>::epsilon());
-    if (issue)
-    {
-        state.activity_streak++;
-        state.quiet_streak = 0;
-        state.floor_streak = at_floor ? state.floor_streak + 1 : 0;
-    }
-    else
-    {
-        state.activity_streak = 0;
-        state.quiet_streak++;
-        state.floor_streak = 0;
-    }
-    if (r.distribution_overshoot || state.activity_streak >= 2)
-    {
-        state.limit_scale = std::max(0.25, 0.5*state.limit_scale);
-        state.activity_streak = 0;
-    }
-    else if (state.quiet_streak >= 3)
-    {
-        state.limit_scale = std::min(1.0, 1.25*state.limit_scale);
-        state.quiet_streak = 0;
-    }
-    r.persistent_overshoot = state.floor_streak >= 2;
+    r.persistent_overshoot = update_safety(state, r.distribution_overshoot, r.activity_overshoot);
     return r;
 }
 
-// groups advance on power-of-two fractions of the horizon, on a lattice of 2^52 ticks; linked
-// groups keep step ratios of at most 16
+// Power-of-two scheduler (the rules of doc/guide_swarm.md 8.7.3). Group c advances by step h / 2^level[c]
+// on a lattice of 2^52 ticks per horizon h; its level is the coarsest whose step does not exceed the
+// requested duration. Linked groups keep levels within 4 of each other (16:1), the coarser one refined.
+// At a refresh a due group may coarsen only if its audit passed, is refined until its step aligns with
+// the tick, is capped so that no pending endpoint of a group that is not due must move, and due groups
+// are then refined again until every link holds. Endpoints: next[c], advanced by step[c].
 struct schedule
 {
     static constexpr int FINEST = 52;
+    static constexpr int LINK_LEVELS = 4;
+    static constexpr int FAR = 1 << 20;       // hop count of unreachable pairs
     double horizon;
-    std::vector<std::pair<int, int>> links;
+    int n;
+    std::vector<std::vector<int>> adj;        // linked groups (an edge either way)
     std::vector<int> level;
     std::vector<uint64_t> step, next;
     uint64_t end;
 
-    static int level_for(double h, double requested, const char *what)
+    // the coarsest level whose step does not exceed `requested`
+    int level_for(double requested, const char *invalid, const char *too_fine) const
     {
-        if (!(requested > 0) || !std::isfinite(requested)) throw status_error(10, what);
-        int lv = 0;
-        double dt = h;
-        while (dt > requested)
+        if (!(requested > 0) || !std::isfinite(requested)) throw status_error(10, invalid);
+        for (int lv = 0; lv <= FINEST; ++lv)
+            if (std::ldexp(horizon, -lv) <= requested) return lv;
+        throw status_error(10, too_fine);
+    }
+
+    // hop counts from `src` along links whose interior groups satisfy `relay` (FAR if unreachable)
+    template <typename R>
+    std::vector<int> hops_from(int src, R &&relay) const
+    {
+        std::vector<int> d(n, FAR);
+        std::vector<int> frontier{src};
+        d[src] = 0;
+        for (size_t k = 0; k < frontier.size(); ++k)
         {
-            if (++lv > FINEST) throw status_error(10, "local collision timestep exceeds time resolution");
-            dt *= 0.5;
+            const int u = frontier[k];
+            if (u != src && !relay(u)) continue;
+            for (int v : adj[u])
+                if (d[v] == FAR) { d[v] = d[u] + 1; frontier.push_back(v); }
         }
-        return lv;
+        return d;
+    }
+
+    // the finest level any group forces on c through the 16:1 links: max over e of lv[e] - 4 hops(c, e),
+    // relayed only through groups in `mutable_set` (the others keep lv and pass nothing on)
+    std::vector<int> raise_for_links(const std::vector<int> &lv, const std::vector<bool> &mutable_set) const
+    {
+        std::vector<int> out = lv;
+        for (int c = 0; c < n; ++c)
+        {
+            if (!mutable_set[c]) continue;
+            const std::vector<int> d = hops_from(c, [&](int u) { return bool(mutable_set[u]); });
+            for (int e = 0; e < n; ++e)
+                if (d[e] != FAR) out[c] = std::max(out[c], lv[e] - LINK_LEVELS*d[e]);
+        }
+        return out;
     }
 
     schedule(double h, const std::vector<double> &requested, const std::vector<uint32_t> &edges)
-        : horizon(h), level(requested.size(), 0), step(requested.size()), next(requested.size(), 0)
+        : horizon(h), n(int(requested.size())), adj(requested.size()), level(requested.size(), 0),
+          step(requested.size()), next(requested.size(), 0)
     {
         if (!(h > 0) || !std::isfinite(h)) throw status_error(10, "invalid local collision horizon");
-        const int n = int(level.size());
         const int words = (n + 31) / 32;
-        auto edge = [&](int c, int d) { return (edges[size_t(c)*words + d / 32] >> (d % 32)) & 1u; };
-        for (int c = 0; c < n; ++c) level[c] = level_for(h, requested[c], "invalid local collision timestep");
         for (int c = 0; c < n; ++c)
-            for (int d = c + 1; d < n; ++d)
-                if (edge(c, d) || edge(d, c)) links.emplace_back(c, d);
-        bool changed;
-        do
-        {
-            changed = false;
-            for (int c = 0; c < n; ++c)
-                for (int d = 0; d < n; ++d)
-                {
-                    if (!edge(c, d)) continue;
-                    if (level[c] < level[d] - 4) { level[c] = level[d] - 4; changed = true; }
-                    if (level[d] < level[c] - 4) { level[d] = level[c] - 4; changed = true; }
-                }
-        } while (changed);
+            for (int d = 0; d < n; ++d)
+                if (c != d && (((edges[size_t(c)*words + d / 32] >> (d % 32)) & 1u)
+                               || ((edges[size_t(d)*words + c / 32] >> (c % 32)) & 1u)))
+                    adj[c].push_back(d);
+        for (int c = 0; c < n; ++c)
+            level[c] = level_for(requested[c], "invalid local collision timestep",
+                                 "local collision timestep exceeds time resolution");
+        level = raise_for_links(level, std::vector<bool>(n, true));
         end = uint64_t(1) << FINEST;
         for (int c = 0; c < n; ++c) step[c] = uint64_t(1) << (FINEST - level[c]);
     }
@@ -227,52 +260,34 @@ struct schedule
     std::vector<int> due(uint64_t tick) const
     {
         std::vector<int> out;
-        for (int c = 0; c < int(next.size()); ++c) if (next[c] == tick) out.push_back(c);
+        for (int c = 0; c < n; ++c) if (next[c] == tick) out.push_back(c);
         return out;
     }
     void adapt(uint64_t tick, const std::vector<int> &groups, const std::vector<double> &requested,
                const std::vector<bool> &passed)
     {
-        const int n = int(level.size());
-        std::vector<bool> active(n, false);
-        for (int c : groups) active[c] = true;
-        std::vector<int> cap(n, FINEST), cand = level;
-        for (int c = 0; c < n; ++c) if (!active[c]) cap[c] = level[c];
-        bool changed;
-        do
-        {
-            changed = false;
-            for (const auto &l : links)
-            {
-                int c = l.first, d = l.second;
-                if (active[c] && cap[c] > cap[d] + 4) { cap[c] = cap[d] + 4; changed = true; }
-                if (active[d] && cap[d] > cap[c] + 4) { cap[d] = cap[c] + 4; changed = true; }
-            }
-        } while (changed);
+        std::vector<bool> is_due(n, false);
+        for (int c : groups) is_due[c] = true;
+        // finest level that still leaves every pending endpoint of a group that is not due in place:
+        // min over such groups e of level[e] + 4 hops(c, e), through due groups only
+        std::vector<int> cap(n, FINEST);
         for (int c : groups)
         {
-            if (!(requested[c] > 0) || !std::isfinite(requested[c])) throw status_error(10, "invalid adaptive collision timestep");
-            int want = 0;
-            double h = horizon;
-            while (h > requested[c])
-            {
-                if (++want > FINEST) throw status_error(10, "adaptive collision time resolution exceeded");
-                h *= 0.5;
-            }
-            cand[c] = passed[c] ? want : std::max(want, level[c]);
-            while (tick % (uint64_t(1) << (FINEST - cand[c]))) ++cand[c];
-            cand[c] = std::min(cand[c], cap[c]);
+            const std::vector<int> d = hops_from(c, [&](int u) { return bool(is_due[u]); });
+            for (int e = 0; e < n; ++e)
+                if (!is_due[e] && d[e] != FAR) cap[c] = std::min(cap[c], level[e] + LINK_LEVELS*d[e]);
         }
-        do
+        std::vector<int> cand = level;
+        for (int c : groups)
         {
-            changed = false;
-            for (const auto &l : links)
-            {
-                int c = l.first, d = l.second;
-                if (active[c] && cand[c] < cand[d] - 4) { cand[c] = cand[d] - 4; changed = true; }
-                if (active[d] && cand[d] < cand[c] - 4) { cand[d] = cand[c] - 4; changed = true; }
-            }
-        } while (changed);
+            const int want = level_for(requested[c], "invalid adaptive collision timestep",
+                                       "adaptive collision time resolution exceeded");
+            int lv = passed[c] ? want : std::max(want, level[c]);
+            // a step aligned with the tick: 2^(52 - lv) divides it
+            if (tick != 0) lv = std::max(lv, FINEST - __builtin_ctzll(tick));
+            cand[c] = std::min(lv, cap[c]);
+        }
+        cand = raise_for_links(cand, is_due);
         for (int c : groups)
         {
             level[c] = cand[c];

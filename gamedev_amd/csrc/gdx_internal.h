@@ -337,6 +337,8 @@ struct context
 };
 
 // ---- geometry.hip ---------------------------------------------------------------------------
+// the reverse neighbour lists of a swarm with a majorant address n_active * N_K entries with 32-bit offsets
+constexpr long long MAJORANT_MAX_OWNERS = 4294967295LL / N_K;
 void scan_exclusive_u32(const unsigned int *in, unsigned int *out, int n, sort_scratch &s, hipStream_t st);
 void radix_sort_pairs(uint64_t *keys, int *vals, int n, int begin_bit, int end_bit, sort_scratch &s, hipStream_t st);
 void morton_build(morton_index &ix, const float4 *points, const unsigned char *active, int n, float ox, float oy,

@@ -1147,6 +1147,9 @@ __global__ __launch_bounds__(256) void k_rev_fill(const int *nbr, long long entr
 void build_reverse(context &c)
 {
     const int n = c.n_active;
+    if ((long long)n > MAJORANT_MAX_OWNERS)
+        throw status_error(GDX_COL_ERR_ARG, "a swarm with a majorant holds at most " + std::to_string(MAJORANT_MAX_OWNERS)
+                                            + " active owners (32-bit reverse neighbour offsets)");
     const size_t na = size_t(std::max(1, n));
     hipStream_t st = c.stream;
     c.rev_beg.alloc(na);

@@ -215,6 +215,11 @@ int gdx_col_host_finish(const double *audit15, int n, double *state4, double *re
 /* local_schedule (_col_sched.cuh:17-183) run with fixed requests and passing audits: due ticks per wave */
 int gdx_col_host_schedule(int n, double horizon, const double *requested, const unsigned int *edges, int max_waves,
                           unsigned long long *ticks, int *due_counts, int *levels, int *waves);
+/* the same with a request and an audit verdict per group and wave (wave_requested, wave_passed: max_waves x n;
+ * wave w > 0 adapts its due groups with row w), levels after every wave in wave_levels (max_waves x n) */
+int gdx_col_host_schedule_adaptive(int n, double horizon, const double *requested, const unsigned int *edges,
+                                   int max_waves, const double *wave_requested, const unsigned char *wave_passed,
+                                   unsigned long long *ticks, int *due_counts, int *wave_levels, int *waves);
 
 /* ---- stand-alone adaptive Morton index: morton_index::build / view (inc/swarm/morton/morton_index.cuh:160-293) ---- */
 typedef struct gdx_morton_ctx *gdx_morton_handle;
@@ -273,6 +278,10 @@ int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, do
  * outlive its use by h. */
 #define GDX_COL_MAJORANT_MIN_OWNERS 262144
 int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety, int min_owners);
+/* the most representatives a swarm with a majorant may hold: its reverse neighbour lists (which owners'
+ * rows hold each slot) use 32-bit offsets over n * N_K entries, so (2^32 - 1) / N_K; callable without a
+ * GPU. gdx_col_use_majorant rejects larger swarms with GDX_COL_ERR_ARG. */
+int gdx_col_majorant_max_owners(long long *out);
 /* bounds across refresh waves and calls: with reuse on (the default) an owner whose state and partner
  * records have not changed since its last evaluation keeps its bound instead of running the network
  * again (a change of the attached model's parameters invalidates every kept bound); off evaluates every

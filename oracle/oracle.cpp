@@ -1773,6 +1773,43 @@ int orc_schedule_run(int n, double horizon, const double *requested, const uint3
     catch (const std::exception &) { return -1; }
 }
 
+// local_schedule driven with a request and an audit verdict per group and wave (test access)
+int orc_schedule_adaptive(int n, double horizon, const double *requested, const uint32_t *edges, int max_waves,
+                          const double *wave_requested, const uint8_t *wave_passed, uint64_t *ticks, int *due_counts,
+                          int *wave_levels)
+{
+    try
+    {
+        std::vector<double> req(requested, requested + n);
+        std::vector<uint32_t> e(edges, edges + size_t(n)*((n + 31) / 32));
+        schedule s(horizon, req, e);
+        std::vector<bool> passed(n, true);
+        int waves = 0;
+        while (s.time() < s.end)
+        {
+            if (waves >= max_waves) return -1;
+            uint64_t tick = s.time();
+            auto groups = s.due(tick);
+            if (tick != 0)
+            {
+                for (int c = 0; c < n; c++)
+                {
+                    req[c] = wave_requested[size_t(waves)*n + c];
+                    passed[c] = wave_passed[size_t(waves)*n + c] != 0;
+                }
+                s.adapt(tick, groups, req, passed);
+            }
+            ticks[waves] = tick;
+            due_counts[waves] = int(groups.size());
+            for (int c = 0; c < n; c++) wave_levels[size_t(waves)*n + c] = s.level[c];
+            waves++;
+            s.advance(groups);
+        }
+        return waves;
+    }
+    catch (const std::exception &) { return -1; }
+}
+
 void *orc_diag_new(void) { return new diag_t(); }
 void orc_diag_free(void *d) { delete static_cast<diag_t *>(d); }
 // the controller archive's reset at output boundaries (swarm_runtime.cu:491-496)

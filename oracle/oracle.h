@@ -128,6 +128,9 @@ int orc_finish_bath(const double *audit15, int n, double *state4, double *result
 // with fixed requests and passing audits (test helper): returns waves or -1 on error.
 int orc_schedule_run(int n, double horizon, const double *requested, const uint32_t *edges,
                      int max_waves, uint64_t *ticks, int *due_counts, int *levels_out);
+int orc_schedule_adaptive(int n, double horizon, const double *requested, const uint32_t *edges, int max_waves,
+                          const double *wave_requested, const uint8_t *wave_passed, uint64_t *ticks, int *due_counts,
+                          int *wave_levels);
 
 // ---- the operator -----------------------------------------------------------------------------
 typedef struct {

@@ -325,6 +325,23 @@ class Oracle:
             raise RuntimeError("schedule rejected its inputs")
         return ticks[:w].copy(), due[:w].copy(), levels
 
+    def schedule_adaptive(self, horizon, requested, edges, wave_requested, wave_passed):
+        req = np.ascontiguousarray(requested, dtype=np.float64)
+        e = np.ascontiguousarray(edges, dtype=np.uint32)
+        wr = np.ascontiguousarray(wave_requested, dtype=np.float64)
+        wp = np.ascontiguousarray(wave_passed, dtype=np.uint8)
+        n, max_waves = len(req), wr.shape[0]
+        ticks = np.empty(max_waves, np.uint64)
+        due = np.empty(max_waves, np.int32)
+        levels = np.empty((max_waves, n), np.int32)
+        self.lib.orc_schedule_adaptive.restype = C.c_int
+        self.lib.orc_schedule_adaptive.argtypes = [C.c_int, C.c_double] + [P]*2 + [C.c_int] + [P]*5
+        w = self.lib.orc_schedule_adaptive(n, horizon, _p(req), _p(e), max_waves, _p(wr), _p(wp), _p(ticks), _p(due),
+                                           _p(levels))
+        if w < 0:
+            raise RuntimeError("schedule rejected its inputs")
+        return ticks[:w].copy(), due[:w].copy(), levels[:w].copy()
+
     # ---- operator ----
     def new_ctl(self):
         buf = np.zeros(self.lib.orc_ctl_size(), np.uint8)

@@ -1,4 +1,5 @@
 """The C-ABI libraries load without a GPU and export every entry point include/gdx_col.h declares."""
+import ctypes as C
 import re
 from pathlib import Path
 
@@ -52,3 +53,15 @@ def test_kernel_timer_ids_match_header():
            for m in re.finditer(r"#define\s+GDX_COL_KERNEL_(\w+)\s+(\d+)", text)}
     assert len(_lib.KERNELS) == n == len(ids)
     assert sorted(ids.values()) == list(range(n))
+
+
+@pytest.mark.parametrize("model,k", [("plain_2d", 16), ("fiducial_3d", 32), ("smol_const", 32)])
+def test_majorant_owner_limit(model, k):
+    """Swarms with a majorant address n * N_K reverse-neighbour entries with 32-bit offsets: the library
+    reports (and gdx_col_use_majorant enforces) n <= (2^32 - 1) / N_K."""
+    from gamedev_amd import _lib
+    lib = _lib.load(model)
+    out = C.c_longlong()
+    assert lib.gdx_col_majorant_max_owners(C.byref(out)) == 0
+    assert out.value == (2**32 - 1)//k
+    assert out.value*k < 2**32 <= (out.value + 1)*k

@@ -167,3 +167,36 @@ def test_schedule_random_tiles_horizon(oracles, lib):
                 if edges[c] >> d & 1:
                     assert abs(int(lv[c]) - int(lv[d])) <= 4
         assert ticks[0] == 0 and np.all(np.diff(ticks.astype(np.int64)) > 0)
+
+
+def test_schedule_adaptive_random_requests_and_audits(oracles, lib):
+    """Requests that change every wave and failed audits (a failed group may not coarsen) exercise the
+    alignment, the cap by pending endpoints and the link refinement: product == oracle wave by wave, the
+    16:1 links hold after every wave and the baths tile the horizon."""
+    rng = np.random.default_rng(19)
+    for trial in range(40):
+        n = int(rng.integers(2, 33))
+        req = 2.0**-rng.uniform(0, 7, n)
+        edges = np.zeros(n, np.uint32)
+        for c in range(n):
+            for d in rng.choice(n, int(rng.integers(0, 4))):
+                if d != c:
+                    edges[c] |= np.uint32(1 << int(d))
+        max_waves = 1 << 12
+        wreq = req[None, :]*2.0**rng.integers(-3, 4, size=(max_waves, n))
+        wpass = (rng.uniform(size=(max_waves, n)) < 0.7).astype(np.uint8)
+        ticks, due, lv = oracles["fiducial_3d"].schedule_adaptive(1.0, req, edges, wreq, wpass)
+        pt = np.empty(max_waves, np.uint64)
+        pd = np.empty(max_waves, np.int32)
+        pl = np.empty((max_waves, n), np.int32)
+        w = C.c_int()
+        assert lib.gdx_col_host_schedule_adaptive(n, 1.0, _p(req), _p(edges), max_waves, _p(wreq), _p(wpass),
+                                                  _p(pt), _p(pd), _p(pl), C.byref(w)) == 0
+        w = w.value
+        assert w == len(ticks) and w > 1
+        assert np.array_equal(pt[:w], ticks) and np.array_equal(pd[:w], due) and np.array_equal(pl[:w], lv)
+        for c in range(n):
+            for d in range(n):
+                if edges[c] >> d & 1:
+                    assert np.all(np.abs(lv[:, c].astype(int) - lv[:, d]) <= 4), (trial, c, d)
+        assert ticks[0] == 0 and np.all(np.diff(ticks.astype(np.int64)) > 0)
