@@ -14,7 +14,15 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
+
+// Scratch of gdx_col_majorant_grad_noalloc: one partial gradient row per wavefront for up to
+// GDX_COL_MAJORANT_GRAD_NOALLOC_MAX samples (128 samples per wavefront), then the reduced row (gradient
+// and loss sum). Every row the kernels read is written first by the same call, so nothing is cleared.
+constexpr int GRAD_NOALLOC_ROWS = GDX_COL_MAJORANT_GRAD_NOALLOC_MAX / 128;
+constexpr size_t GRAD_NOALLOC_FLOATS = size_t(GRAD_NOALLOC_ROWS + 1)*gdx::mlp::P_STRIDE;
+static __device__ __align__(256) float scratch[GRAD_NOALLOC_FLOATS];
 
 namespace gdx {
 namespace mlp {
@@ -375,6 +383,27 @@ void check_arg(bool ok, const char *what)
     if (!ok) throw gdx::status_error(GDX_COL_ERR_ARG, what);
 }
 
+// Device address of `scratch` on the current GPU. The array's name is a host-side stand-in in host code;
+// every GPU holds its own copy at its own address, looked up once per device with that device current.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[MAX_DEVICES] = {};
+
+float *scratch_on_current_device()
+{
+    int dev = -1;
+    gdx::hip_check(hipGetDevice(&dev), "current device");
+    if (dev < 0 || dev >= MAX_DEVICES) throw gdx::status_error(GDX_COL_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev])
+    {
+        void *p = nullptr;
+        gdx::hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -479,6 +508,33 @@ int gdx_col_majorant_grad(gdx_col_majorant h, const float *features, const float
                        "majorant gradient");
         std::memcpy(grad, h_grad.data(), sizeof(float)*P_COUNT);
         if (loss) *loss = double(h_grad[P_LOSS]) / n;
+    });
+}
+
+int gdx_col_majorant_grad_noalloc(gdx_col_majorant h, const float *features, const float *log_target, int n,
+                                  float *grad, double *loss)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return mguard(h, [&] {
+        using namespace gdx::mlp;
+        static_assert(16*TILES_PER_WAVE == 128 && GRAD_NOALLOC_ROWS % WAVES == 0, "scratch rows per wavefront");
+        check_arg(n > 0 && features && log_target && grad, "invalid gradient arguments");
+        check_arg(n <= GDX_COL_MAJORANT_GRAD_NOALLOC_MAX,
+                  "gradient without allocation: n exceeds GDX_COL_MAJORANT_GRAD_NOALLOC_MAX samples");
+        const int waves = grid_waves(n);
+        const int blocks = (waves + WAVES - 1) / WAVES;
+        const int rows = blocks*WAVES;
+        if (rows > GRAD_NOALLOC_ROWS) throw gdx::status_error(GDX_COL_ERR_ARG, "scratch rows exceeded");
+        float *partial = scratch_on_current_device();
+        float *sum = partial + size_t(GRAD_NOALLOC_ROWS)*P_STRIDE;
+        k_mlp_grad<<<blocks, BLOCK>>>(h->m.P.p, features, log_target, nullptr, 0, n, partial);
+        gdx::hip_check(hipGetLastError(), "majorant gradient launch");
+        k_mlp_reduce<<<(P_LOSS + 256) / 256, 256>>>(partial, rows, sum);
+        gdx::hip_check(hipGetLastError(), "majorant gradient launch");
+        float loss_sum = 0.0f;
+        gdx::hip_check(hipMemcpy(grad, sum, sizeof(float)*P_COUNT, hipMemcpyDeviceToHost), "majorant gradient");
+        gdx::hip_check(hipMemcpy(&loss_sum, sum + P_LOSS, sizeof(float), hipMemcpyDeviceToHost), "majorant gradient");
+        if (loss) *loss = double(loss_sum) / n;
     });
 }
 

@@ -264,6 +264,15 @@ int gdx_col_majorant_infer(gdx_col_majorant m, const float *features, long long 
 /* gradient (host, GDX_COL_MAJORANT_PARAMS) and value of the mean loss (ln softplus(z) - log_target)^2 */
 int gdx_col_majorant_grad(gdx_col_majorant m, const float *features, const float *log_target, int n, float *grad,
                           double *loss);
+/* No counterpart in the reference. gdx_col_majorant_grad for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same kernels
+ * and the same results, with the per-wavefront partial gradient rows and their sum kept in one fixed
+ * device array per GPU (about 9.9 MB, held until the process ends) instead of the model's buffers. n above
+ * GDX_COL_MAJORANT_GRAD_NOALLOC_MAX returns GDX_COL_ERR_ARG. Every call on a GPU uses that GPU's array:
+ * calls that overlap on one GPU (from several threads or models) race. */
+#define GDX_COL_MAJORANT_GRAD_NOALLOC_MAX 65536
+int gdx_col_majorant_grad_noalloc(gdx_col_majorant m, const float *features, const float *log_target, int n,
+                                  float *grad, double *loss);
 /* Adam training over device samples; epoch_loss (host, epochs entries, may be NULL) gets each epoch's
  * mean minibatch loss before the updates */
 int gdx_col_majorant_train(gdx_col_majorant m, const float *features, const float *log_target, long long n,
