@@ -50,6 +50,9 @@ K = 32
 #   chain_audit:  per chained owner: queue entry 4 + chain sums 36 + published and final state 24
 #   majorant:     per due owner: stale flag 1; per evaluated owner: codes 4K + FP32 partner records 8K + owner
 #                 state 16 + measure 8 + environment 40 + bound 8
+# ensembles: realizations below this many active owners keep the plain method (their serial event chains,
+# not their rate sums, set their pace)
+BATCH_MIN_OWNERS = 262_144
 PER_OWNER = {"publish": 32, "mark_stale": 1, "screen": 52}
 RATED_BYTES = 20*K + 29
 CAND_BYTES, CAND_PAIR_BYTES = 40, 20
@@ -194,7 +197,7 @@ def batch_leg(count: int, steps: int, horizon: float, dev, model=None, learned=N
            "waves_per_step_mean": float(waves.mean()), "waves_per_step_max": int(waves.max())}
     if model is not None:
         for op, _ in ops:
-            op.use_majorant(model, learned["training"]["safety"])
+            op.use_majorant(model, learned["training"]["safety"], min_owners=BATCH_MIN_OWNERS)
         el, waves, stats = run_batched(ops)
         out["updates_per_s_batched_learned"] = steps*active/el
         out["learned_waves_per_step_mean"] = float(waves.mean())
@@ -315,7 +318,7 @@ def clumped_leg(count: int, total: int, steps: int, horizon: float, dev, samples
             M_start = ensemble_moments([d for _, d in ops])
         if mode == "learned":
             for op, _ in ops:
-                op.use_majorant(model, safety)
+                op.use_majorant(model, safety, min_owners=BATCH_MIN_OWNERS)
         active = sum(op.active_count() for op, _ in ops)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

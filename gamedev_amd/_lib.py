@@ -111,6 +111,7 @@ _SIGS = {
     "gdx_col_majorant_infer": [P, P, C.c_longlong, P],
     "gdx_col_majorant_grad": [P, P, P, I, P, C.POINTER(D)],
     "gdx_col_majorant_grad_noalloc": [P, P, P, I, P, C.POINTER(D)],
+    "gdx_col_pair_rates": [P, P],
     "gdx_col_majorant_train": [P, P, P, C.c_longlong, C.POINTER(TrainConfig), P],
     "gdx_col_majorant_sample": [P, P, P, P],
     "gdx_col_use_majorant": [P, P, D, I],

@@ -189,6 +189,12 @@ class CollisionOperator:
         self._call(self.lib.gdx_col_get_cached, _np_ptr(out))
         return out
 
+    def pair_rates(self) -> np.ndarray:
+        """n x N_K bath-start pair rates (particle order, slot order); publishes the reservoir first."""
+        out = np.empty((self.n, self.info["n_k"]), dtype=np.float64)
+        self._call(self.lib.gdx_col_pair_rates, _np_ptr(out))
+        return out
+
     def rng(self) -> np.ndarray:
         out = np.empty(self.n, dtype=np.uint64)
         self._call(self.lib.gdx_col_get_rng, _np_ptr(out))
@@ -243,7 +249,7 @@ class CollisionOperator:
 
     def use_majorant(self, majorant, safety: float = 1.2, min_owners: int = 0) -> None:
         """Draw event times by thinning against the majorant (None: the plain method). Swarms with fewer
-        than min_owners active owners (0: the library default, 262,144) keep the plain method."""
+        than min_owners active owners (0: the library default, 65,536) keep the plain method."""
         self._call(self.lib.gdx_col_use_majorant, majorant.h if majorant is not None else None, float(safety),
                    int(min_owners))
         self._majorant = majorant

@@ -1220,6 +1220,22 @@ int gdx_col_rate_pass(gdx_col_handle h)
     });
 }
 
+int gdx_col_pair_rates(gdx_col_handle h, double *rates)
+{
+    if (!h || !rates) return GDX_COL_ERR_ARG;
+    int rc = gdx_col_rate_pass(h);
+    if (rc != GDX_COL_OK) return rc;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        dbuf<double> tmp;
+        tmp.alloc(size_t(c.n)*N_K);
+        hip_check(hipMemsetAsync(tmp.p, 0, sizeof(double)*size_t(c.n)*N_K, c.stream), "zero pair rates");
+        launch_pair_rates(c, make_gref(c, 0, h->rb, 0, 0), tmp.p);
+        copy_d2h(rates, tmp.p, sizeof(double)*size_t(c.n)*N_K, c.stream);
+        hip_check(hipStreamSynchronize(c.stream), "read pair rates");
+    });
+}
+
 int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, double *rate)
 {
     if (!h) return GDX_COL_ERR_ARG;

@@ -374,6 +374,7 @@ void launch_chain(hipStream_t st, const gdx_wave &w, int *work, int resident);
 void launch_chain_audit(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_event_sum(context &c, hipStream_t st);
 void launch_gather_cached(context &c, double *out_particle_order);
+void launch_pair_rates(context &c, const gdx_gref &G, double *out_particle_order);
 
 }  // namespace gdx
 

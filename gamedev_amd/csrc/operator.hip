@@ -1585,7 +1585,38 @@ __global__ __launch_bounds__(256) void k_cached_to_particles(const double4 *cach
     o[0] = c.x; o[1] = c.y; o[2] = c.z; o[3] = c.w;
 }
 
+// every pair term of every owner (test access): the rate k_bath_rate folds for slot k of owner s
+// (_get_col_chain_rate / measure x outcome factor, col_bath_rate.cu:13-129), written in particle order and
+// the reference's slot order; empty slots and owners without a measure read 0
+__global__ __launch_bounds__(256) void k_pair_rates(gdx_gref G, int n_active, double *out)
+{
+    const long long e = (long long)blockIdx.x*256 + threadIdx.x;
+    if (e >= (long long)n_active*N_K) return;
+    const int s = int(e / N_K), pos = int(e % N_K);
+    const double2 me = G.st[s];
+    const double m = G.measure[s];
+    double r = 0.0;
+    if (m > 0.0)
+    {
+        gdx_env env{};
+        double hg_i = 0.0;
+        if constexpr (COAG_KERNEL == 3) env = G.env[s];
+        if constexpr (PARTNER_HG) hg_i = G.hgs[s];
+        const owner_terms t = make_owner(env, me.x, m, hg_i);
+        r = eval_pair_rate(t, load_partner(G.nbr[e], G.res, G.hgs), G.lambda_0).rate;
+    }
+    out[(long long)G.slot_pidx[s]*N_K + bitrev_slot(pos)] = r;
+}
+
 }  // namespace
+
+void launch_pair_rates(context &c, const gdx_gref &G, double *out)
+{
+    if (c.n_active <= 0) return;
+    const long long total = (long long)c.n_active*N_K;
+    k_pair_rates<<<unsigned((total + 255) / 256), 256, 0, c.stream>>>(G, c.n_active, out);
+    hip_check(hipGetLastError(), "launch pair rates");
+}
 
 void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng)
 {

@@ -202,6 +202,11 @@ int gdx_col_set_rng(gdx_col_handle h, const unsigned long long *counters);
 int gdx_col_get_graph(gdx_col_handle h, unsigned long long *rows);
 /* bath-start rating pass only (publish + col_bath_rate for every owner); for rate parity checks */
 int gdx_col_rate_pass(gdx_col_handle h);
+/* every owner's K pair rates at the bath start, host n x N_K in particle order and slot order: the terms
+ * col_bath_rate sums (_get_col_chain_rate / measure x outcome factor, src/swarm/col_bath_rate.cu:30-58;
+ * inc/swarm/_col_rates.cuh:167-223), 0 for empty slots and inactive owners. Runs gdx_col_rate_pass
+ * first, so the reservoir is published from the owners' states. */
+int gdx_col_pair_rates(gdx_col_handle h, double *rates);
 
 /* ---- host controller and scheduler, callable without a GPU (same code the operator runs) ---- */
 /* _build_col_binmap (inc/swarm/_col_bound.cuh:45-98): returns the merged-bin count */
@@ -285,7 +290,7 @@ int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, do
  * swarm with fewer than min_owners active owners (<= 0: GDX_COL_MAJORANT_MIN_OWNERS) keeps the plain
  * method: its exact 1/16 subsample would leave the controller's audit too few owners. The model must
  * outlive its use by h. */
-#define GDX_COL_MAJORANT_MIN_OWNERS 262144
+#define GDX_COL_MAJORANT_MIN_OWNERS 65536
 int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety, int min_owners);
 /* the most representatives a swarm with a majorant may hold: its reverse neighbour lists (which owners'
  * rows hold each slot) use 32-bit offsets over n * N_K entries, so (2^32 - 1) / N_K; callable without a

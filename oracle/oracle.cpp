@@ -1861,6 +1861,24 @@ void orc_bath_rates(int n, const double *y, const double *z, const double *size,
     }
 }
 
+// every owner's K pair terms at the bath start (col_bath_rate.cu:30-58): n x N_K rates in slot order
+void orc_pair_rates(int n, const double *y, const double *z, const double *size, const double *numr,
+                    const uint8_t *active, const int *nbr, const double *measure, double lambda_0, double *rates,
+                    int threads)
+{
+    #pragma omp parallel num_threads(thread_count(threads))
+    {
+        pair_work w{std::vector<double>(N_K), std::vector<double>(N_K), std::vector<double>(N_K),
+                    std::vector<double>(N_K), std::vector<double>(N_K), {}, {}};
+        #pragma omp for schedule(static)
+        for (int i = 0; i < n; i++)
+        {
+            owner_pairs(i, size[i], y, z, nbr, measure, size, numr, lambda_0, true, active, w);
+            for (int k = 0; k < N_K; k++) rates[size_t(i)*N_K + k] = w.rate[k];
+        }
+    }
+}
+
 // evolve_local_collisions (inc/swarm/_col_chain.cuh:108-539) on the host, particle order
 int orc_operator(int n, const double *y, const double *z, double *size, double *numr, const uint8_t *active,
                  const int *spatial, const int *nbr, const double *measure, uint64_t *rng, uint64_t seed,

@@ -155,6 +155,10 @@ int orc_diag_json(void *d, int which, char *buf, int len);
 void orc_bath_rates(int n, const double *y, const double *z, const double *size, const double *numr,
                     const uint8_t *active, const int *nbr, const double *measure, double lambda_0,
                     double *cached4, int threads);
+// every owner's K pair terms at the bath start (size_old = size), n x N_K in slot order
+void orc_pair_rates(int n, const double *y, const double *z, const double *size, const double *numr,
+                    const uint8_t *active, const int *nbr, const double *measure, double lambda_0, double *rates,
+                    int threads);
 
 #ifdef __cplusplus
 }

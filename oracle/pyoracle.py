@@ -93,6 +93,7 @@ class Oracle:
         L.orc_diag_json.restype = i
         L.orc_diag_json.argtypes = [P, i, C.c_char_p, i]
         L.orc_bath_rates.argtypes = [i, P, P, P, P, P, P, P, d, P, i]
+        L.orc_pair_rates.argtypes = [i, P, P, P, P, P, P, P, d, P, i]
         info = ModelInfo()
         L.orc_get_model_info(C.byref(info))
         self.info = {f: getattr(info, f) for f, _ in info._fields_}
@@ -355,6 +356,16 @@ class Oracle:
         me = np.ascontiguousarray(measure, dtype=np.float64)
         out = np.empty((len(a[0]), 4), np.float64)
         self.lib.orc_bath_rates(len(a[0]), *[_p(v) for v in a], _p(act), _p(nb), _p(me), lambda_0, _p(out), threads)
+        return out
+
+    def pair_rates(self, y, z, size, numr, active, nbr, measure, lambda_0, threads=0):
+        """n x N_K pair terms at the bath start, slot order (0 for empty slots)."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (y, z, size, numr)]
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+        nb = np.ascontiguousarray(nbr, dtype=np.int32)
+        me = np.ascontiguousarray(measure, dtype=np.float64)
+        out = np.empty((len(a[0]), self.info["n_k"]), np.float64)
+        self.lib.orc_pair_rates(len(a[0]), *[_p(v) for v in a], _p(act), _p(nb), _p(me), lambda_0, _p(out), threads)
         return out
 
     def operator(self, y, z, size, numr, active, spatial, nbr, measure, rng, seed, duration, total_mass, ctl,

@@ -118,3 +118,26 @@ def test_bath_rates_match_oracle(built, oracles, model):
         a, b = cached[ok, f], ref[ok, f]
         scale = np.maximum(np.abs(b), 1e-300)
         assert np.max(np.abs(a - b) / scale) <= 1e-12, f"field {f}"
+
+
+@pytest.mark.parametrize("model", ["plain_2d", "fiducial_3d"])
+def test_pair_rates_match_oracle(built, oracles, model):
+    """Every owner's K pair rates (gdx_col_pair_rates) against the oracle's pair terms, 1e-12 relative per
+    pair; empty slots and inactive owners are exactly zero on both sides."""
+    op, dev, st = built[model]
+    o = oracles[model]
+    rates = op.pair_rates()
+    pts, act, _ = op.sites()
+    codes, meas = op.neighbors()
+    lam = len(st["x"]) / o.info["n_k"] / st["total_mass"]
+    ref = o.pair_rates(st["y"], st["z"], st["size"], st["numr"], act, codes, meas, lam)
+    assert np.array_equal(rates == 0.0, ref == 0.0)
+    nz = ref != 0.0
+    assert nz.sum() > 0.5*act.sum()*o.info["n_k"]
+    rel = np.abs(rates[nz] - ref[nz])/np.abs(ref[nz])
+    print(f"[pair rates] {model}: {nz.sum()} nonzero pairs, max rel {rel.max():.2e}", flush=True)
+    assert rel.max() <= 1e-12
+    # the owner totals of the rate pass are the sums of these terms
+    tot = op.cached()[act == 1, 0]
+    s = rates[act == 1].sum(axis=1)
+    assert np.allclose(tot, s, rtol=1e-13, atol=0.0)
