@@ -429,7 +429,7 @@ def kernel_table(kt: dict, tot: dict, steps: int) -> dict:
         b = kernel_bytes(name, rec, tot)
         avg = rec["ms"]/rec["launches"]
         gbs = b/rec["launches"]/(avg*1e-3)/1e9 if avg > 0 else 0.0
-        out[name] = {"ms_per_step": rec["ms"]/steps, "avg_launch_ms": avg, "launches": rec["launches"],
+        out[name] = {"ms_per_step": rec["ms"]/max(1, steps), "avg_launch_ms": avg, "launches": rec["launches"],
                      "algorithmic_GBps": gbs, "hbm_frac": gbs/HBM_PEAK_GBS, "limit": LIMIT.get(name, "")}
     return out
 
@@ -510,6 +510,9 @@ def main() -> int:
     if args.autotune:
         tune = op.autotune((16, 32, 64, 128, 256), repeats=1)
         log(f"autotuned Morton leaf target {tune['best']}: " + ", ".join(f"{k}: {v:.1f} ms" for k, v in tune["ms"].items()))
+        tile = op.autotune_tile((0, 8, 16, 32), repeats=1)
+        log(f"autotuned top-K tile {tile['best']}: " + ", ".join(f"{k}: {v:.1f} ms" for k, v in tile["ms"].items()))
+        tune = {"best": tune["best"], "ms": tune["ms"], "tile": tile}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     op.geometry()
@@ -555,7 +558,7 @@ def main() -> int:
                    "exact_pair_evals_per_update": l_tot["pair_evals"]/max(1, l_updates),
                    "events_per_s": l_tot["events"]/l_elapsed, "waves_per_step": l_tot["waves"]/args.steps,
                    "longest_chain_events": l_tot["longest_chain"],
-                   "kernel_ms_per_step": {k: v["ms"]/lp_steps for k, v in l_kt.items()},
+                   "kernel_ms_per_step": {k: v["ms"]/max(1, lp_steps) for k, v in l_kt.items()},
                    "kernels": kernel_table(l_kt, lp_tot, lp_steps),
                    "roofline": roofline_of(l_kt, lp_tot), "totals": l_tot}
         op.use_majorant(None)
@@ -583,7 +586,7 @@ def main() -> int:
              "mean_baths_per_owner_step": tot["owner_updates"]/max(1, updates),
              "waves_per_step": tot["waves"]/args.steps, "chain_owners_per_step": tot["chain_owners"]/args.steps,
              "longest_chain_events": tot["longest_chain"],
-             "kernel_ms_per_step": {k: v["ms"]/p_steps for k, v in kt.items()},
+             "kernel_ms_per_step": {k: v["ms"]/max(1, p_steps) for k, v in kt.items()},
              "kernels": kernel_table(kt, p_tot, p_steps)}
     workload = (f"fiducial 3D disk, single swarm of {args.n} representatives, K={K}, coagulation+fragmentation, "
                 f"h_op={args.horizon}, operator-only with the state resident in HBM")

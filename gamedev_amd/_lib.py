@@ -122,6 +122,8 @@ _SIGS = {
     "gdx_col_host_schedule_adaptive": [I, D, P, P, I, P, P, P, P, P, P],
     "gdx_col_set_profile": [P, I],
     "gdx_col_autotune": [P, P, I, I, P, P],
+    "gdx_col_set_topk_tile": [P, I],
+    "gdx_col_autotune_tile": [P, P, I, I, P, P],
 }
 EXPORTS = tuple(_SIGS) + ("gdx_col_last_error", "gdx_col_majorant_last_error")
 

@@ -52,6 +52,21 @@ def build_check(model: str) -> Path:
     return out
 
 
+def topk_check_path() -> Path:
+    return LIB / "topk_check"
+
+
+def build_topk_check() -> Path:
+    """Host-only build of the top-K merge check program (model independent)."""
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = topk_check_path()
+    tmp = out.with_suffix(".tmp")
+    _run([HIPCC, "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", "-Wall", f"-I{CSRC}",
+          str(CSRC / "topk_check.cpp"), "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def _run(cmd: list[str]) -> None:
     proc = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (proc.stdout + proc.stderr).splitlines() if "packed-fp32-ops" not in l)
@@ -77,7 +92,7 @@ def build(targets: list[str] | None = None, jobs: int | None = None) -> list[Pat
     jobs = jobs or max(1, min(len(targets)*len(SOURCES), os.cpu_count() or 4, 8))
     with cf.ThreadPoolExecutor(jobs) as ex:
         futs = {(m, s): ex.submit(_compile, m, s) for m in targets for s in SOURCES}
-        checks = [ex.submit(build_check, m) for m in targets]
+        checks = [ex.submit(build_check, m) for m in targets] + [ex.submit(build_topk_check)]
         objs = {k: f.result() for k, f in futs.items()}
         for f in checks:
             f.result()

@@ -235,6 +235,18 @@ class CollisionOperator:
         self._call(self.lib.gdx_col_autotune, _np_ptr(c), len(c), int(repeats), C.byref(best), _np_ptr(ms))
         return {"best": best.value, "ms": {int(k): float(v) for k, v in zip(c, ms)}}
 
+    def set_topk_tile(self, tile: int) -> None:
+        """Top-K selection of the Morton search: 0 register insertion, 8/16/32 LDS-tiled merges."""
+        self._call(self.lib.gdx_col_set_topk_tile, int(tile))
+
+    def autotune_tile(self, candidates=(0, 8, 16, 32), repeats: int = 2) -> dict:
+        """Time the geometry epoch per top-K tile at the current leaf target, keep the fastest."""
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c), np.float64)
+        best = C.c_int()
+        self._call(self.lib.gdx_col_autotune_tile, _np_ptr(c), len(c), int(repeats), C.byref(best), _np_ptr(ms))
+        return {"best": best.value, "ms": {int(k): float(v) for k, v in zip(c, ms)}}
+
     # ---- learned majorant ----
     def majorant_sample(self):
         """Features (n x 8, float32), partner scale S and exact bath-start rate R per particle at the

@@ -1300,6 +1300,54 @@ int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int rep
     });
 }
 
+int gdx_col_set_topk_tile(gdx_col_handle h, int tile)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(tile == 0 || tile == 8 || tile == 16 || tile == 32, GDX_COL_ERR_ARG, "top-K tile must be 0, 8, 16 or 32");
+        h->c.topk_tile = tile;
+        h->c.geometry_valid = h->c.state_loaded = false;
+    });
+}
+
+int gdx_col_autotune_tile(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        context &c = h->c;
+        require(candidates && count > 0 && repeats > 0, GDX_COL_ERR_ARG, "invalid autotuning arguments");
+        require(c.user.bound, GDX_COL_ERR_STATE, "no particle state is bound");
+        for (int i = 0; i < count; ++i)
+            require(candidates[i] == 0 || candidates[i] == 8 || candidates[i] == 16 || candidates[i] == 32,
+                    GDX_COL_ERR_ARG, "top-K tile must be 0, 8, 16 or 32");
+        if (c.state_loaded) store_state(c);
+        hip_check(hipStreamSynchronize(c.stream), "autotune");
+        int pick = c.topk_tile;
+        double fastest = 0.0;
+        for (int i = 0; i < count; ++i)
+        {
+            c.topk_tile = candidates[i];
+            geometry_epoch(c);
+            double t = 0.0;
+            for (int r = 0; r < repeats; ++r)
+            {
+                hip_check(hipStreamSynchronize(c.stream), "autotune");
+                auto t0 = std::chrono::steady_clock::now();
+                geometry_epoch(c);
+                hip_check(hipStreamSynchronize(c.stream), "autotune");
+                double v = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                t = r ? std::min(t, v) : v;
+            }
+            if (ms) ms[i] = t;
+            if (i == 0 || t < fastest) { fastest = t; pick = candidates[i]; }
+        }
+        c.topk_tile = pick;
+        geometry_epoch(c);
+        hip_check(hipStreamSynchronize(c.stream), "autotune");
+        if (best) *best = pick;
+    });
+}
+
 int gdx_col_set_profile(gdx_col_handle h, int on)
 {
     if (!h) return GDX_COL_ERR_ARG;

@@ -9,6 +9,7 @@
 // among active records with d^2 <= q^2, ascending, where code = 3*particle index.
 #include "gdx_internal.h"
 #include "gdx_device.h"
+#include "gdx_topk.h"
 
 #include <algorithm>
 #include <climits>
@@ -499,14 +500,16 @@ __device__ __forceinline__ void topk_insert_dedup(float (&d)[K], int (&c)[K], fl
 __device__ __forceinline__ float readlane_f(float v, int l)
 { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 
-template <int K, bool DEDUP = false>
-__device__ void topk_wave(const float4 *rec, const gdx_node *nodes, int dim, int max_level, float qx, float qy,
-                          float qz, float q2, bool qactive, float cx, float cy, float cz, float (&d)[K], int (&c)[K],
-                          int *stack, unsigned int *overflow)
+// Traversal shared by both selection schemes: the wavefront pops nodes from one LDS stack, children
+// nearest-first from the wavefront's centre, and pushes only the nodes some lane needs; every needed leaf
+// goes to leaf(begin, end, need) with need the lane's own test. cut() is the lane's current pruning
+// distance (the smaller of its cap and its K-th candidate so far).
+template <class Cut, class Leaf>
+__device__ __forceinline__ void traverse(const gdx_node *nodes, int dim, int max_level, float qx, float qy, float qz,
+                                         bool qactive, float cx, float cy, float cz, int *stack,
+                                         unsigned int *overflow, Cut cut_of, Leaf leaf)
 {
     const int lane = threadIdx.x % WAVE;
-#pragma unroll
-    for (int k = 0; k < K; ++k) { d[k] = __builtin_inff(); c[k] = INT_MAX; }
     if (__ballot(qactive) == 0ull) return;
     int sp = 1;
     if (lane == 0) stack[0] = 0;
@@ -516,38 +519,17 @@ __device__ void topk_wave(const float4 *rec, const gdx_node *nodes, int dim, int
         --sp;
         int id = __builtin_amdgcn_readfirstlane(stack[sp]);
         gdx_node nd = nodes[id];
-        float cut = fminf(q2, d[K - 1]);
+        float cut = cut_of();
         bool need = qactive && node_d2(qx, qy, qz, nd, dim, max_level) <= cut;
         if (__ballot(need) == 0ull) continue;
         if (nd.child_count == 0)
         {
-            const int end = nd.begin + nd.count;
-            for (int t0 = nd.begin; t0 < end; t0 += WAVE)
-            {
-                int idx = t0 + lane;
-                float4 v = idx < end ? rec[idx] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-                int cnt = min(WAVE, end - t0);
-                for (int j = 0; j < cnt; ++j)
-                {
-                    float px = readlane_f(v.x, j), py = readlane_f(v.y, j), pz = readlane_f(v.z, j);
-                    int code = __builtin_amdgcn_readlane(__float_as_int(v.w), j);
-                    float dx = qx - px, dy = qy - py, dz = qz - pz;
-                    float d2 = dx*dx + dy*dy + dz*dz;
-                    bool e = need && code >= 0 && d2 <= q2 && nb_less(d2, code, d[K - 1], c[K - 1]);
-                    if (e)
-                    {
-                        if constexpr (DEDUP) topk_insert_dedup<K>(d, c, d2, code);
-                        else topk_insert<K>(d, c, d2, code);
-                    }
-                }
-            }
+            leaf(nd.begin, nd.begin + nd.count, need);
             continue;
         }
-        // children nearest-first from the wavefront's centre; push only those some lane needs
         int ids[8];
         float key[8];
         int nv = 0;
-        cut = fminf(q2, d[K - 1]);
         const int nch = nd.child_count & 15;
         for (int ch = 0; ch < nch; ++ch)
         {
@@ -576,6 +558,109 @@ __device__ void topk_wave(const float4 *rec, const gdx_node *nodes, int dim, int
     }
 }
 
+// insertion scheme: every eligible record goes through the lane's sorted K-entry register list
+template <int K, bool DEDUP = false>
+__device__ void topk_wave(const float4 *rec, const gdx_node *nodes, int dim, int max_level, float qx, float qy,
+                          float qz, float q2, bool qactive, float cx, float cy, float cz, float (&d)[K], int (&c)[K],
+                          int *stack, unsigned int *overflow)
+{
+    const int lane = threadIdx.x % WAVE;
+#pragma unroll
+    for (int k = 0; k < K; ++k) { d[k] = __builtin_inff(); c[k] = INT_MAX; }
+    traverse(nodes, dim, max_level, qx, qy, qz, qactive, cx, cy, cz, stack, overflow,
+             [&] { return fminf(q2, d[K - 1]); },
+             [&](int begin, int end, bool need) {
+                 for (int t0 = begin; t0 < end; t0 += WAVE)
+                 {
+                     int idx = t0 + lane;
+                     float4 v = idx < end ? rec[idx] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                     int cnt = min(WAVE, end - t0);
+                     for (int j = 0; j < cnt; ++j)
+                     {
+                         float px = readlane_f(v.x, j), py = readlane_f(v.y, j), pz = readlane_f(v.z, j);
+                         int code = __builtin_amdgcn_readlane(__float_as_int(v.w), j);
+                         float dx = qx - px, dy = qy - py, dz = qz - pz;
+                         float d2 = dx*dx + dy*dy + dz*dz;
+                         bool e = need && code >= 0 && d2 <= q2 && nb_less(d2, code, d[K - 1], c[K - 1]);
+                         if (e)
+                         {
+                             if constexpr (DEDUP) topk_insert_dedup<K>(d, c, d2, code);
+                             else topk_insert<K>(d, c, d2, code);
+                         }
+                     }
+                 }
+             });
+}
+
+// ---- tiled scheme: candidates collected per lane in LDS, merged B at a time (gdx_topk.h) ----
+// the lane's `cnt` buffered keys buf[r * WAVE] merged into L
+template <int K, int B>
+__device__ __forceinline__ void topk_flush(uint64_t (&L)[K], const uint64_t *buf, int cnt)
+{
+    uint64_t S[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+    {
+        const uint64_t v = buf[r*WAVE];
+        S[r] = r < cnt ? v : TOPK_EMPTY;
+    }
+    bitonic_sort<B>(S);
+    topk_merge_sorted<K, B>(L, S);
+}
+
+// tiled scheme: a record passes the lane's filter when it lies within the cap and precedes the lane's
+// K-th key as of its last merge (a stale, hence larger, cutoff: the filter keeps a superset of the
+// records the exact list would take); passing keys are appended to the lane's column of an LDS buffer
+// and the wavefront merges every lane's buffer once some lane holds B. The K smallest keys of a union do
+// not depend on the order of merging, so the result is the insertion scheme's, bit for bit.
+template <int K, int B>
+__device__ void topk_wave_tiled(const float4 *rec, const gdx_node *nodes, int dim, int max_level, float qx, float qy,
+                                float qz, float q2, bool qactive, float cx, float cy, float cz, float (&d)[K],
+                                int (&c)[K], int *stack, uint64_t *buf, unsigned int *overflow)
+{
+    const int lane = threadIdx.x % WAVE;
+    uint64_t L[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) L[k] = TOPK_EMPTY;
+    uint64_t *col = buf + lane;
+    int cnt = 0;
+    traverse(nodes, dim, max_level, qx, qy, qz, qactive, cx, cy, cz, stack, overflow,
+             [&] { return fminf(q2, __uint_as_float(uint32_t(L[K - 1] >> 32))); },
+             [&](int begin, int end, bool need) {
+                 for (int t0 = begin; t0 < end; t0 += WAVE)
+                 {
+                     int idx = t0 + lane;
+                     float4 v = idx < end ? rec[idx] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                     int n = min(WAVE, end - t0);
+                     for (int j = 0; j < n; ++j)
+                     {
+                         float px = readlane_f(v.x, j), py = readlane_f(v.y, j), pz = readlane_f(v.z, j);
+                         int code = __builtin_amdgcn_readlane(__float_as_int(v.w), j);
+                         float dx = qx - px, dy = qy - py, dz = qz - pz;
+                         float d2 = dx*dx + dy*dy + dz*dz;
+                         const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(code);
+                         if (need && code >= 0 && d2 <= q2 && key < L[K - 1])
+                         {
+                             col[cnt*WAVE] = key;
+                             ++cnt;
+                         }
+                         if (__ballot(cnt == B))
+                         {
+                             topk_flush<K, B>(L, col, cnt);
+                             cnt = 0;
+                         }
+                     }
+                 }
+             });
+    if (__ballot(cnt > 0)) topk_flush<K, B>(L, col, cnt);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        d[k] = __uint_as_float(uint32_t(L[k] >> 32));
+        c[k] = int(uint32_t(L[k]));
+    }
+}
+
 __device__ __forceinline__ void wave_centre(float qx, float qy, float qz, bool qactive, float &cx, float &cy, float &cz)
 {
     float lo[3] = {qactive ? qx : __builtin_inff(), qactive ? qy : __builtin_inff(), qactive ? qz : __builtin_inff()};
@@ -597,7 +682,8 @@ __device__ __forceinline__ void wave_centre(float qx, float qy, float qz, bool q
 
 // col_cache_get (inc/swarm/_col_cache.cuh:73-159) over Morton-ordered queries
 // one query per physical record; DEDUP when two images of one particle can share a search ball
-template <bool DEDUP>
+// TILE = 0: the insertion scheme; TILE = B > 0: the tiled scheme with B-key merges (no image dedup)
+template <bool DEDUP, int TILE>
 __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, const gdx_node *nodes, int n, int dim,
                                                         int max_level, const int *sidx, const float4 *pts,
                                                         const int *pidx_slot, const unsigned char *group,
@@ -605,7 +691,9 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
                                                         unsigned long long *graph, unsigned int *overflow,
                                                         unsigned char *nbr_image)
 {
+    static_assert(!(DEDUP && TILE), "image dedup runs on the insertion scheme");
     __shared__ int stacks[TOPK_BLOCK / WAVE][TOPK_STACK];
+    __shared__ uint64_t tiles[TOPK_BLOCK / WAVE][TILE > 0 ? TILE : 1][WAVE];
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     long long m = ((long long)xcd_block()*(TOPK_BLOCK / WAVE) + wave)*WAVE + lane;
     bool valid = m < n;
@@ -619,8 +707,12 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, cons
     wave_centre(me.x, me.y, me.z, qactive, cx, cy, cz);
     float d[N_K];
     int c[N_K];
-    topk_wave<N_K, DEDUP>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, d, c, stacks[wave],
-                          overflow);
+    if constexpr (TILE > 0)
+        topk_wave_tiled<N_K, TILE>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, d, c,
+                                   stacks[wave], &tiles[wave][0][0], overflow);
+    else
+        topk_wave<N_K, DEDUP>(rec, nodes, dim, max_level, me.x, me.y, me.z, q2, qactive, cx, cy, cz, d, c,
+                              stacks[wave], overflow);
     unsigned long long mask = 0ull;
     int g = qactive ? int(group[p]) : -1;
     if (qactive)
@@ -1057,14 +1149,21 @@ void geometry_epoch(context &c)
         int waves = blocks_for(nr, WAVE);
         dim3 grid = xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE));
         if (X_WEDGE) c.nbr_image.alloc(na*N_K);
-        if (c.unique_ids)
-            k_col_topk<false><<<grid, TOPK_BLOCK, 0, st>>>(
-                c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
-                c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p);
+#define GDX_TOPK_LAUNCH(DEDUP, TILE)                                                                         \
+    k_col_topk<DEDUP, TILE><<<grid, TOPK_BLOCK, 0, st>>>(                                                     \
+        c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,     \
+        c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p)
+        if (!c.unique_ids && X_WEDGE)
+            GDX_TOPK_LAUNCH(bool(X_WEDGE), 0);
+        else if (c.topk_tile == 8)
+            GDX_TOPK_LAUNCH(false, 8);
+        else if (c.topk_tile == 16)
+            GDX_TOPK_LAUNCH(false, 16);
+        else if (c.topk_tile == 32)
+            GDX_TOPK_LAUNCH(false, 32);
         else
-            k_col_topk<X_WEDGE><<<grid, TOPK_BLOCK, 0, st>>>(
-                c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,
-                c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p);
+            GDX_TOPK_LAUNCH(false, 0);
+#undef GDX_TOPK_LAUNCH
         hip_check(hipGetLastError(), "launch top-K");
     }
     unsigned int ovf = 0;

@@ -135,6 +135,14 @@ int gdx_col_set_profile(gdx_col_handle h, int on);
  * around a synchronised epoch), keeps the fastest, and leaves a valid epoch built with it. A deferred
  * state is written back first. ms (may be NULL) gets `count` times. */
 int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms);
+/* top-K selection of the Morton search, no reference counterpart (the reference sorts each 64-record tile
+ * with a block-wide bitonic merge, morton_index.cuh:353-455): 0 inserts every eligible record into the
+ * query lane's sorted register list; 8, 16 or 32 collects eligible records per lane in LDS and merges them
+ * that many at a time by register sorting networks. Rows are identical for every choice; only the epoch's
+ * time differs. The autotuner times the epoch at each candidate tile (at the current leaf target) and
+ * keeps the fastest. */
+int gdx_col_set_topk_tile(gdx_col_handle h, int tile);
+int gdx_col_autotune_tile(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms);
 
 /* one operator call over `duration`: evolve_local_collisions (inc/swarm/_col_chain.cuh:108-539) */
 int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);

@@ -1,18 +1,21 @@
-"""The geometry autotuner: the Morton leaf target changes only the search time. Neighbour rows and a
-following operator call are bit-identical for every leaf target; the tuner keeps the fastest."""
+"""The geometry autotuner: the Morton leaf target and the top-K merge tile change only the search time.
+Neighbour rows and a following operator call are bit-identical for every leaf target and tile; the tuner
+keeps the fastest."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def _run(st, gpu, leaf):
+def _run(st, gpu, leaf, tile=None, model="fiducial_3d"):
     from gamedev_amd import CollisionOperator, synth
     d = synth.to_device(st, gpu)
-    op = CollisionOperator("fiducial_3d", len(st["x"]), st["total_mass"], seed=3)
+    op = CollisionOperator(model, len(st["x"]), st["total_mass"], seed=3)
     op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
     if leaf is not None:
         op.set_leaf_target(leaf)
+    if tile is not None:
+        op.set_topk_tile(tile)
     op.geometry()
     codes, meas = op.neighbors()
     s = op.evolve(0.01)
@@ -52,3 +55,36 @@ def test_autotune_keeps_the_fastest_and_a_valid_epoch(gpu):
     op.evolve(0.01)
     with pytest.raises(GdxError):
         op.autotune((0,), repeats=1)
+
+
+@pytest.mark.parametrize("model", ["fiducial_3d", "plain_2d"])
+def test_topk_tile_does_not_change_results(gpu, model):
+    """Insertion (tile 0) and LDS-tiled merges of 8, 16 and 32 keys give the same rows, measures and
+    operator results; leaf targets 8 and 300 with tile 8 stress many small and few large leaves."""
+    from gamedev_amd import synth
+    st = synth.fiducial_3d_state(30000, seed=18) if model == "fiducial_3d" else synth.plain_2d_state(30000, seed=18)
+    ref = _run(st, gpu, None, 0, model)
+    for leaf, tile in ((None, 8), (None, 16), (None, 32), (8, 8), (300, 32)):
+        print(f"[autotune] {model} leaf {leaf} tile {tile}", flush=True)
+        r = _run(st, gpu, leaf, tile, model)
+        for a, b in zip(r[:5], ref[:5]):
+            np.testing.assert_array_equal(a, b)
+        assert r[5]["events"] == ref[5]["events"] and r[5]["waves"] == ref[5]["waves"]
+
+
+def test_autotune_tile_keeps_the_fastest(gpu):
+    from gamedev_amd import CollisionOperator, GdxError, synth
+    st = synth.fiducial_3d_state(50000, seed=19)
+    d = synth.to_device(st, gpu)
+    op = CollisionOperator("fiducial_3d", 50000, st["total_mass"], seed=3)
+    op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
+    op.geometry()
+    codes0, _ = op.neighbors()
+    res = op.autotune_tile((0, 8, 16, 32), repeats=2)
+    print("[autotune] tile", res, flush=True)
+    assert res["best"] in (0, 8, 16, 32)
+    assert res["ms"][res["best"]] == min(res["ms"].values())
+    codes1, _ = op.neighbors()
+    np.testing.assert_array_equal(codes0, codes1)
+    with pytest.raises(GdxError):
+        op.set_topk_tile(12)
