@@ -50,9 +50,8 @@ K = 32
 #   chain_audit:  per chained owner: queue entry 4 + chain sums 36 + published and final state 24
 #   majorant:     per due owner: stale flag 1; per evaluated owner: codes 4K + FP32 partner records 8K + owner
 #                 state 16 + measure 8 + environment 40 + bound 8
-# ensembles: realizations below this many active owners keep the plain method (their serial event chains,
-# not their rate sums, set their pace)
-BATCH_MIN_OWNERS = 262_144
+# concurrent streams (and host threads) of the ensemble legs: one per hardware queue of the process
+ENSEMBLE_STREAMS = 4
 PER_OWNER = {"publish": 32, "mark_stale": 1, "screen": 52}
 RATED_BYTES = 20*K + 29
 CAND_BYTES, CAND_PAIR_BYTES = 40, 20
@@ -140,72 +139,106 @@ def cpu_baseline(n_sample: int, steps: int, horizon: float, threads: int) -> dic
                       f"h_op={horizon}, operator-only (geometry epoch {geo_s:.2f} s excluded)"}
 
 
-def batch_leg(count: int, steps: int, horizon: float, dev, model=None, learned=None) -> dict:
+def batch_leg(model_dir: str, count: int, steps: int, dev, samples: int, epochs: int) -> dict:
     """Ensemble of independent realizations of log-uniform size (floor(2^U[14,20]) representatives,
-    seeds 0..count-1), each with its own lambda_0, controller and random key: one batched operator call
-    per step (shared launches) against the same realizations advanced one call at a time; with a trained
-    majorant, the batched call again with every realization thinning against it."""
+    seeds 1000..), each with its own lambda_0, controller and random key. Timed three ways over the same
+    initial states: one batched call per step (shared launches, one stream), the ensemble split over four
+    concurrent streams (gdx_col_operator_ensemble), and one call per realization at a time; then the
+    four-stream ensemble with a majorant trained on the ensemble's own states. model_dir ensemble_2d is
+    the C1 distribution with K = 32 (constant kernel: rates grow with the realization's size; the horizon
+    gives the largest realization 5% events per step); fiducial_3d the 3D physical disk (rates fall with
+    size: the smallest realizations carry the events)."""
     import numpy as np
     import torch
-    from gamedev_amd import CollisionOperator, evolve_batch, synth
+    from gamedev_amd import CollisionOperator, Majorant, evolve_batch, evolve_ensemble, synth
 
-    rng = np.random.default_rng(12345)
-    sizes = [int(2.0**rng.uniform(14, 20)) for _ in range(count)]
+    sizes = synth.ensemble_sizes(count, 14, 20, seed=12345)
+    if model_dir == "ensemble_2d":
+        make_state = lambda r, n: synth.plain_2d_state(n, seed=1000 + r)
+    else:
+        make_state = lambda r, n: synth.fiducial_3d_state(n, seed=1000 + r)
 
     def build():
         ops = []
         for r, n in enumerate(sizes):
-            st = synth.fiducial_3d_state(n, seed=1000 + r)
+            st = make_state(r, n)
             d = synth.to_device(st, dev)
-            op = CollisionOperator("fiducial_3d", n, st["total_mass"], seed=r, defer_writeback=True)
+            op = CollisionOperator(model_dir, n, st["total_mass"], seed=r, defer_writeback=True)
             op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
             op.geometry()
             ops.append((op, d))
         return ops
 
-    def run_batched(ops):
-        evolve_batch([op for op, _ in ops], horizon)                  # warmup
+    ops = build()
+    if model_dir == "ensemble_2d":
+        big = ops[int(np.argmax(sizes))][0]
+        big.rate_pass()
+        _, act, _ = big.sites()
+        horizon = 0.05/float(big.cached()[act == 1, 0].mean())
+    else:
+        horizon = 0.05
+    active = sum(op.active_count() for op, _ in ops)
+
+    def timed(ops, call):
+        """call(operators, steps) -> stats per step; one warmup step, then `steps` timed"""
+        call([op for op, _ in ops], 1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        waves, stats = [], []
-        for _ in range(steps):
-            out = evolve_batch([op for op, _ in ops], horizon)
-            waves.append([s["waves"] for s in out])
-            stats.extend(out)
+        per_step = call([op for op, _ in ops], steps)
         torch.cuda.synchronize()
-        return time.perf_counter() - t0, np.array(waves), stats
+        el = time.perf_counter() - t0
+        waves = np.array([[s["waves"] for s in out] for out in per_step])
+        return steps*active/el, waves, [s for out in per_step for s in out]
 
-    log(f"batch leg: {count} realizations, {sum(sizes)} representatives")
+    batch_steps = lambda o, k: [evolve_batch(o, horizon) for _ in range(k)]
+    stream_steps = lambda o, k: evolve_ensemble(o, horizon, k, ENSEMBLE_STREAMS)
+    one_at_a_time = lambda o, k: [[op.evolve(horizon) for op in o] for _ in range(k)]
+
+    log(f"batch leg ({model_dir}): {count} realizations, {sum(sizes)} representatives, horizon {horizon:.4g}")
+    batched, waves, _ = timed(ops, batch_steps)
+    del ops
     ops = build()
-    active = sum(op.active_count() for op, _ in ops)
-    el, waves, _ = run_batched(ops)
-    batched = steps*active/el
+    streams, _, st_e = timed(ops, stream_steps)
     # the same realizations advanced one operator call at a time
-    for op, _ in ops:
-        op.evolve(horizon)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for op, _ in ops:
-            op.evolve(horizon)
-    torch.cuda.synchronize()
-    sequential = steps*active/(time.perf_counter() - t0)
-    log(f"batch leg: {batched:.3e} updates/s batched, {sequential:.3e} one call at a time")
-    out = {"realizations": count, "representatives": sum(sizes), "active_owners": active, "steps": steps,
-           "updates_per_s_batched": batched, "updates_per_s_one_call_at_a_time": sequential,
-           "batched_over_sequential": batched/sequential,
+    sequential, _, _ = timed(ops, one_at_a_time)
+    log(f"batch leg ({model_dir}): {batched:.3e} updates/s batched, {streams:.3e} on {ENSEMBLE_STREAMS} streams, "
+        f"{sequential:.3e} one call at a time")
+    out = {"model": model_dir, "realizations": count, "representatives": sum(sizes), "active_owners": active,
+           "steps": steps, "horizon": horizon, "updates_per_s_batched": batched,
+           "updates_per_s_ensemble_streams": streams, "ensemble_streams": ENSEMBLE_STREAMS,
+           "updates_per_s_one_call_at_a_time": sequential, "batched_over_sequential": batched/sequential,
+           "events_per_step": sum(s["events"] for s in st_e)/steps,
            "waves_per_step_mean": float(waves.mean()), "waves_per_step_max": int(waves.max())}
-    if model is not None:
-        for op, _ in ops:
-            op.use_majorant(model, learned["training"]["safety"], min_owners=BATCH_MIN_OWNERS)
-        el, waves, stats = run_batched(ops)
-        out["updates_per_s_batched_learned"] = steps*active/el
-        out["learned_waves_per_step_mean"] = float(waves.mean())
-        out["learned_waves_per_step_max"] = int(waves.max())
-        out["learned_candidates"] = int(sum(s["candidates"] for s in stats))
-        chk = sum(s["exact_checks"] for s in stats)
-        out["learned_violation_rate"] = sum(s["violations"] for s in stats)/max(1, chk)
-        log(f"batch leg: {out['updates_per_s_batched_learned']:.3e} updates/s batched with the majorant")
+    # a majorant trained on the four largest realizations' states (two plain steps each)
+    order = np.argsort(sizes)[::-1][:4]
+    X, T = [], []
+    per = max(1, samples//8)
+    for _ in range(2):
+        for r in order:
+            f, S, R = ops[r][0].majorant_sample()
+            ok = torch.nonzero((S > 0) & (R > 0)).squeeze(1)
+            pick = ok[torch.randperm(len(ok), device=dev)[:per]]
+            X.append(f[pick].clone())
+            T.append(torch.log(R[pick]/S[pick]).float())
+        evolve_ensemble([ops[r][0] for r in order], horizon, 1, ENSEMBLE_STREAMS)
+    model = Majorant(model_dir, seed=1)
+    losses, safety = model.fit(torch.cat(X), torch.cat(T), epochs=epochs, batch=8192, lr=1e-3, seed=1)
+    del ops, X, T
+    ops = build()
+    for op, _ in ops:
+        op.use_majorant(model, safety)
+    learned, lwaves, stats = timed(ops, stream_steps)
+    chk = sum(s["exact_checks"] for s in stats)
+    cand = sum(s["candidates"] for s in stats)
+    out.update({"updates_per_s_ensemble_learned": learned, "learned_over_plain_ensemble": learned/streams,
+                "learned_realizations": int(sum(1 for op, _ in ops if op.active_count() >= 65536)),
+                "learned_safety": safety, "learned_waves_per_step_mean": float(lwaves.mean()),
+                "learned_candidates": int(cand),
+                "learned_violation_rate": sum(s["violations"] for s in stats)/max(1, chk),
+                "learned_acceptance_rate": sum(s["accepted"] for s in stats)/max(1, cand)})
+    log(f"batch leg ({model_dir}): {learned:.3e} updates/s on {ENSEMBLE_STREAMS} streams with the majorant")
+    del ops
+    torch.cuda.empty_cache()
     return out
 
 
@@ -269,12 +302,12 @@ def wedge_leg(n: int, steps: int, dev) -> dict:
 def clumped_leg(count: int, total: int, steps: int, horizon: float, dev, samples: int, epochs: int) -> dict:
     """The clumped stress case, batched: `count` realizations totalling `total` representatives (sizes
     floor(2^U[16,20]) rescaled), vortex and ring overdensities, sigma_ln = 1.5. The majorant is trained on
-    clumped states (10 plain steps of eight extra realizations), then the plain and the learned batched
-    operators advance identical copies of the ensemble; reports both rates, the violation and acceptance
-    rates and the ensemble moments' relative difference."""
+    clumped states (10 plain steps of eight extra realizations), then the plain and the learned operators
+    advance identical copies of the ensemble, split over four concurrent streams (gdx_col_operator_ensemble);
+    reports both rates, the violation and acceptance rates and the ensemble moments' relative difference."""
     import numpy as np
     import torch
-    from gamedev_amd import CollisionOperator, Majorant, evolve_batch, synth
+    from gamedev_amd import CollisionOperator, Majorant, evolve_batch, evolve_ensemble, synth
 
     sizes = synth.ensemble_sizes(count, 16, 20, seed=54321, total=total)
 
@@ -318,13 +351,15 @@ def clumped_leg(count: int, total: int, steps: int, horizon: float, dev, samples
             M_start = ensemble_moments([d for _, d in ops])
         if mode == "learned":
             for op, _ in ops:
-                op.use_majorant(model, safety, min_owners=BATCH_MIN_OWNERS)
+                op.use_majorant(model, safety)
         active = sum(op.active_count() for op, _ in ops)
+        # one untimed step: the ensemble call balances its streams by each realization's last waves
+        evolve_ensemble([op for op, _ in ops], horizon, 1, ENSEMBLE_STREAMS)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         stats = []
-        for i in range(steps):
-            stats.extend(evolve_batch([op for op, _ in ops], horizon))
+        for step_stats in evolve_ensemble([op for op, _ in ops], horizon, steps, ENSEMBLE_STREAMS):
+            stats.extend(step_stats)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         for op, _ in ops:
@@ -537,6 +572,26 @@ def main() -> int:
         op.evolve(args.horizon)
     torch.cuda.synchronize()
     with_geo = args.geometry_steps*n_active/(time.perf_counter() - t0)
+    # the transported Strang cadence (doc/guide_swarm.md:3013-3020): M steps of C(h/2) T(h) C(h/2) make
+    # 2M operator calls of h/2 with M+1 geometry epochs and M-1 reused epochs (state screen only); the
+    # transport itself is not run, its end of the epoch is the forced rebuild
+    M = max(1, args.geometry_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.geometry()
+    op.evolve(0.5*args.horizon)
+    for k in range(M):
+        op.geometry()
+        op.evolve(0.5*args.horizon)
+        if k + 1 < M:
+            op.check_state()
+            op.evolve(0.5*args.horizon)
+    torch.cuda.synchronize()
+    strang_s = time.perf_counter() - t0
+    strang = {"transported_steps": M, "operator_calls": 2*M, "geometry_epochs": M + 1, "reused_epochs": M - 1,
+              "half_horizon": 0.5*args.horizon, "wall_s": strang_s,
+              "calls_per_s_x_owners": 2*M*n_active/strang_s, "transported_steps_per_s_x_owners": M*n_active/strang_s}
+    log(f"Strang cadence: {M} steps, {2*M} half-calls, {M + 1} epochs in {1e3*strang_s:.1f} ms")
 
     learned = None
     model = None
@@ -563,11 +618,12 @@ def main() -> int:
                    "roofline": roofline_of(l_kt, lp_tot), "totals": l_tot}
         op.use_majorant(None)
 
-    batched = None
+    batched = batched_fid = None
     if args.batch > 0:
         del op, d
         torch.cuda.empty_cache()
-        batched = batch_leg(args.batch, args.batch_steps, args.horizon, dev, model, learned)
+        batched = batch_leg("ensemble_2d", args.batch, args.batch_steps, dev, args.train_samples, args.epochs)
+        batched_fid = batch_leg("fiducial_3d", args.batch, args.batch_steps, dev, args.train_samples, args.epochs)
 
     clumped = None
     if args.clumped > 0:
@@ -622,8 +678,10 @@ def main() -> int:
             "geometry_epoch_ms": geometry_ms,
             "geometry_autotune": tune,
             "updates_per_s_with_geometry_rebuild": with_geo,
+            "strang_cadence": strang,
             "active_owners": n_active,
             "batched_ensemble": batched,
+            "batched_fiducial": batched_fid,
             "clumped_batched": clumped,
             "wedge": wedge,
         },

@@ -78,6 +78,7 @@ _SIGS = {
     "gdx_col_reset_controller": [P],
     "gdx_col_synchronize": [P],
     "gdx_col_operator_batch": [P, C.c_int, C.c_double, P],
+    "gdx_col_operator_ensemble": [P, C.c_int, C.c_double, C.c_int, C.c_int, P],
     "gdx_col_open_log": [P, C.c_char_p],
     "gdx_col_set_clock": [P, C.c_double],
     "gdx_col_save_controller": [P, C.c_char_p],

@@ -82,6 +82,11 @@ class CollisionOperator:
         self._keep = (records,)
         self._call(self.lib.gdx_col_bind_swarm, _ptr(records))
 
+    def set_stream(self, stream) -> None:
+        """Launch on a HIP stream (a torch.cuda.Stream or a raw handle; None: the null stream)."""
+        handle = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        self._call(self.lib.gdx_col_set_stream, C.c_void_p(handle))
+
     # ---- operator ----
     def geometry(self) -> None:
         self._call(self.lib.gdx_col_geometry)
@@ -279,6 +284,23 @@ def evolve_batch(ops: list, duration: float) -> list[dict]:
     stats = (_lib.Stats*len(ops))()
     _lib.check(lib, ops[0].h, lib.gdx_col_operator_batch(handles, len(ops), float(duration), stats))
     return [s.as_dict() for s in stats]
+
+
+def evolve_ensemble(ops: list, duration: float, steps: int = 1, streams: int = 4) -> list[list[dict]]:
+    """`steps` operator calls on every operator of an ensemble of independent realizations, split over
+    `streams` concurrent groups (one HIP stream and host thread each, advancing independently); the same
+    results as separate calls. Returns the stats per step, one dict per operator in `ops` order."""
+    if not ops:
+        return []
+    lib = ops[0].lib
+    if any(op.lib is not lib for op in ops):
+        raise ValueError("all operators of an ensemble must use the same model library")
+    n = len(ops)
+    handles = (C.c_void_p*n)(*[op.h for op in ops])
+    stats = (_lib.Stats*(n*steps))()
+    _lib.check(lib, ops[0].h, lib.gdx_col_operator_ensemble(handles, n, float(duration), int(steps), int(streams),
+                                                           stats))
+    return [[stats[s*n + i].as_dict() for i in range(n)] for s in range(steps)]
 
 
 class Majorant:

@@ -12,8 +12,10 @@
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "controller.h"
@@ -603,6 +605,7 @@ struct op_run
         stats.bound_evals = (long long)counters[GDX_CTR_BOUND_EVALS];
         stats.dt_col = dt_col;
         stats.clock_dyn = duration;
+        c.last_waves = stats.waves;
         if (diag && dg->log.is_open())
         {
             rec.clock_sim = dg->clock_sim;
@@ -754,6 +757,51 @@ void run_operator(context &c, diag_state *dg, run_buffers &rb, double duration, 
 }  // namespace gdx
 
 using namespace gdx;
+
+namespace {
+
+// one batched call: every swarm's device work on the first handle's stream and run buffers
+void run_batch(std::vector<gdx_col_ctx *> &hs, double duration, const std::vector<gdx_col_stats *> &out)
+{
+    gdx_col_ctx *h0 = hs[0];
+    std::vector<std::unique_ptr<op_run>> owned;
+    std::vector<op_run *> runs;
+    for (gdx_col_ctx *h : hs)
+    {
+        owned.push_back(std::make_unique<op_run>(h->c, &h->diag, duration));
+        runs.push_back(owned.back().get());
+    }
+    for (size_t i = 1; i < hs.size(); ++i) hip_check(hipStreamSynchronize(hs[i]->c.stream), "batch stream");
+    kernel_timer timer(h0->c);
+    run_swarms(runs, h0->rb, h0->c.stream, timer);
+    for (size_t i = 0; i < hs.size(); ++i)
+        runs[i]->finish(h0->c.stream, h0->rb.h_counters.data() + size_t(GDX_CTRS)*i, out[i]);
+    timer.resolve();
+}
+
+// streams of concurrent ensemble groups, per device: created on first use at the default priority and
+// kept until the process ends
+constexpr int ENSEMBLE_STREAMS = 4, ENSEMBLE_MAX_STREAMS = 8;
+std::mutex g_stream_mu;
+std::vector<hipStream_t> g_streams[64];
+
+const std::vector<hipStream_t> &ensemble_streams(int count)
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "current device");
+    require(dev >= 0 && dev < 64, GDX_COL_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_stream_mu);
+    std::vector<hipStream_t> &v = g_streams[dev];
+    while (int(v.size()) < count)
+    {
+        hipStream_t st = nullptr;
+        hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "ensemble stream");
+        v.push_back(st);
+    }
+    return v;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -921,21 +969,79 @@ int gdx_col_operator_batch(const gdx_col_handle *handles, int count, double dura
         for (int j = 0; j < i; ++j)
             if (handles[j] == handles[i]) return GDX_COL_ERR_ARG;
     }
+    std::vector<gdx_col_ctx *> hs(handles, handles + count);
+    std::vector<gdx_col_stats *> outs(static_cast<size_t>(count), nullptr);
+    if (out)
+        for (int i = 0; i < count; ++i) outs[size_t(i)] = out + i;
+    return guard(&handles[0]->c, [&] { run_batch(hs, duration, outs); });
+}
+
+int gdx_col_operator_ensemble(const gdx_col_handle *handles, int count, double duration, int steps, int streams,
+                              gdx_col_stats *out)
+{
+    if (!handles || count <= 0 || !handles[0] || steps <= 0) return GDX_COL_ERR_ARG;
+    for (int i = 0; i < count; ++i)
+    {
+        if (!handles[i]) return GDX_COL_ERR_ARG;
+        for (int j = 0; j < i; ++j)
+            if (handles[j] == handles[i]) return GDX_COL_ERR_ARG;
+    }
     gdx_col_ctx *h0 = handles[0];
     return guard(&h0->c, [&] {
-        std::vector<std::unique_ptr<op_run>> owned;
-        std::vector<op_run *> runs;
+        const int lanes = std::max(1, std::min({streams > 0 ? streams : ENSEMBLE_STREAMS, count, ENSEMBLE_MAX_STREAMS}));
+        // a realization's critical path is its sequence of refresh waves, each a few dependent launches and
+        // host synchronisations: realizations ordered by the waves of their last call (before any call, by
+        // active-owner count) and dealt in turn, so the heaviest ones land on different streams (measured on
+        // 64-realization ensembles: better than greedy balancing of the wave sums, which stacks the many
+        // light realizations onto the streams of the heavy ones)
+        std::vector<int> order(static_cast<size_t>(count));
+        for (int i = 0; i < count; ++i) order[size_t(i)] = i;
+        bool known = true;
+        for (int i = 0; i < count; ++i) known &= handles[i]->c.last_waves > 0;
+        auto cost = [&](int i) -> long long {
+            return known ? (long long)handles[i]->c.last_waves : (long long)handles[i]->c.n_active;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        std::vector<std::vector<int>> part(static_cast<size_t>(lanes));
+        for (int p = 0; p < count; ++p) part[size_t(p % lanes)].push_back(order[size_t(p)]);
+        std::vector<hipStream_t> saved(static_cast<size_t>(count));
         for (int i = 0; i < count; ++i)
         {
-            owned.push_back(std::make_unique<op_run>(handles[i]->c, &handles[i]->diag, duration));
-            runs.push_back(owned.back().get());
+            saved[size_t(i)] = handles[i]->c.stream;
+            hip_check(hipStreamSynchronize(handles[i]->c.stream), "ensemble stream");
         }
-        // every swarm's device work goes to the first handle's stream
-        for (int i = 1; i < count; ++i) hip_check(hipStreamSynchronize(handles[i]->c.stream), "batch stream");
-        kernel_timer timer(h0->c);
-        run_swarms(runs, h0->rb, h0->c.stream, timer);
-        for (int i = 0; i < count; ++i) runs[i]->finish(h0->c.stream, h0->rb.h_counters.data() + size_t(GDX_CTRS)*i, out ? out + i : nullptr);
-        timer.resolve();
+        const std::vector<hipStream_t> &pool = ensemble_streams(lanes);
+        // one host thread per stream; each group advances `steps` batched calls on its own stream, independent
+        // of the other groups (the realizations are independent)
+        std::vector<int> rc(static_cast<size_t>(lanes), GDX_COL_OK);
+        auto work = [&](int g) {
+            std::vector<gdx_col_ctx *> hs;
+            for (int i : part[size_t(g)])
+            {
+                handles[i]->c.stream = pool[size_t(g)];
+                hs.push_back(handles[i]);
+            }
+            if (hs.empty()) return;
+            rc[size_t(g)] = guard(&hs[0]->c, [&] {
+                std::vector<gdx_col_stats *> outs(hs.size(), nullptr);
+                for (int s = 0; s < steps; ++s)
+                {
+                    if (out)
+                        for (size_t k = 0; k < hs.size(); ++k)
+                            outs[k] = out + size_t(s)*size_t(count) + size_t(part[size_t(g)][k]);
+                    run_batch(hs, duration, outs);
+                }
+            });
+        };
+        std::vector<std::thread> th;
+        for (int g = 1; g < lanes; ++g) th.emplace_back(work, g);
+        work(0);
+        for (std::thread &t : th) t.join();
+        for (int i = 0; i < count; ++i) handles[i]->c.stream = saved[size_t(i)];
+        for (int g = 0; g < lanes; ++g)
+            if (rc[size_t(g)] != GDX_COL_OK)
+                throw status_error(rc[size_t(g)], "ensemble stream " + std::to_string(g) + ": "
+                                   + handles[part[size_t(g)][0]]->c.last_error);
     });
 }
 

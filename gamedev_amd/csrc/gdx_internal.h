@@ -252,6 +252,7 @@ struct context
     int flags = 0;
     int search = GDX_COL_SEARCH_MORTON;
     int leaf_target = MORTON_LEAF_TARGET;   // Morton refinement threshold (gdx_col_set_leaf_target, autotuner)
+    int last_waves = 0;        // refresh waves of the last operator call (ensemble load balance)
     int topk_tile = 16;        // top-K merge tile: 0 insertion, 8/16/32 keys per merge (gdx_col_set_topk_tile, autotuner)
     hipStream_t stream = nullptr;
     bound_state user;

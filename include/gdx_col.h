@@ -152,6 +152,16 @@ int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);
  * reference runs one swarm per process); for ensembles of small realizations. Launches go to the
  * first handle's stream; out holds `count` entries. */
 int gdx_col_operator_batch(const gdx_col_handle *handles, int count, double duration, gdx_col_stats *out);
+/* no reference counterpart: `steps` operator calls of `duration` on every swarm of an ensemble of
+ * independent realizations. The swarms split into `streams` groups (<= 0: 4; at most 8), dealt in turn
+ * in order of the refresh waves of their last call (before any call: of their active-owner count), so
+ * the heaviest land on different groups. Each group runs its batched calls on its own HIP stream from its
+ * own host thread and advances through the steps without waiting for the other groups, so that one
+ * realization's long sequence of refresh waves overlaps the others' work. Results equal separate
+ * gdx_col_operator calls; the handles' own streams are synchronised first and not used. out (may be NULL)
+ * holds steps x count entries, step-major (out[s * count + i]). */
+int gdx_col_operator_ensemble(const gdx_col_handle *handles, int count, double duration, int steps, int streams,
+                              gdx_col_stats *out);
 /* controller memory reset at output boundaries (src/swarm/swarm_runtime.cu:491-496); also resets the
  * adaptive scale of an attached majorant's bound */
 int gdx_col_reset_controller(gdx_col_handle h);

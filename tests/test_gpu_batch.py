@@ -76,3 +76,33 @@ def test_mixed_learned_and_plain_swarms_in_one_batch(gpu):
         assert np.array_equal(single[i][0].rng(), batch[i][0].rng())
     mass1 = (batch[1][1]["numr"]*batch[1][1]["size"]**3).sum().item()
     assert abs(mass1/mass0 - 1) < 1e-10
+
+
+def test_ensemble_streams_equal_separate_calls(gpu):
+    """gdx_col_operator_ensemble: the swarms split over three streams driven by three host threads, each
+    group advancing through the steps on its own, give exactly the separate calls' states, random counters
+    and statistics, and leave the handles' own streams in place."""
+    from gamedev_amd import evolve_ensemble, synth
+    sizes = [3000, 17000, 41000, 8000, 25000, 5000, 12000]
+    states = [synth.fiducial_3d_state(n, seed=60 + i) for i, n in enumerate(sizes)]
+    single = [_make(gpu, st, 300 + i) for i, st in enumerate(states)]
+    ens = [_make(gpu, st, 300 + i) for i, st in enumerate(states)]
+    # one call of two steps, then one of one step: the groups are formed by active owners, then by the
+    # waves of the last call
+    for steps in (2, 1):
+        print(f"[batch] ensemble of {steps} step(s)", flush=True)
+        b = evolve_ensemble([op for op, _ in ens], 0.05, steps=steps, streams=3)
+        assert len(b) == steps and all(len(x) == len(ens) for x in b)
+        for step in range(steps):
+            a = [op.evolve(0.05) for op, _ in single]
+            for sa, sb in zip(a, b[step]):
+                for k in ("events", "chain_owners", "waves", "baths", "owner_updates", "pair_evals",
+                          "max_chain_events"):
+                    assert sa[k] == sb[k], k
+    assert sum(s["events"] for s in a) > 0
+    for (op1, d1), (op2, d2) in zip(single, ens):
+        assert np.array_equal(d1["size"].cpu().numpy(), d2["size"].cpu().numpy())
+        assert np.array_equal(d1["numr"].cpu().numpy(), d2["numr"].cpu().numpy())
+        assert np.array_equal(op1.rng(), op2.rng())
+    # a plain call on one of the handles afterwards still works on its own stream
+    ens[0][0].evolve(0.05)
