@@ -527,33 +527,36 @@ __device__ __forceinline__ void traverse(const gdx_node *nodes, int dim, int max
             leaf(nd.begin, nd.begin + nd.count, need);
             continue;
         }
-        int ids[8];
-        float key[8];
-        int nv = 0;
+        // children some lane needs, each child's distance from the wavefront's centre kept by the lane of
+        // its index; the needed children go on the stack farthest first (nearest on top, equal distances
+        // in child order), each lane writing its own child at its rank
         const int nch = nd.child_count & 15;
+        unsigned needed = 0u;
+        float kd = __builtin_inff();
         for (int ch = 0; ch < nch; ++ch)
         {
-            int cid = nd.first_child + ch;
-            gdx_node cn = nodes[cid];
+            gdx_node cn = nodes[nd.first_child + ch];
             bool cneed = qactive && node_d2(qx, qy, qz, cn, dim, max_level) <= cut;
             if (__ballot(cneed) == 0ull) continue;
-            float kd = node_d2(cx, cy, cz, cn, dim, max_level);
-            int p = nv;
-            while (p > 0 && key[p - 1] > kd) { key[p] = key[p - 1]; ids[p] = ids[p - 1]; --p; }
-            key[p] = kd;
-            ids[p] = cid;
-            ++nv;
+            needed |= 1u << ch;
+            const float k = node_d2(cx, cy, cz, cn, dim, max_level);
+            if (lane == ch) kd = k;
         }
+        int nv = __popc(needed);
         if (sp + nv > TOPK_STACK)
         {
             if (lane == 0) atomicOr(overflow, 1u);
             nv = 0;
         }
-        for (int p = nv - 1; p >= 0; --p)
+        // ranks by whole-wavefront shuffles (every lane active), then each needed child's lane writes it
+        int rank = 0;
+        for (int m = 0; m < nch; ++m)
         {
-            if (lane == 0) stack[sp] = ids[p];
-            ++sp;
+            const float km = __shfl(kd, m, WAVE);
+            rank += ((needed >> m) & 1u) && (km < kd || (km == kd && m < lane));
         }
+        if (nv > 0 && lane < nch && ((needed >> lane) & 1u)) stack[sp + nv - 1 - rank] = nd.first_child + lane;
+        sp += nv;
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -613,8 +616,12 @@ __device__ __forceinline__ void topk_flush(uint64_t (&L)[K], const uint64_t *buf
 // records the exact list would take); passing keys are appended to the lane's column of an LDS buffer
 // and the wavefront merges every lane's buffer once some lane holds B. The K smallest keys of a union do
 // not depend on the order of merging, so the result is the insertion scheme's, bit for bit.
+// records per overflow check of the tiled scheme: a merge runs before a group when some lane could not
+// take all of it
+constexpr int TILE_GROUP = 4;
+
 template <int K, int B>
-__device__ void topk_wave_tiled(const float4 *rec, const gdx_node *nodes, int dim, int max_level, float qx, float qy,
+__device__ void topk_wave_tiled(const float4 *__restrict__ rec, const gdx_node *__restrict__ nodes, int dim, int max_level, float qx, float qy,
                                 float qz, float q2, bool qactive, float cx, float cy, float cz, float (&d)[K],
                                 int (&c)[K], int *stack, uint64_t *buf, unsigned int *overflow)
 {
@@ -627,27 +634,33 @@ __device__ void topk_wave_tiled(const float4 *rec, const gdx_node *nodes, int di
     traverse(nodes, dim, max_level, qx, qy, qz, qactive, cx, cy, cz, stack, overflow,
              [&] { return fminf(q2, __uint_as_float(uint32_t(L[K - 1] >> 32))); },
              [&](int begin, int end, bool need) {
-                 for (int t0 = begin; t0 < end; t0 += WAVE)
+                 // records at uniform addresses (scalar loads, four at a time); the filter is the lane's
+                 // cap and the distance of its K-th key as of its last merge (lanes that do not need the
+                 // leaf take nothing: d^2 >= 0 > -1); ties at that distance pass and the merge decides them
+                 const float cap = need ? q2 : -1.0f;
+                 float lim = fminf(cap, __uint_as_float(uint32_t(L[K - 1] >> 32)));
+                 for (int t = begin; t < end; t += TILE_GROUP)
                  {
-                     int idx = t0 + lane;
-                     float4 v = idx < end ? rec[idx] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-                     int n = min(WAVE, end - t0);
-                     for (int j = 0; j < n; ++j)
+                     if (__ballot(cnt > B - TILE_GROUP))
                      {
-                         float px = readlane_f(v.x, j), py = readlane_f(v.y, j), pz = readlane_f(v.z, j);
-                         int code = __builtin_amdgcn_readlane(__float_as_int(v.w), j);
-                         float dx = qx - px, dy = qy - py, dz = qz - pz;
-                         float d2 = dx*dx + dy*dy + dz*dz;
-                         const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(code);
-                         if (need && code >= 0 && d2 <= q2 && key < L[K - 1])
+                         topk_flush<K, B>(L, col, cnt);
+                         cnt = 0;
+                         lim = fminf(cap, __uint_as_float(uint32_t(L[K - 1] >> 32)));
+                     }
+                     // the group's loads first (indices clamped into the leaf), then its tests
+                     float4 v[TILE_GROUP];
+#pragma unroll
+                     for (int u = 0; u < TILE_GROUP; ++u) v[u] = rec[min(t + u, end - 1)];
+#pragma unroll
+                     for (int u = 0; u < TILE_GROUP; ++u)
+                     {
+                         const int code = __float_as_int(v[u].w);
+                         const float dx = qx - v[u].x, dy = qy - v[u].y, dz = qz - v[u].z;
+                         const float d2 = dx*dx + dy*dy + dz*dz;
+                         if (t + u < end && code >= 0 && d2 <= lim)
                          {
-                             col[cnt*WAVE] = key;
+                             col[cnt*WAVE] = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(code);
                              ++cnt;
-                         }
-                         if (__ballot(cnt == B))
-                         {
-                             topk_flush<K, B>(L, col, cnt);
-                             cnt = 0;
                          }
                      }
                  }
@@ -684,14 +697,20 @@ __device__ __forceinline__ void wave_centre(float qx, float qy, float qz, bool q
 // one query per physical record; DEDUP when two images of one particle can share a search ball
 // TILE = 0: the insertion scheme; TILE = B > 0: the tiled scheme with B-key merges (no image dedup)
 template <bool DEDUP, int TILE>
-__global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *rec, const gdx_node *nodes, int n, int dim,
-                                                        int max_level, const int *sidx, const float4 *pts,
-                                                        const int *pidx_slot, const unsigned char *group,
-                                                        bound_state u, int *nbr, double *measure,
-                                                        unsigned long long *graph, unsigned int *overflow,
-                                                        unsigned char *nbr_image)
+__global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *__restrict__ rec,
+                                                        const gdx_node *__restrict__ nodes, int n, int dim,
+                                                        int max_level, const int *__restrict__ sidx,
+                                                        const float4 *__restrict__ pts,
+                                                        const int *__restrict__ pidx_slot,
+                                                        const unsigned char *__restrict__ group,
+                                                        bound_state u, int *__restrict__ nbr,
+                                                        double *__restrict__ measure,
+                                                        unsigned long long *__restrict__ graph,
+                                                        unsigned int *__restrict__ overflow,
+                                                        unsigned char *__restrict__ nbr_image)
 {
     static_assert(!(DEDUP && TILE), "image dedup runs on the insertion scheme");
+    static_assert(TILE == 0 || TILE >= TILE_GROUP, "a merge tile holds at least one record group");
     __shared__ int stacks[TOPK_BLOCK / WAVE][TOPK_STACK];
     __shared__ uint64_t tiles[TOPK_BLOCK / WAVE][TILE > 0 ? TILE : 1][WAVE];
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
