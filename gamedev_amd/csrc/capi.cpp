@@ -110,6 +110,7 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.mlp = c.mlp;
     r.safety = c.safety*c.safety_scale;
     r.lam = c.lam.p;
+    r.lam_add = c.lam_add.p;
     r.dirty = c.dirty.p;
     const bool track = c.model && c.rev_valid;
     r.stale = track ? c.stale.p : nullptr;
@@ -602,7 +603,8 @@ struct op_run
         stats.exact_checks = (long long)counters[GDX_CTR_CHECKS];
         stats.violations = (long long)counters[GDX_CTR_VIOLATIONS];
         stats.candidate_pairs = (long long)counters[GDX_CTR_CAND_PAIRS];
-        stats.bound_evals = (long long)counters[GDX_CTR_BOUND_EVALS];
+        stats.bound_evals = 0;
+        for (int k = 0; k < GDX_CTR_SPREAD; ++k) stats.bound_evals += (long long)counters[GDX_CTR_BOUND_EVALS + k];
         stats.dt_col = dt_col;
         stats.clock_dyn = duration;
         c.last_waves = stats.waves;

@@ -153,7 +153,7 @@ struct run_buffers
     size_t cap = 0;
     dbuf<int> qprefix;
     dbuf<int> work;                   // chain pull counters: heavy queue, light queue
-    dbuf<int2> changed;               // a publish's changed slots {table index, slot} (stale marking)
+    dbuf<gdx_changed> changed;        // a publish's changed slots and their replaced records (bound upkeep)
     dbuf<int> nchanged;
     int resident = 0;                 // resident blocks of the persistent chain kernel
     // controller sums, queue lengths, counters and errors of every group and swarm of the run. The
@@ -304,6 +304,7 @@ struct context
     dbuf<int> cqueue;          // learned: thinning candidates per group slot range
     dbuf<double2> cstart;      // learned: {candidate time, bound} per candidate
     dbuf<double> lam;          // learned: per-slot S * softplus(net) of the last evaluation
+    dbuf<double> lam_add;      // learned: exact pair-rate increases from partner changes since then
     dbuf<unsigned char> dirty; // state changed since the slot's last publish
     dbuf<unsigned char> stale; // learned: bound inputs changed since the slot's last evaluation
     dbuf<unsigned> rev_beg, rev_end;   // learned: reverse neighbour lists of the geometry epoch

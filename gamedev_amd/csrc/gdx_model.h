@@ -112,6 +112,9 @@ struct gdx_gref
     double safety;                            // bound = safety * S * softplus(net)
     double *lam;                              // per slot: S * softplus(net) of the owner's last evaluation
                                               // (the bound before the safety factors)
+    double *lam_add;                          // per slot: exact increases of the owner's pair rates from
+                                              // partner changes since that evaluation (added after the
+                                              // safety factors: bound = safety * lam + lam_add)
     unsigned char *dirty;                     // per slot: state changed since the slot's last publish
     unsigned char *stale;                     // per slot: bound inputs changed since its last evaluation
                                               // (null: not tracked, the swarm has no majorant)
@@ -125,11 +128,22 @@ struct gdx_gref
     int *ccount;                              // the group's candidate count
 };
 
-// operator counters per swarm
+// a slot whose published record changed: its table entry, slot and the record it replaced
+struct gdx_changed
+{
+    int ref, slot;
+    double size, numr;
+};
+
+// operator counters per swarm; the bound-evaluation count is spread over GDX_CTR_SPREAD slots (one per
+// block index modulo the spread, summed on the host) so that the majorant's blocks do not all add to one
+// address
+constexpr int GDX_CTR_SPREAD = 32;
 enum
 {
     GDX_CTR_EVENTS, GDX_CTR_CHAINED, GDX_CTR_PAIRS, GDX_CTR_LONGEST, GDX_CTR_THINNED, GDX_CTR_CANDIDATES,
-    GDX_CTR_ACCEPTED, GDX_CTR_CHECKS, GDX_CTR_VIOLATIONS, GDX_CTR_CAND_PAIRS, GDX_CTR_BOUND_EVALS, GDX_CTRS
+    GDX_CTR_ACCEPTED, GDX_CTR_CHECKS, GDX_CTR_VIOLATIONS, GDX_CTR_CAND_PAIRS, GDX_CTR_BOUND_EVALS,
+    GDX_CTRS = GDX_CTR_BOUND_EVALS + GDX_CTR_SPREAD
 };
 // per-wave counters per swarm
 enum

@@ -1215,6 +1215,7 @@ void load_state(context &c)
     c.cqueue.alloc(na);
     c.cstart.alloc(na);
     c.lam.alloc(na);
+    c.lam_add.alloc(na);
     c.dirty.alloc(na);
     c.stale.alloc(na);
     c.qsum.alloc(na);
@@ -1230,6 +1231,7 @@ void load_state(context &c)
     }
     // a new state: no kept bound is valid, no publish is pending
     hip_check(hipMemsetAsync(c.dirty.p, 0, na, st), "reset dirty flags");
+    hip_check(hipMemsetAsync(c.lam_add.p, 0, sizeof(double)*na, st), "reset bound increments");
     invalidate_bounds(c);
     c.state_loaded = true;
 }

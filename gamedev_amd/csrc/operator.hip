@@ -185,10 +185,11 @@ __global__ void k_reset_scratch(const gdx_gref *table, int ng)
 }
 
 // col_bath_init + col_comp_zero + col_size_scan (col_bath_init.cu:11-23, col_size_scan.cu:13-25)
-__global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, int2 *changed, int *nchanged)
+__global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *changed, int *nchanged)
 {
     __shared__ unsigned long long red[2][WAVES];
     __shared__ int list[PUBLISH_OWNERS];
+    __shared__ double2 replaced[PUBLISH_OWNERS];
     __shared__ int count, base;
     int gi, s0, s1;
     if (!due_block(w, PUBLISH_OWNERS, gi, s0, s1)) return;
@@ -201,16 +202,24 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, int2 *changed, in
     for (int s = s0 + threadIdx.x; s < s1; s += BLOCK)
     {
         double2 v = st[s];
+        const bool changed_here = G.dirty[s];
+        const double2 was = (changed_here && G.stale) ? res[s] : v;
         res[s] = v;
         G.res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
         // positive doubles order like their bit patterns (col_size_scan.cu:20-24)
         if (v.x > 0 && isfinite(v.x)) { lo = min(lo, dbits(v.x)); hi = max(hi, dbits(v.x)); }
-        // a changed record: the bounds of the slot and of every owner whose row holds it go stale
-        // (k_mark_stale, from the list of changed slots)
-        if (G.dirty[s])
+        // a changed record: the slot's own bound goes stale and every owner whose row holds it takes the
+        // exact change of that pair rate (k_mark_stale, from the list of changed slots and the records
+        // they replaced)
+        if (changed_here)
         {
             G.dirty[s] = 0;
-            if (G.stale) list[atomicAdd(&count, 1)] = s;
+            if (G.stale)
+            {
+                const int k = atomicAdd(&count, 1);
+                list[k] = s;
+                replaced[k] = was;
+            }
         }
     }
 #pragma unroll
@@ -233,23 +242,84 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, int2 *changed, in
         base = count ? atomicAdd(nchanged, count) : 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < count; i += BLOCK) changed[base + i] = make_int2(w.ref[gi], list[i]);
+    for (int i = threadIdx.x; i < count; i += BLOCK)
+        changed[base + i] = gdx_changed{w.ref[gi], list[i], replaced[i].x, replaced[i].y};
 }
 
-// stale flags of the changed slots of a publish and of the owners whose rows hold them (one K-lane
-// segment per changed slot, the reverse list in strides of K)
-__global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, const int2 *changed, const int *nchanged)
+// bound upkeep after a publish: the changed slot's own bound goes stale (its owner is evaluated again),
+// and every owner whose row holds the slot and whose bound is kept adds the increase of that one pair
+// rate, old record to new (decreases are dropped). The pair rate is the exact FP64 term of the rate pass
+// for the owner's current state, so a kept bound that held at its evaluation still holds: R rises by at
+// most the sum of its increments. A wavefront takes MARK_SLOTS changed slots at a time and spreads the
+// (slot, reader) pairs of all of them over its lanes, so that reader lists of uneven length keep the
+// lanes busy.
+constexpr int MARK_SLOTS = 16;
+
+__global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, const gdx_changed *changed,
+                                                     const int *nchanged)
 {
+    __shared__ int incl[WAVES][MARK_SLOTS];
+    __shared__ unsigned first[WAVES][MARK_SLOTS];
+    __shared__ gdx_changed slot_of[WAVES][MARK_SLOTS];
     const int n = *nchanged;
-    const int kl = threadIdx.x % SEG;
-    constexpr int SEGS = BLOCK / SEG;
-    for (int v = blockIdx.x*SEGS + int(threadIdx.x) / SEG; v < n; v += gridDim.x*SEGS)
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const int gw = blockIdx.x*WAVES + wv, nw = gridDim.x*WAVES;
+    for (int v0 = gw*MARK_SLOTS; v0 < n; v0 += nw*MARK_SLOTS)
     {
-        const int2 c = changed[v];
-        const gdx_gref &G = table[c.x];
-        if (kl == 0) G.stale[c.y] = 1;
-        const unsigned b = G.rev_beg[c.y], e = G.rev_end[c.y];
-        for (unsigned i = b + kl; i < e; i += SEG) G.stale[G.rev[i]] = 1;
+        int cnt = 0;
+        if (lane < MARK_SLOTS)
+        {
+            gdx_changed c{0, 0, 0.0, 0.0};
+            unsigned b = 0;
+            if (v0 + lane < n)
+            {
+                c = changed[v0 + lane];
+                const gdx_gref &G = table[c.ref];
+                G.stale[c.slot] = 1;
+                if (G.bound_reuse)
+                {
+                    b = G.rev_beg[c.slot];
+                    cnt = int(G.rev_end[c.slot] - b);
+                }
+            }
+            slot_of[wv][lane] = c;
+            first[wv][lane] = b;
+        }
+        // inclusive prefix of the reader counts over the MARK_SLOTS lanes (every lane shuffles)
+#pragma unroll
+        for (int d = 1; d < MARK_SLOTS; d <<= 1)
+        {
+            const int t = __shfl_up(cnt, d, WAVE);
+            if (lane >= d && lane < MARK_SLOTS) cnt += t;
+        }
+        if (lane < MARK_SLOTS) incl[wv][lane] = cnt;
+        const int total = __shfl(cnt, MARK_SLOTS - 1, WAVE);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int p = lane; p < total; p += WAVE)
+        {
+            int e = 0;
+#pragma unroll
+            for (int k = 0; k < MARK_SLOTS - 1; ++k) e += incl[wv][k] <= p;
+            const gdx_changed c = slot_of[wv][e];
+            const gdx_gref &G = table[c.ref];
+            const int o = G.rev[first[wv][e] + unsigned(p - (e ? incl[wv][e - 1] : 0))];
+            const double m = G.measure[o];
+            if (!(m > 0.0) || G.stale[o]) continue;      // a stale bound is evaluated again anyway
+            partner fresh = load_partner(c.slot, G.res, G.hgs);
+            partner before = fresh;
+            before.size_j = c.size;
+            before.numr_j = c.numr;
+            gdx_env env{};
+            double hg_o = 0.0;
+            if constexpr (COAG_KERNEL == 3) env = G.env[o];
+            if constexpr (PARTNER_HG) hg_o = G.hgs[o];
+            const owner_terms t = make_owner(env, G.st[o].x, m, hg_o);
+            const double up = eval_pair_rate(t, fresh, G.lambda_0).rate - eval_pair_rate(t, before, G.lambda_0).rate;
+            if (up > 0.0 && isfinite(up)) atomicAdd(G.lam_add + o, up);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -416,7 +486,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         if (sub >= 0)
         {
             const bool check = own && live && V;
-            const bool viol = check && !(R <= GP->safety*GP->lam[s]);
+            const bool viol = check && !(R <= GP->safety*GP->lam[s] + GP->lam_add[s]);
             const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
             if (lane == 0 && nc) atomicAdd(&checked[0], nc);
             if (lane == 0 && nv) atomicAdd(&checked[1], nv);
@@ -599,7 +669,8 @@ __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
     if (stale) list[base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
     __syncthreads();
     const int n = count;
-    if (threadIdx.x == 0 && n > 0) atomicAdd(G.counters + GDX_CTR_BOUND_EVALS, (unsigned long long)n);
+    if (threadIdx.x == 0 && n > 0)
+        atomicAdd(G.counters + GDX_CTR_BOUND_EVALS + blockIdx.x % GDX_CTR_SPREAD, (unsigned long long)n);
     // wavefront-uniform: the wavefronts past the list are done, no block barrier follows
     if (wv*WAVE >= n) return;
     const bool has = int(threadIdx.x) < n;
@@ -628,6 +699,7 @@ __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
     if (has)
     {
         G.lam[so] = S*static_cast<double>(ys[threadIdx.x]);
+        G.lam_add[so] = 0.0;
         if (G.stale) G.stale[so] = 0;
     }
 }
@@ -760,7 +832,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     if (valid && learned && !exact_owner(w, gi, s))
     {
         thin = measure[s] > 0.0;
-        const double L = thin ? G.safety*G.lam[s] : 0.0;
+        const double L = thin ? G.safety*G.lam[s] + G.lam_add[s] : 0.0;
         if (thin && (!isfinite(L) || L < 0.0))
         {
             // unusable bound: the owner takes the plain chain from the bath start
@@ -1639,8 +1711,8 @@ void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &
 void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb)
 {
     if (!w.total) return;
-    constexpr int SEGS = BLOCK / SEG;
-    const int grid = std::max(1, std::min(4096, (w.total + SEGS - 1) / SEGS));
+    constexpr int PER_BLOCK = WAVES*MARK_SLOTS;
+    const int grid = std::max(1, std::min(4096, (w.total + PER_BLOCK - 1) / PER_BLOCK));
     k_mark_stale<<<grid, BLOCK, 0, st>>>(w.table, rb.changed.p, rb.nchanged.p);
     hip_check(hipGetLastError(), "launch mark stale");
 }
