@@ -161,7 +161,7 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
         // the same count as rated_count on the device
         int rated = e[i].sub < 0 ? e[i].cnt : thin_rated(e[i].cnt, e[i].sub);
         z += (rated + 255) / 256;
-        u += e[i].sub < 0 ? 0 : (e[i].cnt + 255) / 256;
+        u += e[i].sub < 0 ? 0 : (e[i].cnt + MAJ_SLOTS - 1) / MAJ_SLOTS;
         learned |= e[i].sub >= 0;
         total += e[i].cnt;
     }

@@ -135,6 +135,9 @@ struct gdx_changed
     double size, numr;
 };
 
+// due slots per block of the majorant kernel (host block starts and device share it)
+constexpr int MAJ_SLOTS = 4096;
+
 // operator counters per swarm; the bound-evaluation count is spread over GDX_CTR_SPREAD slots (one per
 // block index modulo the spread, summed on the host) so that the majorant's blocks do not all add to one
 // address

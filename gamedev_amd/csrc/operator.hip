@@ -643,64 +643,72 @@ __device__ double owner_features(const gdx_gref &G, int s, float *f)
     return (COAG_KERNEL == 3 ? double(W) : G.lambda_0*double(W)) / m;
 }
 
-// bound of every due owner of a learned swarm. An owner whose state and partner records are unchanged
-// since its last evaluation keeps it (stale flags: set by the publishes that change a record, for the
-// slot and for every owner whose row holds it, cleared here). The others are listed in LDS and evaluated
-// 64 per wavefront: features per lane, then the network over the wavefront's 64 owners as four 16-owner
-// tiles on the matrix cores.
+// bounds of the due owners of learned swarms. An owner whose own state is unchanged since its last
+// evaluation keeps its bound (stale flags: set by the publish that changes the owner's record, cleared
+// here; partner changes raise kept bounds exactly, k_mark_stale). A block takes MAJ_SLOTS due slots of one
+// group, lists its stale ones in LDS (a few percent of them once bounds are kept, so a block of 256 slots
+// would leave its one evaluating wavefront mostly idle), and evaluates the list 64 owners per wavefront:
+// features per lane, then the network over the wavefront's owners as four 16-owner tiles on the matrix
+// cores.
 __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
 {
     __shared__ float feat[BLOCK][mlp::F + 1];
     __shared__ float ys[BLOCK];
-    __shared__ int list[BLOCK];
+    __shared__ int list[MAJ_SLOTS];
     __shared__ int count;
     int gi, s0, s1;
-    if (!due_block(w, RATE_OWNERS, gi, s0, s1)) return;
+    if (!due_block(w, MAJ_SLOTS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
     if (threadIdx.x == 0) count = 0;
     __syncthreads();
     const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
-    const int s = s0 + int(threadIdx.x);
-    const bool stale = s < s1 && (!G.bound_reuse || !G.stale || G.stale[s]);
-    const unsigned long long ball = __ballot(stale);
-    int base = 0;
-    if (lane == 0 && ball) base = atomicAdd(&count, __popcll(ball));
-    base = __shfl(base, 0, WAVE);
-    if (stale) list[base + __popcll(ball & ((1ull << lane) - 1ull))] = s;
+    for (int t = 0; t < MAJ_SLOTS; t += BLOCK)
+    {
+        const int s = s0 + t + int(threadIdx.x);
+        const bool stale = s < s1 && (!G.bound_reuse || !G.stale || G.stale[s]);
+        const unsigned long long ball = __ballot(stale);
+        int b = 0;
+        if (lane == 0 && ball) b = atomicAdd(&count, __popcll(ball));
+        b = __shfl(b, 0, WAVE);
+        if (stale) list[b + __popcll(ball & ((1ull << lane) - 1ull))] = s;
+    }
     __syncthreads();
     const int n = count;
     if (threadIdx.x == 0 && n > 0)
         atomicAdd(G.counters + GDX_CTR_BOUND_EVALS + blockIdx.x % GDX_CTR_SPREAD, (unsigned long long)n);
-    // wavefront-uniform: the wavefronts past the list are done, no block barrier follows
-    if (wv*WAVE >= n) return;
-    const bool has = int(threadIdx.x) < n;
-    const int so = has ? list[threadIdx.x] : s0;
-    float f[mlp::F];
-    double S = 0.0;
-    if (has) S = owner_features(G, so, f);
-    else for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
-    for (int k = 0; k < mlp::F; ++k) feat[threadIdx.x][k] = f[k];
-    // the tiles read only this wavefront's rows
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const mlp::pack *P = static_cast<const mlp::pack *>(G.mlp);
+    // wavefront-uniform loop, no block barrier inside
+    for (int base = wv*WAVE; base < n; base += BLOCK)
+    {
+        const bool has = base + lane < n;
+        const int so = has ? list[base + lane] : s0;
+        float f[mlp::F];
+        double S = 0.0;
+        if (has) S = owner_features(G, so, f);
+        else for (int k = 0; k < mlp::F; ++k) f[k] = 0.0f;
+        for (int k = 0; k < mlp::F; ++k) feat[threadIdx.x][k] = f[k];
+        // the tiles read only this wavefront's rows
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-    for (int tt = 0; tt < WAVE / 16; ++tt)
-    {
-        const int row = wv*WAVE + tt*16 + (lane & 15);
-        mlp::forward_state fs;
-        float z = mlp::tile_forward(P, mlp::input_fragment(P, &feat[row][0], true), fs);
-        if (lane < 16) ys[row] = mlp::softplus(z);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (has)
-    {
-        G.lam[so] = S*static_cast<double>(ys[threadIdx.x]);
-        G.lam_add[so] = 0.0;
-        if (G.stale) G.stale[so] = 0;
+        for (int tt = 0; tt < WAVE / 16; ++tt)
+        {
+            const int row = wv*WAVE + tt*16 + (lane & 15);
+            mlp::forward_state fs;
+            float z = mlp::tile_forward(P, mlp::input_fragment(P, &feat[row][0], true), fs);
+            if (lane < 16) ys[row] = mlp::softplus(z);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (has)
+        {
+            G.lam[so] = S*static_cast<double>(ys[threadIdx.x]);
+            G.lam_add[so] = 0.0;
+            if (G.stale) G.stale[so] = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
