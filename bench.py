@@ -650,7 +650,7 @@ def main() -> int:
         # the metric's subject: the learned operator on the 1e7-representative swarm, beside the plain one
         value, ms = learned["value"], learned["ms_per_step"]
         roofline = learned["roofline"]
-        variant = ("learned: trained BF16 majorant, Poisson thinning with exact FP64 acceptance, exact 1/16 "
+        variant = ("learned: trained BF16 majorant, Poisson thinning with exact FP64 acceptance, exact 1/32 "
                    "subsample for the controller")
     else:
         value, ms, roofline, variant = plain_value, 1e3*elapsed/args.steps, roofline_plain, "plain"

@@ -109,6 +109,7 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.seed = c.seed;
     r.mlp = c.mlp;
     r.safety = c.safety*c.safety_scale;
+    r.thin_stride = c.thin_stride;
     r.lam = c.lam.p;
     r.lam_add = c.lam_add.p;
     r.dirty = c.dirty.p;
@@ -159,7 +160,7 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
         x += (e[i].cnt + 1023) / 1024;
         y += (e[i].cnt + 255) / 256;
         // the same count as rated_count on the device
-        int rated = e[i].sub < 0 ? e[i].cnt : thin_rated(e[i].cnt, e[i].sub);
+        int rated = e[i].sub < 0 ? e[i].cnt : thin_rated(e[i].cnt, e[i].sub, rb.h_table[size_t(e[i].ref)].thin_stride);
         z += (rated + 255) / 256;
         u += e[i].sub < 0 ? 0 : (e[i].cnt + MAJ_SLOTS - 1) / MAJ_SLOTS;
         learned |= e[i].sub >= 0;
@@ -302,6 +303,7 @@ struct op_run
             throw status_error(GDX_COL_ERR_SCHEDULE, "invalid local collision horizon");
         begin = wall::now();
         learned = c.mlp != nullptr && c.n_active >= c.majorant_min_owners;
+        c.thin_stride = c.n_active >= THIN_LARGE_OWNERS ? THIN_STRIDE_LARGE : THIN_STRIDE;
         sub_phase.assign(LOCAL_GROUPS, 0);
         stats.active_owners = c.n_active;
         dt_col = duration;
@@ -326,7 +328,7 @@ struct op_run
         for (int g : groups)
             if (c.group_cnt[g] > 0)
                 e.push_back(phase_entry{tbase + g, c.group_beg[g], c.group_cnt[g], chain ? steps[g] : 0.0,
-                                        learned ? sub_phase[g] % THIN_STRIDE : -1});
+                                        learned ? sub_phase[g] % c.thin_stride : -1});
     }
     long long due_owners() const
     {
@@ -397,8 +399,8 @@ struct op_run
             if (viol > 0.002*checks + 2.0*std::sqrt(0.002*checks) + 1.0) adapt_bound(1.25);
             else if (viol*2000.0 < checks) adapt_bound(0.97);
         }
-        // a learned rate pass rated one owner in THIN_STRIDE: bins merge on the population estimate
-        for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count*(rate_learned ? THIN_STRIDE : 1);
+        // a learned rate pass rated one owner in thin_stride: bins merge on the population estimate
+        for (int r = 0; r < RAW_BINS; ++r) counts[r] = raw_rate[r].count*(rate_learned ? c.thin_stride : 1);
         merged = build_binmap(counts, binmap);
         std::fill(ratebin.begin(), ratebin.begin() + merged, rate_bin{});
         for (int r = 0; r < RAW_BINS; ++r)
@@ -567,8 +569,8 @@ struct op_run
         {
             for (int g : groups)
             {
-                const int cnt = std::max(0, c.group_cnt[g]), sub = sub_phase[g] % THIN_STRIDE;
-                stats.thinned += cnt - thin_rated(cnt, sub);
+                const int cnt = std::max(0, c.group_cnt[g]), sub = sub_phase[g] % c.thin_stride;
+                stats.thinned += cnt - thin_rated(cnt, sub, c.thin_stride);
                 sub_phase[g]++;
             }
             // candidates above their bound (each taken exactly by superposition) also raise the scale

@@ -317,6 +317,7 @@ struct context
     int bound_reuse = 1;       // learned: keep the bounds of owners whose inputs are unchanged
     double safety = 1.2;
     double safety_scale = 1.0; // learned: adaptive factor on the bound, kept across operator calls
+    int thin_stride = THIN_STRIDE;   // learned: exact subsample stride (THIN_STRIDE_LARGE for large swarms)
     int majorant_min_owners = GDX_COL_MAJORANT_MIN_OWNERS;   // smaller swarms keep the plain method
     double fixed_bath = 0.0;   // > 0: every bath lasts this long (test access; the controller still audits)
     bool diag = false;         // GDX_COL_DIAGNOSTICS

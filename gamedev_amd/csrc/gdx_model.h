@@ -110,6 +110,7 @@ struct gdx_gref
     // learned majorant (mlp null: the plain method)
     const void *mlp;                          // packed network of the swarm's majorant
     double safety;                            // bound = safety * S * softplus(net)
+    int thin_stride;                          // exact subsample: one run of THIN_RUN slots in thin_stride
     double *lam;                              // per slot: S * softplus(net) of the owner's last evaluation
                                               // (the bound before the safety factors)
     double *lam_add;                          // per slot: exact increases of the owner's pair rates from
@@ -154,17 +155,21 @@ enum
     GDX_WAVE_LONGEST, GDX_WAVE_BLOCKS, GDX_WAVE_CHAINED, GDX_WAVE_CHECKS, GDX_WAVE_VIOLATIONS,
     GDX_WAVE_RATE_CHECKS, GDX_WAVE_RATE_VIOLATIONS, GDX_WAVE_CTRS
 };
-// learned swarms rate one run of THIN_RUN consecutive slots in every THIN_STRIDE runs exactly per bath
-// (the controller and audit subsample; consecutive slots keep the accesses coalesced)
+// learned swarms rate one run of THIN_RUN consecutive slots in every `stride` runs exactly per bath (the
+// controller and audit subsample; consecutive slots keep the accesses coalesced): a stride of
+// THIN_STRIDE (1/16 of the owners), or THIN_STRIDE_LARGE (1/32) for swarms of at least THIN_LARGE_OWNERS
+// active owners, whose subsample still holds ~10^4 owners per controller group
 constexpr int THIN_STRIDE = 16;
+constexpr int THIN_STRIDE_LARGE = 32;
+constexpr int THIN_LARGE_OWNERS = 1 << 21;
 constexpr int THIN_RUN = 32;
-// exactly rated owners among the first cnt slots of a group for subsample phase sub
-constexpr int thin_rated(int cnt, int sub)
+// owners of a group of cnt slots rated exactly in subsample phase sub
+constexpr int thin_rated(int cnt, int sub, int stride)
 {
-    return (cnt / (THIN_RUN*THIN_STRIDE))*THIN_RUN
-        + (cnt % (THIN_RUN*THIN_STRIDE) - sub*THIN_RUN < 0 ? 0
-           : (cnt % (THIN_RUN*THIN_STRIDE) - sub*THIN_RUN > THIN_RUN ? THIN_RUN
-              : cnt % (THIN_RUN*THIN_STRIDE) - sub*THIN_RUN));
+    return (cnt / (THIN_RUN*stride))*THIN_RUN
+        + (cnt % (THIN_RUN*stride) - sub*THIN_RUN < 0 ? 0
+           : (cnt % (THIN_RUN*stride) - sub*THIN_RUN > THIN_RUN ? THIN_RUN
+              : cnt % (THIN_RUN*stride) - sub*THIN_RUN));
 }
 
 // the due groups of one phase of a run, in device memory: group-table index, owner slot range in the

@@ -61,19 +61,19 @@ __device__ __forceinline__ bool due_block(const gdx_wave &w, int per_block, int 
     return s0 < s1;
 }
 
-// owners of due group gi rated with the exact K-sum in this bath: all of a plain swarm's, one in
-// THIN_STRIDE of a learned swarm's (slots beg + sub, beg + sub + THIN_STRIDE, ...)
+// owners of due group gi rated with the exact K-sum in this bath: all of a plain swarm's, one run of
+// THIN_RUN slots in the swarm's thin_stride runs of a learned swarm's (run sub, sub + stride, ...)
 __device__ __forceinline__ int rated_count(const gdx_wave &w, int gi)
 {
     const int sub = w.learned ? w.sub[gi] : -1;
     const int cnt = w.cnt[gi];
-    return sub < 0 ? cnt : thin_rated(cnt, sub);
+    return sub < 0 ? cnt : thin_rated(cnt, sub, w.table[w.ref[gi]].thin_stride);
 }
 __device__ __forceinline__ bool exact_owner(const gdx_wave &w, int gi, int s)
 {
     if (!w.learned) return true;
     const int sub = w.sub[gi];
-    return sub < 0 || ((s - w.beg[gi]) / THIN_RUN) % THIN_STRIDE == sub;
+    return sub < 0 || ((s - w.beg[gi]) / THIN_RUN) % w.table[w.ref[gi]].thin_stride == sub;
 }
 
 // wavefront reductions of a double through DPP lane moves (no LDS): quad exchanges, row shifts by 4
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     if (threadIdx.x < 2) checked[threadIdx.x] = 0u;
     __syncthreads();
     const int lane = threadIdx.x % WAVE;
-    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : ((i / THIN_RUN)*THIN_STRIDE + sub)*THIN_RUN + i % THIN_RUN);
+    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : ((i / THIN_RUN)*GP->thin_stride + sub)*THIN_RUN + i % THIN_RUN);
     const bool own = s < w.beg[gi] + w.cnt[gi];
     double2 me = own ? st[s] : make_double2(1.0, 0.0);
     const double m = own ? measure[s] : 0.0;

@@ -306,7 +306,8 @@ int gdx_col_majorant_train(gdx_col_majorant m, const float *features, const floa
 int gdx_col_majorant_sample(gdx_col_handle h, float *features, double *scale, double *rate);
 /* operator calls of h use the majorant (NULL: the plain method); safety >= 1 multiplies the bound. A
  * swarm with fewer than min_owners active owners (<= 0: GDX_COL_MAJORANT_MIN_OWNERS) keeps the plain
- * method: its exact 1/16 subsample would leave the controller's audit too few owners. The model must
+ * method: its exact 1/16 subsample (1/32 from 2^21 active owners) would leave the controller's audit too
+ * few owners. The model must
  * outlive its use by h. */
 #define GDX_COL_MAJORANT_MIN_OWNERS 65536
 int gdx_col_use_majorant(gdx_col_handle h, gdx_col_majorant m, double safety, int min_owners);
