@@ -731,7 +731,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             phase_wave pw = upload_phase(rb, e, st);
             timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256); });
             if (pw.w256.learned && pw.blocks256)
-                timer.time(GDX_COL_KERNEL_CANDIDATES, pw.w256.total, [&] { launch_candidates(st, pw.w256, pw.blocks256); });
+                timer.time(GDX_COL_KERNEL_CANDIDATES, pw.w256.total, [&] { launch_candidates(st, pw.w256, pw.blocks256, rb.cprefix.p); });
             if (pw.blocks256)
                 timer.time(GDX_COL_KERNEL_PREFIX, pw.w256.total, [&] { launch_queue_prefix(st, pw.w256, rb.qprefix.p); });
             timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.work.p, rb.resident); });

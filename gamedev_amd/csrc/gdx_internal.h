@@ -152,6 +152,7 @@ struct run_buffers
     size_t due_ints_bytes = 0;
     size_t cap = 0;
     dbuf<int> qprefix;
+    dbuf<int> cprefix;                // candidate blocks per due group (exclusive prefix), learned chain phases
     dbuf<int> work;                   // chain pull counters: heavy queue, light queue
     dbuf<gdx_changed> changed;        // a publish's changed slots and their replaced records (bound upkeep)
     dbuf<int> nchanged;
@@ -216,6 +217,7 @@ struct run_buffers
         d_int.p = reinterpret_cast<int *>(due_d.p);
         d_dt.p = reinterpret_cast<double *>(due_d.p + due_ints_bytes);
         qprefix.alloc(2*(m + 1));
+        cprefix.alloc(m + 1);
         work.alloc(2);
         nchanged.alloc(1);
         cap = m;
@@ -363,7 +365,7 @@ void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &
 void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb);
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_screen(hipStream_t st, const gdx_wave &w, int blocks);
-void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks, int *vstart);
 void launch_queue_prefix(hipStream_t st, const gdx_wave &w, int *qprefix);
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_zero(hipStream_t st, const zero_ranges &z);

@@ -962,20 +962,68 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 // size and the next wait. Owners whose next event still falls inside the bath enter the chain after
 // that first event, and the chain repeats the pending draw; any invalid rate or state sends the owner
 // to the chain from the candidate time with its draws uncommitted, and the chain reports it.
-__global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
+// per due group of a learned swarm: its candidate blocks (ceil(candidates / CHAIN_OWNERS)), exclusive
+// prefix in vstart[0..n], the total in vstart[n]
+__global__ __launch_bounds__(256) void k_cand_prefix(gdx_wave w, int *vstart)
+{
+    __shared__ int part[256];
+    const int t = threadIdx.x;
+    const int per = (w.n + 255) / 256;
+    auto blocks_of = [&](int i) {
+        return w.sub[i] < 0 ? 0 : (*w.table[w.ref[i]].ccount + CHAIN_OWNERS - 1) / CHAIN_OWNERS;
+    };
+    int sum = 0;
+    for (int i = t*per; i < min(w.n, (t + 1)*per); ++i) sum += blocks_of(i);
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0)
+    {
+        int acc = 0;
+        for (int k = 0; k < 256; ++k) { const int v = part[k]; part[k] = acc; acc += v; }
+    }
+    __syncthreads();
+    int a = part[t];
+    for (int i = t*per; i < min(w.n, (t + 1)*per); ++i)
+    {
+        vstart[i] = a;
+        a += blocks_of(i);
+    }
+    if (t == 255) vstart[w.n] = a;
+}
+
+// a grid of CAND_GRID blocks walks the candidate blocks of every group (vstart from k_cand_prefix),
+// each XCD's blocks over one contiguous range of them, so that no block is launched for the many due
+// slots without a candidate
+constexpr int CAND_GRID = 2048;
+
+__global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vstart)
 {
     // candidates, exact checks, violations, accepted, pair evaluations beyond the first K-sum, events
     // (= owners) finished here
     __shared__ unsigned tally[6];
     __shared__ int block_off[2][WAVES], block_tot[2], block_base[2];
     __shared__ double chunk_rate[N_K / 4][BLOCK];
-    int gi, s0, s1;
-    if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
-    if (w.sub[gi] < 0) return;
+    const int total = vstart[w.n];
+    const int per = (total + XCDS - 1) / XCDS, xcd = int(blockIdx.x) % XCDS;
+    const int local = int(blockIdx.x) / XCDS, stride = int(gridDim.x) / XCDS;
+    for (int v = local; v < per && xcd*per + v < total; v += stride)
+    {
+    const int vb = xcd*per + v;
+    int gi = 0;
+    {
+        int lo = 0, hi = w.n - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (vstart[mid] <= vb) lo = mid;
+            else hi = mid - 1;
+        }
+        gi = lo;
+    }
     const gdx_gref *GP = w.table + w.ref[gi];
     const int beg = w.beg[gi];
     const int n_cand = *GP->ccount;
-    if (s0 - beg >= n_cand) return;
+    const int s0 = beg + (vb - vstart[gi])*CHAIN_OWNERS;
     if (threadIdx.x < 6) tally[threadIdx.x] = 0u;
     if (threadIdx.x < 2) block_tot[threadIdx.x] = 0;
     __syncthreads();
@@ -1215,6 +1263,9 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w)
             atomicAdd(GP->wave_ctr + GDX_WAVE_BLOCKS, (unsigned long long)tally[5]);
             atomicAdd(GP->wave_ctr + GDX_WAVE_CHAINED, (unsigned long long)tally[5]);
         }
+    }
+    // the next candidate block resets the block's shared counters
+    __syncthreads();
     }
 }
 
@@ -1806,10 +1857,13 @@ void launch_screen(hipStream_t st, const gdx_wave &w, int blocks)
 }
 
 // learned waves only, after the screen of the same wave
-void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks)
+void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks, int *vstart)
 {
     if (!blocks || !w.learned) return;
-    k_candidates<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    k_cand_prefix<<<1, 256, 0, st>>>(w, vstart);
+    hip_check(hipGetLastError(), "launch candidate prefix");
+    const int grid = std::min(blocks, CAND_GRID);
+    k_candidates<<<std::max(XCDS, grid / XCDS*XCDS), BLOCK, 0, st>>>(w, vstart);
     hip_check(hipGetLastError(), "launch candidates");
 }
 
