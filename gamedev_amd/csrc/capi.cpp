@@ -726,15 +726,14 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.hcount.p, sizeof(int)*ng);
             z.add(rb.ccount.p, sizeof(int)*ng);
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
-            z.add(rb.work.p, 2*sizeof(int));
             launch_zero(st, z);
             phase_wave pw = upload_phase(rb, e, st);
-            timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w256, pw.blocks256); });
+            timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w1024, pw.blocks1024); });
             if (pw.w256.learned && pw.blocks256)
                 timer.time(GDX_COL_KERNEL_CANDIDATES, pw.w256.total, [&] { launch_candidates(st, pw.w256, pw.blocks256, rb.cprefix.p); });
             if (pw.blocks256)
                 timer.time(GDX_COL_KERNEL_PREFIX, pw.w256.total, [&] { launch_queue_prefix(st, pw.w256, rb.qprefix.p); });
-            timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.work.p, rb.resident); });
+            timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.resident); });
             timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w256, pw.blocks256); });
             rb.read_back(rb.off_wave, rb.back_bytes, st);          // wave counters, errors, audit bins
             hip_check(hipStreamSynchronize(st), "chain pass");

@@ -125,8 +125,6 @@ struct kd_tree
     dbuf<float> world;         // data bounds {lo xyz, hi xyz}
 };
 
-// buffers of operator runs: the group table, pinned staging and device copies of the due lists,
-// the chain's queue prefix and work counter
 // a typed view into one of run_buffers' shared allocations
 template <typename T>
 struct view
@@ -136,6 +134,8 @@ struct view
     const T *data() const { return p; }
 };
 
+// buffers of operator runs: the group table, pinned staging and device copies of the due lists,
+// the chain's queue prefix
 struct run_buffers
 {
     // due-list layout per phase: ref, beg, cnt, sub (n each), then four block-start arrays (n + 1 each),
@@ -153,9 +153,9 @@ struct run_buffers
     size_t cap = 0;
     dbuf<int> qprefix;
     dbuf<int> cprefix;                // candidate blocks per due group (exclusive prefix), learned chain phases
-    dbuf<int> work;                   // chain pull counters: heavy queue, light queue
     dbuf<gdx_changed> changed;        // a publish's changed slots and their replaced records (bound upkeep)
-    dbuf<int> nchanged;
+    dbuf<int> nchanged;               // entries of each changed-slot list
+    int changed_cap = 0;              // entries each changed-slot list can hold
     int resident = 0;                 // resident blocks of the persistent chain kernel
     // controller sums, queue lengths, counters and errors of every group and swarm of the run. The
     // read-backs share one device block and one page-locked mirror laid out [raw_rate][wave_ctr][err]
@@ -218,8 +218,7 @@ struct run_buffers
         d_dt.p = reinterpret_cast<double *>(due_d.p + due_ints_bytes);
         qprefix.alloc(2*(m + 1));
         cprefix.alloc(m + 1);
-        work.alloc(2);
-        nchanged.alloc(1);
+        nchanged.alloc(CHANGED_LISTS);
         cap = m;
     }
 };
@@ -376,7 +375,7 @@ unsigned long long majorant_version(gdx_col_majorant m);
 void invalidate_bounds(context &c);
 void build_reverse(context &c);
 int chain_resident_blocks();
-void launch_chain(hipStream_t st, const gdx_wave &w, int *work, int resident);
+void launch_chain(hipStream_t st, const gdx_wave &w, int resident);
 void launch_chain_audit(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_event_sum(context &c, hipStream_t st);
 void launch_gather_cached(context &c, double *out_particle_order);

@@ -136,6 +136,10 @@ struct gdx_changed
     double size, numr;
 };
 
+// changed-slot lists of one publish: the blocks on each XCD append to their own list (counter atomics
+// spread over as many addresses), each list holding up to its blocks' owners
+constexpr int CHANGED_LISTS = 8;
+
 // due slots per block of the majorant kernel (host block starts and device share it)
 constexpr int MAJ_SLOTS = 4096;
 

@@ -33,7 +33,6 @@ constexpr int WAVES = BLOCK / WAVE;
 constexpr int PUBLISH_OWNERS = 1024;
 constexpr int RATE_OWNERS = 256;
 constexpr int CHAIN_OWNERS = 256;
-constexpr int CHAIN_CHUNK = 8*NSEG;     // queue positions a chain wavefront takes per counter atomic
 // owners expected to take at least this many events in the bath (rate at the bath start times the
 // bath) join the heavy end of their group's queue region, which the chain takes first: long chains
 // then start with the launch instead of holding it open after the other owners are done
@@ -185,7 +184,7 @@ __global__ void k_reset_scratch(const gdx_gref *table, int ng)
 }
 
 // col_bath_init + col_comp_zero + col_size_scan (col_bath_init.cu:11-23, col_size_scan.cu:13-25)
-__global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *changed, int *nchanged)
+__global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *changed, int cap, int *nchanged)
 {
     __shared__ unsigned long long red[2][WAVES];
     __shared__ int list[PUBLISH_OWNERS];
@@ -193,32 +192,42 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
     __shared__ int count, base;
     int gi, s0, s1;
     if (!due_block(w, PUBLISH_OWNERS, gi, s0, s1)) return;
-    const gdx_gref G = w.table[w.ref[gi]];
-    const double2 *st = G.st;
-    double2 *res = G.res;
+    const gdx_gref &G = w.table[w.ref[gi]];
+    // the group's arrays, read from the table once (the loop's stores could otherwise alias them)
+    const double2 *__restrict__ st = G.st;
+    double2 *__restrict__ res = G.res;
+    float2 *__restrict__ res32 = G.res32;
+    unsigned char *__restrict__ dirty = G.dirty;
+    const bool track = G.stale != nullptr;
     if (threadIdx.x == 0) count = 0;
     __syncthreads();
     unsigned long long lo = ~0ull, hi = 0ull;
     for (int s = s0 + threadIdx.x; s < s1; s += BLOCK)
     {
-        double2 v = st[s];
-        const bool changed_here = G.dirty[s];
-        const double2 was = (changed_here && G.stale) ? res[s] : v;
+        const double2 v = st[s];
+        const bool changed_here = dirty[s];
+        double was_x = v.x, was_y = v.y;
+        if (changed_here && track)
+        {
+            const double2 r = res[s];
+            was_x = r.x;
+            was_y = r.y;
+        }
         res[s] = v;
-        G.res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+        res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
         // positive doubles order like their bit patterns (col_size_scan.cu:20-24)
         if (v.x > 0 && isfinite(v.x)) { lo = min(lo, dbits(v.x)); hi = max(hi, dbits(v.x)); }
         // a changed record: the slot's own bound goes stale and every owner whose row holds it takes the
-        // exact change of that pair rate (k_mark_stale, from the list of changed slots and the records
+        // exact change of that pair rate (k_mark_stale, from the lists of changed slots and the records
         // they replaced)
         if (changed_here)
         {
-            G.dirty[s] = 0;
-            if (G.stale)
+            dirty[s] = 0;
+            if (track)
             {
-                const int k = atomicAdd(&count, 1);
-                list[k] = s;
-                replaced[k] = was;
+                const int e = atomicAdd(&count, 1);
+                list[e] = s;
+                replaced[e] = make_double2(was_x, was_y);
             }
         }
     }
@@ -231,6 +240,8 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     if (lane == 0) { red[0][wave] = lo; red[1][wave] = hi; }
     __syncthreads();
+    // the blocks on one XCD share a changed-slot list
+    const int li = int(blockIdx.x) % CHANGED_LISTS;
     if (threadIdx.x == 0)
     {
         for (int w = 1; w < WAVES; ++w) { lo = min(lo, red[0][w]); hi = max(hi, red[1][w]); }
@@ -239,11 +250,12 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
             atomicMin(G.scr_min, lo);
             atomicMax(G.scr_max, hi);
         }
-        base = count ? atomicAdd(nchanged, count) : 0;
+        base = count ? atomicAdd(nchanged + li, count) : 0;
     }
     __syncthreads();
+    gdx_changed *out = changed + (long long)li*cap + base;
     for (int i = threadIdx.x; i < count; i += BLOCK)
-        changed[base + i] = gdx_changed{w.ref[gi], list[i], replaced[i].x, replaced[i].y};
+        out[i] = gdx_changed{w.ref[gi], list[i], replaced[i].x, replaced[i].y};
 }
 
 // bound upkeep after a publish: the changed slot's own bound goes stale (its owner is evaluated again),
@@ -255,13 +267,18 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
 // lanes busy.
 constexpr int MARK_SLOTS = 16;
 
-__global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, const gdx_changed *changed,
+__global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, const gdx_changed *changed, int cap,
                                                      const int *nchanged)
 {
     __shared__ int incl[WAVES][MARK_SLOTS];
     __shared__ unsigned first[WAVES][MARK_SLOTS];
     __shared__ gdx_changed slot_of[WAVES][MARK_SLOTS];
-    const int n = *nchanged;
+    // the publish's lists, taken one after the other: entry v of the whole is entry v - start[l] of list l
+    int start[CHANGED_LISTS + 1];
+    start[0] = 0;
+#pragma unroll
+    for (int l = 0; l < CHANGED_LISTS; ++l) start[l + 1] = start[l] + nchanged[l];
+    const int n = start[CHANGED_LISTS];
     const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const int gw = blockIdx.x*WAVES + wv, nw = gridDim.x*WAVES;
     for (int v0 = gw*MARK_SLOTS; v0 < n; v0 += nw*MARK_SLOTS)
@@ -273,7 +290,14 @@ __global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, con
             unsigned b = 0;
             if (v0 + lane < n)
             {
-                c = changed[v0 + lane];
+                const int v = v0 + lane;
+                int l = 0;
+#pragma unroll
+                for (int k = 1; k < CHANGED_LISTS; ++k) l = v >= start[k] ? k : l;
+                int at = v;
+#pragma unroll
+                for (int k = 0; k < CHANGED_LISTS; ++k) at = l == k ? v - start[k] : at;
+                c = changed[(long long)l*cap + at];
                 const gdx_gref &G = table[c.ref];
                 G.stale[c.slot] = 1;
                 if (G.bound_reuse)
@@ -803,14 +827,21 @@ __device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int 
     }
 }
 
-// screen (col_skip_scan.cu:13-59): one lane per owner. Owners with no event in the bath finish here
-// and enter the audit sums; the rest are appended to their group's queue region [beg, beg + cnt).
+// screen (col_skip_scan.cu:13-59): one lane per owner, SCREEN_ROUNDS owners per lane (a block covers
+// SCREEN_OWNERS slots, so that the block's fixed latencies - group search, counter atomics, audit flush
+// - are paid once per 1024 owners). Owners with no event in the bath finish here and enter the audit
+// sums; the rest are appended to their group's queue region [beg, beg + cnt).
+constexpr int SCREEN_ROUNDS = 4;
+constexpr int SCREEN_OWNERS = BLOCK*SCREEN_ROUNDS;
+static_assert(CHANGED_LISTS == XCDS, "one changed-slot list per XCD");
+static_assert(SCREEN_OWNERS == PUBLISH_OWNERS, "the screen and the publish share the 1024-owner block starts");
+
 __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 {
     __shared__ audit_lds a;
-    __shared__ int block_off[3][WAVES], block_tot[3], block_base[3];
+    __shared__ int block_tot[3], block_base[3];
     int gi, s0, s1;
-    if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
+    if (!due_block(w, SCREEN_OWNERS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
     const double4 *cached = G.cached;
     const double *measure = G.measure;
@@ -827,125 +858,145 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     __syncthreads();
 
     const int lane = threadIdx.x % WAVE;
-    const int s = s0 + int(threadIdx.x);
-    const bool valid = s < s1;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const bool learned = w.learned && w.sub[gi] >= 0;
-    bool chain = false, done = false, heavy = false;
-    // learned swarms: owners outside the exact subsample draw a candidate time from their bound
-    // (thinning); the subsample follows the plain screen (its bound was checked by the rate pass)
-    bool thin = false, violation = false, candidate = false;
-    double2 cs = make_double2(0.0, 0.0);
-    double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
-    double2 old = make_double2(1.0, 0.0);
-    if (valid && learned && !exact_owner(w, gi, s))
+    // per round: the owner's list (0 light queue, 1 candidates, 2 heavy queue, -1 none) and its offset
+    // within the block's share of that list, and a candidate's {time, bound}
+    int list[SCREEN_ROUNDS], off[SCREEN_ROUNDS];
+    double2 cand[SCREEN_ROUNDS];
+    unsigned n_viol = 0;
+#pragma unroll
+    for (int k = 0; k < SCREEN_ROUNDS; ++k)
     {
-        thin = measure[s] > 0.0;
-        const double L = thin ? G.safety*G.lam[s] + G.lam_add[s] : 0.0;
-        if (thin && (!isfinite(L) || L < 0.0))
+        const int s = s0 + k*BLOCK + int(threadIdx.x);
+        const bool valid = s < s1;
+        bool chain = false, done = false, heavy = false;
+        // learned swarms: owners outside the exact subsample draw a candidate time from their bound
+        // (thinning); the subsample follows the plain screen (its bound was checked by the rate pass)
+        bool violation = false, candidate = false;
+        double2 cs = make_double2(0.0, 0.0);
+        double hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0, rate0 = 0.0;
+        double2 old = make_double2(1.0, 0.0);
+        if (valid && learned && !exact_owner(w, gi, s))
         {
-            // unusable bound: the owner takes the plain chain from the bath start
-            violation = chain = true;
-        }
-        else if (thin && L > 0.0)
-        {
-            uint64_t trial = rng[s];
-            const double t1 = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / L;
-            rng[s] = trial;
-            if (t1 < end)
+            const bool thin = measure[s] > 0.0;
+            const double L = thin ? G.safety*G.lam[s] + G.lam_add[s] : 0.0;
+            if (thin && (!isfinite(L) || L < 0.0))
             {
-                candidate = true;
-                cs = make_double2(t1, L);
+                // unusable bound: the owner takes the plain chain from the bath start
+                violation = chain = true;
+            }
+            else if (thin && L > 0.0)
+            {
+                uint64_t trial = rng[s];
+                const double t1 = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / L;
+                rng[s] = trial;
+                if (t1 < end)
+                {
+                    candidate = true;
+                    cs = make_double2(t1, L);
+                }
             }
         }
-    }
-    else if (valid)
-    {
-        double4 c = cached[s];
-        double m = measure[s];
-        if (!(m > 0.0))
+        else if (valid)
         {
-            done = true;
-        }
-        else if (isfinite(c.x) && c.x >= 0.0)
-        {
-            if (c.x == 0.0)
+            double4 c = cached[s];
+            double m = measure[s];
+            if (!(m > 0.0))
             {
                 done = true;
             }
-            else
+            else if (isfinite(c.x) && c.x >= 0.0)
             {
-                uint64_t trial = rng[s];
-                double wait = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / c.x;
-                double remaining = end;
-                if (isfinite(wait) && wait > 0.0 && wait >= remaining)
+                if (c.x == 0.0)
                 {
-                    hazard = c.x*remaining;
-                    j1 = c.y*remaining;
-                    j2 = c.z*remaining;
-                    jmax = fmax(0.0, c.w);
                     done = true;
-                    rng[s] = trial;
                 }
-                // otherwise the draw stays uncommitted: the chain repeats the same deviate
+                else
+                {
+                    uint64_t trial = rng[s];
+                    double wait = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / c.x;
+                    double remaining = end;
+                    if (isfinite(wait) && wait > 0.0 && wait >= remaining)
+                    {
+                        hazard = c.x*remaining;
+                        j1 = c.y*remaining;
+                        j2 = c.z*remaining;
+                        jmax = fmax(0.0, c.w);
+                        done = true;
+                        rng[s] = trial;
+                    }
+                    // otherwise the draw stays uncommitted: the chain repeats the same deviate
+                }
+            }
+            chain = !done;
+            heavy = chain && isfinite(c.x) && c.x*end >= HEAVY_EVENTS;
+            if (done)
+            {
+                old = res[s];
+                rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
             }
         }
-        chain = !done;
-        heavy = chain && isfinite(c.x) && c.x*end >= HEAVY_EVENTS;
-        if (done)
+        int bs = done ? size_bin(old.x, lo, up) : -1;
+        double weight = old.y*grain_mass(old.x);
+        double probability = -expm1(-rate0*end);
+        bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard)
+            || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
+            || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
+        audit_wave<false>(a, done, bs, bs, weight, probability, hazard, j1, j2, jmax, 0.0, 0.0, bad);
+        // owners with an event join the group's queue (heavy ones from the region's start, the others
+        // from its end) and thinning candidates its candidate list: offsets within the block's share
+        // from one LDS atomic per wavefront and list
+        const unsigned long long bl = __ballot(chain && !heavy), bh = __ballot(heavy), cb = __ballot(candidate);
+        int ol = 0, oc = 0, oh = 0;
+        if (lane == 0)
         {
-            old = res[s];
-            rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
+            ol = bl ? atomicAdd(&block_tot[0], __popcll(bl)) : 0;
+            oc = cb ? atomicAdd(&block_tot[1], __popcll(cb)) : 0;
+            oh = bh ? atomicAdd(&block_tot[2], __popcll(bh)) : 0;
         }
+        ol = __shfl(ol, 0, WAVE);
+        oc = __shfl(oc, 0, WAVE);
+        oh = __shfl(oh, 0, WAVE);
+        list[k] = heavy ? 2 : chain ? 0 : candidate ? 1 : -1;
+        off[k] = heavy ? oh + __popcll(bh & below) : chain ? ol + __popcll(bl & below)
+                                                           : oc + __popcll(cb & below);
+        cand[k] = cs;
+        n_viol += __popcll(__ballot(violation));
     }
-    int bs = done ? size_bin(old.x, lo, up) : -1;
-    double weight = old.y*grain_mass(old.x);
-    double probability = -expm1(-rate0*end);
-    bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard)
-        || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
-        || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
-    audit_wave<false>(a, done, bs, bs, weight, probability, hazard, j1, j2, jmax, 0.0, 0.0, bad);
-    // owners with an event join the group's queue (heavy ones from the region's start, the others from
-    // its end) and thinning candidates its candidate list: wavefront offsets from LDS, one returning
-    // global atomic per block and list (the group counters are shared by hundreds of blocks on every XCD)
-    const unsigned long long bl = __ballot(chain && !heavy), bh = __ballot(heavy), cb = __ballot(candidate);
-    const int wv = threadIdx.x / WAVE;
-    if (lane == 0)
-    {
-        block_off[0][wv] = bl ? atomicAdd(&block_tot[0], __popcll(bl)) : 0;
-        block_off[1][wv] = cb ? atomicAdd(&block_tot[1], __popcll(cb)) : 0;
-        block_off[2][wv] = bh ? atomicAdd(&block_tot[2], __popcll(bh)) : 0;
-    }
+    // one returning global atomic per block and list (the group counters are shared by hundreds of
+    // blocks on every XCD)
     __syncthreads();
     if (threadIdx.x == 0) block_base[0] = block_tot[0] ? atomicAdd(G.qcount, block_tot[0]) : 0;
     if (threadIdx.x == WAVE) block_base[1] = block_tot[1] ? atomicAdd(G.ccount, block_tot[1]) : 0;
     if (threadIdx.x == 2*WAVE) block_base[2] = block_tot[2] ? atomicAdd(G.hcount, block_tot[2]) : 0;
     __syncthreads();
-    if (chain)
+#pragma unroll
+    for (int k = 0; k < SCREEN_ROUNDS; ++k)
     {
-        const unsigned long long below = (1ull << lane) - 1ull;
-        const int pos = heavy ? w.beg[gi] + block_base[2] + block_off[2][wv] + __popcll(bh & below)
-                              : w.beg[gi] + w.cnt[gi] - 1 - (block_base[0] + block_off[0][wv] + __popcll(bl & below));
-        G.queue[pos] = s;
-        if (learned) G.qstart[pos] = make_double2(0.0, 0.0);
-    }
-    if (candidate)
-    {
-        const int pos = w.beg[gi] + block_base[1] + block_off[1][wv] + __popcll(cb & ((1ull << lane) - 1ull));
-        G.cqueue[pos] = s;
-        G.cstart[pos] = cs;
+        const int s = s0 + k*BLOCK + int(threadIdx.x);
+        if (list[k] == 0 || list[k] == 2)
+        {
+            const int pos = list[k] == 2 ? w.beg[gi] + block_base[2] + off[k]
+                                         : w.beg[gi] + w.cnt[gi] - 1 - (block_base[0] + off[k]);
+            G.queue[pos] = s;
+            if (learned) G.qstart[pos] = make_double2(0.0, 0.0);
+        }
+        else if (list[k] == 1)
+        {
+            const int pos = w.beg[gi] + block_base[1] + off[k];
+            G.cqueue[pos] = s;
+            G.cstart[pos] = cand[k];
+        }
     }
     // unusable bounds (rare) are counted here; candidates are counted by the chain, screened owners by
     // the host
-    if (learned && __ballot(violation))
+    if (learned && n_viol && lane == 0)
     {
-        const unsigned n_viol = __popcll(__ballot(violation));
-        if (lane == 0)
-        {
-            atomicAdd(G.counters + GDX_CTR_CHECKS, (unsigned long long)n_viol);
-            atomicAdd(G.wave_ctr + GDX_WAVE_CHECKS, (unsigned long long)n_viol);
-            atomicAdd(G.counters + GDX_CTR_VIOLATIONS, (unsigned long long)n_viol);
-            atomicAdd(G.wave_ctr + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
-        }
+        atomicAdd(G.counters + GDX_CTR_CHECKS, (unsigned long long)n_viol);
+        atomicAdd(G.wave_ctr + GDX_WAVE_CHECKS, (unsigned long long)n_viol);
+        atomicAdd(G.counters + GDX_CTR_VIOLATIONS, (unsigned long long)n_viol);
+        atomicAdd(G.wave_ctr + GDX_WAVE_VIOLATIONS, (unsigned long long)n_viol);
     }
     __syncthreads();
     flush_audit(a, G.raw_audit);
@@ -1307,12 +1358,16 @@ __global__ __launch_bounds__(256) void k_queue_prefix(gdx_wave w, int *qprefix)
     }
 }
 
-// event chains (col_chain_run.cu:127-382): a persistent grid pulls the queued owners from one global
-// counter. Each K-lane segment of a wavefront runs one owner to its bath end, then takes the next;
-// the chain sums go to the owner's queue position for the audit pass.
+// event chains (col_chain_run.cu:127-382): a persistent grid over the queued owners. Each K-lane segment
+// of a wavefront runs one owner to its bath end, then takes the wavefront's next owner; the chain sums go
+// to the owner's queue position for the audit pass. The owners are dealt out without atomics: heavy
+// positions round-robin over the wavefronts (the long chains start together, one per wavefront), light
+// positions as one contiguous range per wavefront. Work counters shared by the whole grid cost one
+// same-address device atomic per refill, and those serialise: with thousands of wavefronts the counter,
+// not the chains, set the launch's length.
 constexpr int CHAIN_PREFIX_LDS = 4096;     // due groups whose queue prefix a chain block keeps in LDS
 
-__global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
+__global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
 {
     __shared__ double walk[WAVES][NSEG][N_K];
     __shared__ int qp[2*(CHAIN_PREFIX_LDS + 1)];
@@ -1344,7 +1399,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
     bool first_event = false;
     bool heavy_owner = false;
     bool moments = true;
-    bool more = true, active = false;
+    bool active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
     double end = 0.0, numr_i = 0.0, time = 0.0, hazard = 0.0, j1 = 0.0, j2 = 0.0, jmax = 0.0;
     uint64_t ctr = 0, seed = 0;
@@ -1387,44 +1442,26 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
         n_events = n_chain = n_pairs = n_blocks = 0;
         longest = 0;
     };
-    // the wavefront's current run of positions [cb, ce): heavy owners from work[0], only as many as the
-    // wavefront has free segments (so that long chains run side by side on many wavefronts), then light
-    // owners from work[1], CHAIN_CHUNK at a time so that the counter sees one atomic per CHAIN_CHUNK owners
-    int cb = 0, ce = 0;
-    bool hmore = heavy > 0;
+    // this wavefront's owners: heavy positions hn, hn + nwaves, ... below heavy, then the light positions
+    // [lc, le) (wavefront-uniform cursors)
+    const int nwaves = int(gridDim.x)*WAVES;
+    const int light = total - heavy;
+    int hn = int(blockIdx.x)*WAVES + wave;
+    int lc = int((long long)light*hn / nwaves), le = int((long long)light*(hn + 1) / nwaves);
     while (true)
     {
-        unsigned long long need = __ballot(!active) & seg_lead;
-        if (need && cb >= ce && more)
+        const unsigned long long need = __ballot(!active) & seg_lead;
+        if (need)
         {
-            int base = 0;
-            if (hmore)
+            // free segments, in lane order, take the next heavy positions, then the next light ones
+            const int k = __popcll(need);
+            const int th = min(k, hn < heavy ? (heavy - 1 - hn) / nwaves + 1 : 0);
+            const int tl = min(k - th, le - lc);
+            const int rank = __popcll(need & ((1ull << segbase) - 1ull));
+            if (!active && rank < th + tl)
             {
-                const int k = __popcll(need);
-                if (lane == 0) base = atomicAdd(work, k);
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (base < heavy) { cb = base; ce = min(base + k, heavy); }
-                else hmore = false;
-            }
-            if (!hmore)
-            {
-                const int light = total - heavy;
-                if (lane == 0) base = atomicAdd(work + 1, CHAIN_CHUNK);
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (base >= light) { more = false; base = light; }
-                cb = heavy + base;
-                ce = heavy + min(base + CHAIN_CHUNK, light);
-            }
-        }
-        if (need && cb < ce)
-        {
-            int n = min(__popcll(need), ce - cb);
-            int rank = __popcll(need & ((1ull << segbase) - 1ull));
-            if (!active && rank < n)
-            {
-                int v = cb + rank;
-                const bool hv = v < heavy;
-                if (!hv) v -= heavy;
+                const bool hv = rank < th;
+                const int v = hv ? hn + rank*nwaves : lc + (rank - th);
                 const int *prefix = hv ? hprefix : lprefix;
                 // due group of run position v: last entry with prefix <= v
                 int lo = 0, hi = w.n - 1;
@@ -1476,13 +1513,11 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w, int *work)
                 heavy_owner = hv;
                 active = true;
             }
-            cb += n;
+            hn += th*nwaves;
+            lc += tl;
         }
-        if (!__ballot(active))
-        {
-            if (cb >= ce && !more) break;
-            continue;
-        }
+        // no segment running after the refill: the wavefront's positions are done
+        if (!__ballot(active)) break;
         // a wavefront running a heavy owner's chain issues ahead of its SIMD's other wavefronts: the
         // longest chains set the launch's length
         if (__ballot(active && heavy_owner)) __builtin_amdgcn_s_setprio(2);
@@ -1760,9 +1795,12 @@ void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng)
 void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb)
 {
     if (!blocks) return;
-    rb.changed.alloc(size_t(std::max(1, w.total)));
-    hip_check(hipMemsetAsync(rb.nchanged.p, 0, sizeof(int), st), "zero changed count");
-    k_publish<<<xcd_grid(blocks), BLOCK, 0, st>>>(w, rb.changed.p, rb.nchanged.p);
+    // each list holds the owners of the blocks on its XCD: grid / CHANGED_LISTS blocks
+    const int grid = xcd_grid(blocks);
+    rb.changed_cap = grid / CHANGED_LISTS*PUBLISH_OWNERS;
+    rb.changed.alloc(size_t(grid)*PUBLISH_OWNERS);
+    hip_check(hipMemsetAsync(rb.nchanged.p, 0, sizeof(int)*CHANGED_LISTS, st), "zero changed counts");
+    k_publish<<<grid, BLOCK, 0, st>>>(w, rb.changed.p, rb.changed_cap, rb.nchanged.p);
     hip_check(hipGetLastError(), "launch publish");
 }
 
@@ -1772,7 +1810,7 @@ void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb)
     if (!w.total) return;
     constexpr int PER_BLOCK = WAVES*MARK_SLOTS;
     const int grid = std::max(1, std::min(4096, (w.total + PER_BLOCK - 1) / PER_BLOCK));
-    k_mark_stale<<<grid, BLOCK, 0, st>>>(w.table, rb.changed.p, rb.nchanged.p);
+    k_mark_stale<<<grid, BLOCK, 0, st>>>(w.table, rb.changed.p, rb.changed_cap, rb.nchanged.p);
     hip_check(hipGetLastError(), "launch mark stale");
 }
 
@@ -1883,11 +1921,11 @@ int chain_resident_blocks()
 }
 
 // a persistent grid of at most `resident` blocks, fewer for small waves
-void launch_chain(hipStream_t st, const gdx_wave &w, int *work, int resident)
+void launch_chain(hipStream_t st, const gdx_wave &w, int resident)
 {
     if (!w.total) return;
     int grid = std::max(1, std::min(resident, (w.total + 4*NSEG - 1) / (4*NSEG)));
-    k_chain<<<grid, BLOCK, 0, st>>>(w, work);
+    k_chain<<<grid, BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch chain");
 }
 
