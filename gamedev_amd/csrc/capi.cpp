@@ -139,7 +139,18 @@ struct phase_wave
     int blocks1024 = 0, blocks256 = 0, blocks_sub = 0, blocks_maj = 0;
 };
 
-phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipStream_t st)
+// the host's wait for the end of a phase: polling the stream rather than sleeping in the runtime's
+// wait, since the next phase's launches depend on what the host reads back (every microsecond of
+// wake-up is a microsecond of idle GPU, several times per operator call)
+void wait_stream(hipStream_t st, const char *what)
+{
+    hipError_t e;
+    while ((e = hipStreamQuery(st)) == hipErrorNotReady) {}
+    hip_check(e, what);
+}
+
+// fills the phase's due list and stages it on the device together with clearing the ranges `clear`
+phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipStream_t st, const zero_ranges &clear)
 {
     const int n = int(e.size());
     rb.reserve(n);
@@ -170,9 +181,8 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
     b256[n] = y;
     bsub[n] = z;
     bmaj[n] = u;
-    // ints and bath ends in one copy
-    hip_check(hipMemcpyAsync(rb.due_d.p, rb.due_h.p, rb.due_ints_bytes + sizeof(double)*std::max(1, n),
-                             hipMemcpyHostToDevice, st), "due list");
+    // ints and bath ends in one launch
+    launch_stage(st, clear, rb.due_d.p, rb.due_h.p, rb.due_ints_bytes + sizeof(double)*std::max(1, n));
     phase_wave pw;
     const int *d = rb.d_int.p;
     auto wave = [&](const int *bstart) {
@@ -698,8 +708,8 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             zero_ranges z;
             z.add(rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng);
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
-            launch_zero(st, z);
-            phase_wave pw = upload_phase(rb, e, st);
+            z.add(rb.nchanged.p, sizeof(int)*CHANGED_LISTS);
+            phase_wave pw = upload_phase(rb, e, st, z);
             timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024, rb); });
             if (track && pw.blocks1024)
                 timer.time(GDX_COL_KERNEL_STALE, pw.w256.total, [&] { launch_mark_stale(st, pw.w1024, rb); });
@@ -707,7 +717,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
                 timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
             timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.wsub, pw.blocks_sub); });
             rb.read_back(rb.off_rate, rb.off_err, st);             // rate bins and wave counters
-            hip_check(hipStreamSynchronize(st), "rating pass");
+            wait_stream(st, "rating pass");
             for (op_run *r : now) r->after_rate();
             upload_changed(now);
         }
@@ -726,8 +736,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.hcount.p, sizeof(int)*ng);
             z.add(rb.ccount.p, sizeof(int)*ng);
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
-            launch_zero(st, z);
-            phase_wave pw = upload_phase(rb, e, st);
+            phase_wave pw = upload_phase(rb, e, st, z);
             timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w1024, pw.blocks1024); });
             if (pw.w256.learned && pw.blocks256)
                 timer.time(GDX_COL_KERNEL_CANDIDATES, pw.w256.total, [&] { launch_candidates(st, pw.w256, pw.blocks256, rb.cprefix.p); });
@@ -736,7 +745,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.resident); });
             timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w256, pw.blocks256); });
             rb.read_back(rb.off_wave, rb.back_bytes, st);          // wave counters, errors, audit bins
-            hip_check(hipStreamSynchronize(st), "chain pass");
+            wait_stream(st, "chain pass");
             for (op_run *r : now) r->after_chain();
             upload_changed(now);
         }
@@ -1320,8 +1329,10 @@ int gdx_col_rate_pass(gdx_col_handle h)
         for (int g = 0; g < LOCAL_GROUPS; ++g)
             if (c.group_cnt[g] > 0) e.push_back(phase_entry{g, c.group_beg[g], c.group_cnt[g], 0.0, -1});
         launch_reset_scratch(c.stream, rb.table.p, LOCAL_GROUPS);
-        hip_check(hipMemsetAsync(rb.raw_rate.p, 0, sizeof(gdx_raw_rate)*RAW_BINS, c.stream), "zero rate bins");
-        phase_wave pw = upload_phase(rb, e, c.stream);
+        zero_ranges z;
+        z.add(rb.raw_rate.p, sizeof(gdx_raw_rate)*RAW_BINS);
+        z.add(rb.nchanged.p, sizeof(int)*CHANGED_LISTS);
+        phase_wave pw = upload_phase(rb, e, c.stream, z);
         launch_publish(c.stream, pw.w1024, pw.blocks1024, rb);
         if (c.model && c.rev_valid && pw.blocks1024) launch_mark_stale(c.stream, pw.w1024, rb);
         launch_bath_rate(c.stream, pw.wsub, pw.blocks_sub);

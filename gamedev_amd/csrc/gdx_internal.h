@@ -368,6 +368,10 @@ void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks, int *vstar
 void launch_queue_prefix(hipStream_t st, const gdx_wave &w, int *qprefix);
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_zero(hipStream_t st, const zero_ranges &z);
+// the ranges cleared and `bytes` of host data (multiple of 4) written to dst by one launch: the data ride
+// in the kernel's arguments when they fit (STAGE_INLINE_BYTES), else a zero launch and a copy
+constexpr int STAGE_INLINE_BYTES = 3072;
+void launch_stage(hipStream_t st, const zero_ranges &z, void *dst, const void *src, size_t bytes);
 void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, double *rate);
 // ---- majorant.hip ---------------------------------------------------------------------------
 const void *majorant_pack(gdx_col_majorant m);

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 
 #pragma clang fp contract(off)
 
@@ -1791,7 +1792,8 @@ void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng)
     hip_check(hipGetLastError(), "launch reset scratch");
 }
 
-// mark: some swarm of the wave tracks stale bounds (has a majorant)
+// mark: some swarm of the wave tracks stale bounds (has a majorant); the changed-list counts
+// (rb.nchanged) are cleared by the caller
 void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb)
 {
     if (!blocks) return;
@@ -1799,7 +1801,6 @@ void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &
     const int grid = xcd_grid(blocks);
     rb.changed_cap = grid / CHANGED_LISTS*PUBLISH_OWNERS;
     rb.changed.alloc(size_t(grid)*PUBLISH_OWNERS);
-    hip_check(hipMemsetAsync(rb.nchanged.p, 0, sizeof(int)*CHANGED_LISTS, st), "zero changed counts");
     k_publish<<<grid, BLOCK, 0, st>>>(w, rb.changed.p, rb.changed_cap, rb.nchanged.p);
     hip_check(hipGetLastError(), "launch publish");
 }
@@ -1871,6 +1872,47 @@ void launch_zero(hipStream_t st, const zero_ranges &z)
     const int blocks = int(std::min<unsigned long long>(256, (words + 255) / 256));
     k_zero<<<blocks, 256, 0, st>>>(z);
     hip_check(hipGetLastError(), "launch zero");
+}
+
+// a phase's due list (ints and bath ends) in the launch's arguments, written by the first block
+struct stage_data
+{
+    unsigned int *dst;
+    int words;
+    unsigned int data[STAGE_INLINE_BYTES / 4];
+};
+
+__global__ __launch_bounds__(256) void k_stage(zero_ranges z, stage_data d)
+{
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < d.words; i += blockDim.x) d.dst[i] = d.data[i];
+    for (int r = 0; r < z.n; ++r)
+    {
+        unsigned int *p = static_cast<unsigned int *>(z.p[r]);
+        const unsigned long long words = z.bytes[r] / 4;
+        for (unsigned long long i = (unsigned long long)blockIdx.x*blockDim.x + threadIdx.x; i < words;
+             i += (unsigned long long)gridDim.x*blockDim.x)
+            p[i] = 0u;
+    }
+}
+
+void launch_stage(hipStream_t st, const zero_ranges &z, void *dst, const void *src, size_t bytes)
+{
+    if (bytes > size_t(STAGE_INLINE_BYTES) || bytes % 4)
+    {
+        launch_zero(st, z);
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "due list");
+        return;
+    }
+    stage_data d;
+    d.dst = static_cast<unsigned int *>(dst);
+    d.words = int(bytes / 4);
+    std::memcpy(d.data, src, bytes);
+    unsigned long long words = 0;
+    for (int r = 0; r < z.n; ++r) words = std::max(words, z.bytes[r] / 4);
+    const int blocks = int(std::max<unsigned long long>(1, std::min<unsigned long long>(256, (words + 255) / 256)));
+    k_stage<<<blocks, 256, 0, st>>>(z, d);
+    hip_check(hipGetLastError(), "launch stage");
 }
 
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks)
