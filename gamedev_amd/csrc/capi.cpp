@@ -131,12 +131,13 @@ struct phase_entry
     int sub;              // learned swarm: exact-subsample phase of this bath; -1 plain
 };
 
-// due entries of one phase in device memory; launch shapes: 1024 and 256 owners per block, the
-// exactly rated owners (all, or a learned subsample) and the learned groups' owners, 256 per block
+// due entries of one phase in device memory; launch shapes: 1024 and 256 owners per block, the plain
+// groups' owners (256 per block), the learned groups' owners (MAJ_SLOTS per block) and the learned
+// groups' exact subsamples (RATE_SUB_OWNERS per block)
 struct phase_wave
 {
-    gdx_wave w1024, w256, wsub, wmaj;
-    int blocks1024 = 0, blocks256 = 0, blocks_sub = 0, blocks_maj = 0;
+    gdx_wave w1024, w256, wsub, wmaj, wlsub;
+    int blocks1024 = 0, blocks256 = 0, blocks_sub = 0, blocks_maj = 0, blocks_lsub = 0;
 };
 
 // the host's wait for the end of a phase: polling the stream rather than sleeping in the runtime's
@@ -155,8 +156,8 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
     const int n = int(e.size());
     rb.reserve(n);
     int *ref = rb.h_int, *beg = ref + n, *cnt = beg + n, *sub = cnt + n, *b1024 = sub + n, *b256 = b1024 + n + 1;
-    int *bsub = b256 + n + 1, *bmaj = bsub + n + 1;
-    int total = 0, x = 0, y = 0, z = 0, u = 0, learned = 0;
+    int *bsub = b256 + n + 1, *bmaj = bsub + n + 1, *blsub = bmaj + n + 1;
+    int total = 0, x = 0, y = 0, z = 0, u = 0, v = 0, learned = 0;
     for (int i = 0; i < n; ++i)
     {
         ref[i] = e[i].ref;
@@ -168,11 +169,14 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
         b256[i] = y;
         bsub[i] = z;
         bmaj[i] = u;
+        blsub[i] = v;
         x += (e[i].cnt + 1023) / 1024;
         y += (e[i].cnt + 255) / 256;
-        // the same count as rated_count on the device
-        int rated = e[i].sub < 0 ? e[i].cnt : thin_rated(e[i].cnt, e[i].sub, rb.h_table[size_t(e[i].ref)].thin_stride);
-        z += (rated + 255) / 256;
+        // plain groups: every owner, 256 per block; learned groups: the exact subsample (runs sub, sub +
+        // stride, ... of THIN_RUN slots, as exact_owner on the device), RATE_SUB_OWNERS per block
+        if (e[i].sub < 0) z += (e[i].cnt + 255) / 256;
+        else v += (thin_rated(e[i].cnt, e[i].sub, rb.h_table[size_t(e[i].ref)].thin_stride) + RATE_SUB_OWNERS - 1)
+                  / RATE_SUB_OWNERS;
         u += e[i].sub < 0 ? 0 : (e[i].cnt + MAJ_SLOTS - 1) / MAJ_SLOTS;
         learned |= e[i].sub >= 0;
         total += e[i].cnt;
@@ -181,6 +185,7 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
     b256[n] = y;
     bsub[n] = z;
     bmaj[n] = u;
+    blsub[n] = v;
     // ints and bath ends in one launch
     launch_stage(st, clear, rb.due_d.p, rb.due_h.p, rb.due_ints_bytes + sizeof(double)*std::max(1, n));
     phase_wave pw;
@@ -192,10 +197,12 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
     pw.w256 = wave(d + (b256 - ref));
     pw.wsub = wave(d + (bsub - ref));
     pw.wmaj = wave(d + (bmaj - ref));
+    pw.wlsub = wave(d + (blsub - ref));
     pw.blocks1024 = x;
     pw.blocks256 = y;
     pw.blocks_sub = z;
     pw.blocks_maj = u;
+    pw.blocks_lsub = v;
     return pw;
 }
 
@@ -715,7 +722,10 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
                 timer.time(GDX_COL_KERNEL_STALE, pw.w256.total, [&] { launch_mark_stale(st, pw.w1024, rb); });
             if (pw.blocks_maj > 0)
                 timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
-            timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] { launch_bath_rate(st, pw.wsub, pw.blocks_sub); });
+            timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] {
+                launch_bath_rate(st, pw.wsub, pw.blocks_sub);
+                launch_bath_rate_sub(st, pw.wlsub, pw.blocks_lsub);
+            });
             rb.read_back(rb.off_rate, rb.off_err, st);             // rate bins and wave counters
             wait_stream(st, "rating pass");
             for (op_run *r : now) r->after_rate();

@@ -138,9 +138,9 @@ struct view
 // the chain's queue prefix
 struct run_buffers
 {
-    // due-list layout per phase: ref, beg, cnt, sub (n each), then four block-start arrays (n + 1 each),
+    // due-list layout per phase: ref, beg, cnt, sub (n each), then five block-start arrays (n + 1 each),
     // then the bath ends (doubles): one page-locked block, one device block, one copy per phase
-    static constexpr int DUE_INTS = 8, DUE_EXTRA = 4;
+    static constexpr int DUE_INTS = 9, DUE_EXTRA = 5;
     dbuf<gdx_gref> table;
     std::vector<gdx_gref> h_table;
     pinned<unsigned char> due_h;
@@ -362,7 +362,11 @@ void kd_query(context &c);
 void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng);
 void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb);
 void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb);
+// plain swarms' groups (block starts of 256 owners), learned swarms' exact subsamples (block starts of
+// RATE_SUB_OWNERS owners)
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
+constexpr int RATE_SUB_OWNERS = 64;
+void launch_bath_rate_sub(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_screen(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks, int *vstart);
 void launch_queue_prefix(hipStream_t st, const gdx_wave &w, int *qprefix);

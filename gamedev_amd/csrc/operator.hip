@@ -63,12 +63,6 @@ __device__ __forceinline__ bool due_block(const gdx_wave &w, int per_block, int 
 
 // owners of due group gi rated with the exact K-sum in this bath: all of a plain swarm's, one run of
 // THIN_RUN slots in the swarm's thin_stride runs of a learned swarm's (run sub, sub + stride, ...)
-__device__ __forceinline__ int rated_count(const gdx_wave &w, int gi)
-{
-    const int sub = w.learned ? w.sub[gi] : -1;
-    const int cnt = w.cnt[gi];
-    return sub < 0 ? cnt : thin_rated(cnt, sub, w.table[w.ref[gi]].thin_stride);
-}
 __device__ __forceinline__ bool exact_owner(const gdx_wave &w, int gi, int s)
 {
     if (!w.learned) return true;
@@ -268,12 +262,27 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
 // lanes busy.
 constexpr int MARK_SLOTS = 16;
 
+// a listed slot as the pair loop reads it: its swarm's arrays (the same in every group entry of the
+// swarm) and its fresh and replaced records, gathered once by the slot's lane
+struct mark_slot
+{
+    const int *rev;
+    const double *measure;
+    const unsigned char *stale;
+    const gdx_env *env;
+    const double2 *st;
+    const double *hgs;
+    double *lam_add;
+    double lambda_0;
+    double size_new, numr_new, hg, size_old, numr_old;
+};
+
 __global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, const gdx_changed *changed, int cap,
                                                      const int *nchanged)
 {
     __shared__ int incl[WAVES][MARK_SLOTS];
     __shared__ unsigned first[WAVES][MARK_SLOTS];
-    __shared__ gdx_changed slot_of[WAVES][MARK_SLOTS];
+    __shared__ mark_slot info[WAVES][MARK_SLOTS];
     // the publish's lists, taken one after the other: entry v of the whole is entry v - start[l] of list l
     int start[CHANGED_LISTS + 1];
     start[0] = 0;
@@ -287,8 +296,8 @@ __global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, con
         int cnt = 0;
         if (lane < MARK_SLOTS)
         {
-            gdx_changed c{0, 0, 0.0, 0.0};
             unsigned b = 0;
+            mark_slot ms{};
             if (v0 + lane < n)
             {
                 const int v = v0 + lane;
@@ -298,7 +307,7 @@ __global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, con
                 int at = v;
 #pragma unroll
                 for (int k = 0; k < CHANGED_LISTS; ++k) at = l == k ? v - start[k] : at;
-                c = changed[(long long)l*cap + at];
+                const gdx_changed c = changed[(long long)l*cap + at];
                 const gdx_gref &G = table[c.ref];
                 G.stale[c.slot] = 1;
                 if (G.bound_reuse)
@@ -306,8 +315,11 @@ __global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, con
                     b = G.rev_beg[c.slot];
                     cnt = int(G.rev_end[c.slot] - b);
                 }
+                const partner fresh = load_partner(c.slot, G.res, G.hgs);
+                ms = mark_slot{G.rev, G.measure, G.stale, G.env, G.st, G.hgs, G.lam_add, G.lambda_0,
+                               fresh.size_j, fresh.numr_j, fresh.hg_j, c.size, c.numr};
             }
-            slot_of[wv][lane] = c;
+            info[wv][lane] = ms;
             first[wv][lane] = b;
         }
         // inclusive prefix of the reader counts over the MARK_SLOTS lanes (every lane shuffles)
@@ -327,22 +339,22 @@ __global__ __launch_bounds__(BLOCK) void k_mark_stale(const gdx_gref *table, con
             int e = 0;
 #pragma unroll
             for (int k = 0; k < MARK_SLOTS - 1; ++k) e += incl[wv][k] <= p;
-            const gdx_changed c = slot_of[wv][e];
-            const gdx_gref &G = table[c.ref];
-            const int o = G.rev[first[wv][e] + unsigned(p - (e ? incl[wv][e - 1] : 0))];
-            const double m = G.measure[o];
-            if (!(m > 0.0) || G.stale[o]) continue;      // a stale bound is evaluated again anyway
-            partner fresh = load_partner(c.slot, G.res, G.hgs);
-            partner before = fresh;
-            before.size_j = c.size;
-            before.numr_j = c.numr;
+            const mark_slot &I = info[wv][e];
+            const int o = I.rev[first[wv][e] + unsigned(p - (e ? incl[wv][e - 1] : 0))];
+            // the reader's record and flags in one round of loads
+            const double m = I.measure[o];
+            const bool stale = I.stale[o];
+            const double size_o = I.st[o].x;
             gdx_env env{};
             double hg_o = 0.0;
-            if constexpr (COAG_KERNEL == 3) env = G.env[o];
-            if constexpr (PARTNER_HG) hg_o = G.hgs[o];
-            const owner_terms t = make_owner(env, G.st[o].x, m, hg_o);
-            const double up = eval_pair_rate(t, fresh, G.lambda_0).rate - eval_pair_rate(t, before, G.lambda_0).rate;
-            if (up > 0.0 && isfinite(up)) atomicAdd(G.lam_add + o, up);
+            if constexpr (COAG_KERNEL == 3) env = I.env[o];
+            if constexpr (PARTNER_HG) hg_o = I.hgs[o];
+            if (!(m > 0.0) || stale) continue;      // a stale bound is evaluated again anyway
+            const partner fresh{I.size_new, I.numr_new, I.hg, true};
+            const partner before{I.size_old, I.numr_old, I.hg, true};
+            const owner_terms t = make_owner(env, size_o, m, hg_o);
+            const double up = eval_pair_rate(t, fresh, I.lambda_0).rate - eval_pair_rate(t, before, I.lambda_0).rate;
+            if (up > 0.0 && isfinite(up)) atomicAdd(I.lam_add + o, up);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -450,9 +462,116 @@ __device__ __forceinline__ rate_sums owner_rate_sums(const gdx_gref *GP, int s, 
     return rate_sums{total, JM, V};
 }
 
-template <bool SUB>
+// owner_rate_sums over a group of G consecutive lanes: lane g of the group streams the row positions
+// [g*N_K/G, (g+1)*N_K/G) with the same chunk partials and carry stack, and the lanes' partials combine
+// by xor shuffles (lane g with g ^ 1, then g ^ 2, ...), which are the upper levels of the same fold: the
+// sums equal owner_rate_sums bit for bit, for a G-th of the serial pair evaluations per lane. The
+// group's lanes share `own`, and all of them must be active (the shuffles read the group's lanes);
+// chunks (optional) receives the rate partials of the lane's own chunks.
+template <int G, bool MOMENTS = true>
+__device__ __forceinline__ rate_sums group_rate_sums(const gdx_gref *GP, int s, bool own, double2 me, double m, int g,
+                                                     double *cat3, double *chunks = nullptr)
+{
+    constexpr int P = N_K / G;                 // positions per lane
+    constexpr int CP = P / 4;                  // chunks per lane
+    constexpr int LP = (P == 4) ? 2 : (P == 8) ? 3 : (P == 16) ? 4 : 5;
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8, "lane groups of 1, 2, 4 or 8");
+    static_assert(P >= 4 && (1 << LP) == P, "at least one chunk of four positions per lane");
+    const int *nbr = GP->nbr;
+    const double2 *res = GP->res;
+    const gdx_env *env = GP->env;
+    const double *hgs = GP->hgs;
+    const double lambda_0 = GP->lambda_0;
+    gdx_env e{};
+    double hg_i = 0.0;
+    if constexpr (COAG_KERNEL == 3) { if (own) e = env[s]; }
+    if constexpr (PARTNER_HG) { if (own) hg_i = hgs[s]; }
+    const owner_terms t = make_owner(e, me.x, m, hg_i);
+    if constexpr (MOMENTS)
+    {
+        cat_moments c = catastrophic_moments(me.x);
+        cat3[0] = c.mean;
+        cat3[1] = c.second;
+        cat3[2] = c.maximum;
+    }
+    const cat_moments_lds cm{cat3};
+    auto eval = [&](const partner &q) {
+        if constexpr (MOMENTS) return eval_pair_terms(t, cm, q, lambda_0);
+        else return eval_pair_rate(t, q, lambda_0);
+    };
+    const bool live = own && m > 0.0;
+    const int4 *row = reinterpret_cast<const int4 *>(nbr + (long long)(own ? s : 0)*N_K) + g*CP;
+
+    fold3 stack[LP > 2 ? LP : 3];
+    fold3 total{0.0, 0.0, 0.0};
+    double JM = 0.0;
+    bool V = true;
+    int4 jj = live ? row[0] : make_int4(-1, -1, -1, -1);
+    for (int c = 0; c < CP; ++c)
+    {
+        int4 jn = (live && c + 1 < CP) ? row[c + 1] : make_int4(-1, -1, -1, -1);
+        const partner q0 = load_partner(jj.x, res, hgs), q1 = load_partner(jj.y, res, hgs);
+        const partner q2 = load_partner(jj.z, res, hgs), q3 = load_partner(jj.w, res, hgs);
+        pair_term p = eval(q0);
+        fold3 a{p.rate, p.j1, p.j2};
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        p = eval(q1);
+        a = add3(a, {p.rate, p.j1, p.j2});
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        p = eval(q2);
+        fold3 b{p.rate, p.j1, p.j2};
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        p = eval(q3);
+        b = add3(b, {p.rate, p.j1, p.j2});
+        V = V && isfinite(p.rate) && p.rate >= 0.0;
+        JM = fmax(JM, p.jmax);
+        fold3 part = add3(a, b);
+        if (chunks) chunks[c] = part.r;
+        bool placed = false;
+#pragma unroll
+        for (int lv = 2; lv < LP; ++lv)
+        {
+            if (!placed)
+            {
+                if (((c >> (lv - 2)) & 1) == 0) { stack[lv] = part; placed = true; }
+                else part = add3(stack[lv], part);
+            }
+        }
+        if (!placed) total = part;
+        jj = jn;
+    }
+    // the levels above a lane's positions: partner lanes within the group
+#pragma unroll
+    for (int mask = 1; mask < G; mask <<= 1)
+    {
+        total.r += __shfl_xor(total.r, mask, WAVE);
+        if constexpr (MOMENTS)
+        {
+            total.a += __shfl_xor(total.a, mask, WAVE);
+            total.b += __shfl_xor(total.b, mask, WAVE);
+            JM = fmax(JM, __shfl_xor(JM, mask, WAVE));
+        }
+        V = V && __shfl_xor(int(V), mask, WAVE);
+    }
+    (void)g;
+    return rate_sums{total, JM, V};
+}
+
+// LANES = 1: the owners of plain swarms' groups, one lane each. LANES = RATE_LANES: the exact subsample
+// of learned swarms' groups, a lane group per owner (the subsample is a small fraction of the due
+// owners, too few wavefronts to hide the serial pair loop of one lane per owner). Each launch has its
+// own block starts, so that a group is rated by exactly one of the two.
+constexpr int RATE_LANES = 4;
+static_assert(BLOCK / RATE_LANES == RATE_SUB_OWNERS, "host block starts of the subsample launch");
+
+template <int LANES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_wave w)
 {
+    constexpr bool SUB = LANES > 1;
+    constexpr int OWNERS = BLOCK / LANES;
     __shared__ double accd[COL_BIN_S][4];     // mass, weighted rate, change, second
     __shared__ int acci[COL_BIN_S][3];        // count, owner count, invalid count
     // blocks cover the group's rated owners (every owner, or a learned swarm's subsample)
@@ -469,7 +588,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         }
         gi = lo;
     }
-    const int i = (b - w.bstart[gi])*RATE_OWNERS + int(threadIdx.x);
+    const int lane = threadIdx.x % WAVE;
+    const int g = lane % LANES;                         // lane within the owner's group
+    const int i = (b - w.bstart[gi])*OWNERS + int(threadIdx.x) / LANES;
     const int sub = SUB ? w.sub[gi] : -1;
     const gdx_gref *GP = w.table + w.ref[gi];
     const double2 *st = GP->st;
@@ -480,19 +601,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     if (threadIdx.x < COL_BIN_S*3) (&acci[0][0])[threadIdx.x] = 0;
     if (threadIdx.x < 2) checked[threadIdx.x] = 0u;
     __syncthreads();
-    const int lane = threadIdx.x % WAVE;
-    const int s = w.beg[gi] + ((!SUB || sub < 0) ? i : ((i / THIN_RUN)*GP->thin_stride + sub)*THIN_RUN + i % THIN_RUN);
+    const int s = w.beg[gi] + (sub < 0 ? i : ((i / THIN_RUN)*GP->thin_stride + sub)*THIN_RUN + i % THIN_RUN);
     const bool own = s < w.beg[gi] + w.cnt[gi];
     double2 me = own ? st[s] : make_double2(1.0, 0.0);
     const double m = own ? measure[s] : 0.0;
     __shared__ double cats[BLOCK][3];
     const bool live = own && m > 0.0;
-    const rate_sums rs = owner_rate_sums(GP, s, own, me, m, &cats[threadIdx.x][0]);
+    rate_sums rs;
+    if constexpr (LANES == 1) rs = owner_rate_sums(GP, s, own, me, m, &cats[threadIdx.x][0]);
+    else rs = group_rate_sums<LANES>(GP, s, own, me, m, g, &cats[threadIdx.x][0]);
+    // the owner's sums are on every lane of its group: its first lane writes and counts
+    const bool lead = own && g == 0;
     const fold3 total = rs.total;
     const double JM = rs.JM;
     const bool V = rs.V;
     const double R = total.r, J1 = total.a, J2 = total.b;
-    if (own) GP->cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
+    if (lead) GP->cached[s] = make_double4(V ? R : -1.0, J1, J2, JM);
 
     // moving bounds from this wave's extrema (col_size_bnds.cu:12-21); read after the pair loop so
     // they hold no registers during it
@@ -508,21 +632,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
 
     if constexpr (SUB)
     {
-        if (sub >= 0)
-        {
-            const bool check = own && live && V;
-            const bool viol = check && !(R <= GP->safety*GP->lam[s] + GP->lam_add[s]);
-            const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
-            if (lane == 0 && nc) atomicAdd(&checked[0], nc);
-            if (lane == 0 && nv) atomicAdd(&checked[1], nv);
-        }
+        const bool check = lead && live && V;
+        const bool viol = check && !(R <= GP->safety*GP->lam[s] + GP->lam_add[s]);
+        const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
+        if (lane == 0 && nc) atomicAdd(&checked[0], nc);
+        if (lane == 0 && nv) atomicAdd(&checked[1], nv);
     }
     // controller-bin sums: each owner adds to the block's LDS bins (native FP64 LDS atomics)
-    int sb = own ? size_bin(me.x, lo, up) : -1;
+    int sb = lead ? size_bin(me.x, lo, up) : -1;
     double weight = me.y*grain_mass(me.x);
     bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(R) || R < 0.0 || !isfinite(J1) || J1 < 0.0
         || !isfinite(J2) || J2 < 0.0;
-    if (own)
+    if (lead)
     {
         atomicAdd(&acci[sb][0], 1);
         atomicAdd(&acci[sb][1], 1);
@@ -1003,18 +1124,28 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     flush_audit(a, G.raw_audit);
 }
 
-// thinning candidates of learned swarms, one lane per candidate: the exact K-sum at the candidate time
-// (the owner and its frozen partners are unchanged since the bath start, so this is the bath-start
-// rate). Below the bound the candidate is an event with probability R / bound, and a rejected candidate
-// continues on R from the candidate time. Above the bound the exact process is the bound's process plus
-// an independent one of rate R - bound: the first event is the earlier of the candidate time and that
-// process's first point, so the owner's event time is still exactly distributed. The owner's first event (at the candidate time when accepted, else at the
-// end of its next wait if that falls inside the bath) is taken here, with the draws and operations of
-// the chain: the partner walk over the same rates in slot order, the outcome, then the K-sum at the new
-// size and the next wait. Owners whose next event still falls inside the bath enter the chain after
-// that first event, and the chain repeats the pending draw; any invalid rate or state sends the owner
-// to the chain from the candidate time with its draws uncommitted, and the chain reports it.
-// per due group of a learned swarm: its candidate blocks (ceil(candidates / CHAIN_OWNERS)), exclusive
+// thinning candidates of learned swarms, a group of CAND_LANES lanes per candidate (the K-sums spread
+// over the group's lanes, the chosen chunk's four pair rates shared): the exact K-sum at the
+// candidate time (the owner and its frozen partners are unchanged since the bath start, so this is the
+// bath-start rate). Below the bound the candidate is an event with probability R / bound, and a
+// rejected candidate continues on R from the candidate time. Above the bound the exact process is the
+// bound's process plus an independent one of rate R - bound: the first event is the earlier of the
+// candidate time and that process's first point, so the owner's event time is still exactly
+// distributed. The owner's first event (at the candidate time when accepted, else at the end of its next
+// wait if that falls inside the bath) is taken here, with the draws and operations of the chain: the
+// partner walk over the same rates, the outcome, then the K-sum at the new size and the next wait.
+// Owners whose next event still falls inside the bath enter the chain after that first event, and the
+// chain repeats the pending draw; any invalid rate or state sends the owner to the chain from the
+// candidate time with its draws uncommitted, and the chain reports it.
+// lanes per candidate. One: groups of 2 or 4 lanes (the K-sums spread over the group, the chosen
+// chunk's four rates shared) measured slower on the fiducial disk, since every lane of a group repeats
+// the candidate's draws and decisions and the extra wavefronts do not fit on the CUs at once, while one
+// lane per candidate already puts every wavefront of a launch in flight together.
+constexpr int CAND_LANES = 1;
+constexpr int CAND_OWNERS = BLOCK / CAND_LANES;
+static_assert(CAND_LANES == 1 || CAND_LANES == 2 || CAND_LANES == 4, "the lanes share a chunk's four rates");
+
+// per due group of a learned swarm: its candidate blocks (ceil(candidates / CAND_OWNERS)), exclusive
 // prefix in vstart[0..n], the total in vstart[n]
 __global__ __launch_bounds__(256) void k_cand_prefix(gdx_wave w, int *vstart)
 {
@@ -1022,7 +1153,7 @@ __global__ __launch_bounds__(256) void k_cand_prefix(gdx_wave w, int *vstart)
     const int t = threadIdx.x;
     const int per = (w.n + 255) / 256;
     auto blocks_of = [&](int i) {
-        return w.sub[i] < 0 ? 0 : (*w.table[w.ref[i]].ccount + CHAIN_OWNERS - 1) / CHAIN_OWNERS;
+        return w.sub[i] < 0 ? 0 : (*w.table[w.ref[i]].ccount + CAND_OWNERS - 1) / CAND_OWNERS;
     };
     int sum = 0;
     for (int i = t*per; i < min(w.n, (t + 1)*per); ++i) sum += blocks_of(i);
@@ -1050,14 +1181,18 @@ constexpr int CAND_GRID = 2048;
 
 __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vstart)
 {
+    constexpr int CHUNKS = N_K / 4;                    // chunk partials of a row
+    constexpr int CP = CHUNKS / CAND_LANES;            // chunks per lane
     // candidates, exact checks, violations, accepted, pair evaluations beyond the first K-sum, events
     // (= owners) finished here
     __shared__ unsigned tally[6];
     __shared__ int block_off[2][WAVES], block_tot[2], block_base[2];
-    __shared__ double chunk_rate[N_K / 4][BLOCK];
+    __shared__ double chunk_rate[CHUNKS][CAND_OWNERS];
     const int total = vstart[w.n];
     const int per = (total + XCDS - 1) / XCDS, xcd = int(blockIdx.x) % XCDS;
     const int local = int(blockIdx.x) / XCDS, stride = int(gridDim.x) / XCDS;
+    const int lane = threadIdx.x % WAVE;
+    const int g = lane % CAND_LANES, cand = int(threadIdx.x) / CAND_LANES, gbase = lane - g;
     for (int v = local; v < per && xcd*per + v < total; v += stride)
     {
     const int vb = xcd*per + v;
@@ -1075,21 +1210,25 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
     const gdx_gref *GP = w.table + w.ref[gi];
     const int beg = w.beg[gi];
     const int n_cand = *GP->ccount;
-    const int s0 = beg + (vb - vstart[gi])*CHAIN_OWNERS;
+    const int s0 = beg + (vb - vstart[gi])*CAND_OWNERS;
     if (threadIdx.x < 6) tally[threadIdx.x] = 0u;
     if (threadIdx.x < 2) block_tot[threadIdx.x] = 0;
     __syncthreads();
-    const int lane = threadIdx.x % WAVE;
-    const int pos = s0 + int(threadIdx.x);
+    const int pos = s0 + cand;
     const bool own = pos - beg < n_cand;
     const int s = own ? GP->cqueue[pos] : beg;
     const double2 me = own ? GP->st[s] : make_double2(1.0, 0.0);
     const double m = own ? GP->measure[s] : 0.0;
-    double chunks[N_K / 4];
-    const rate_sums rs = owner_rate_sums<false>(GP, s, own, me, m, nullptr, chunks);
+    double chunks[CP];
+    const rate_sums rs = group_rate_sums<CAND_LANES, false>(GP, s, own, me, m, g, nullptr, chunks);
     const double R = rs.total.r;
 #pragma unroll
-    for (int c = 0; c < N_K / 4; ++c) chunk_rate[c][threadIdx.x] = chunks[c];
+    for (int c = 0; c < CP; ++c) chunk_rate[g*CP + c][cand] = chunks[c];
+    // the walk below reads the chunk partials of the group's other lanes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // every lane of a candidate's group runs the same draws and decisions; its first lane writes
     bool chain = false, check = false, viol = false, accepted = false, heavy = false, finished = false;
     unsigned extra_pairs = 0;
     double2 qs = make_double2(0.0, 0.0);
@@ -1153,9 +1292,9 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
                 double before = 0.0;
                 int cc = -1, last_c = -1;
                 double last_before = 0.0;
-                for (int c = 0; c < N_K / 4; ++c)
+                for (int c = 0; c < CHUNKS; ++c)
                 {
-                    const double v = chunk_rate[c][threadIdx.x];
+                    const double v = chunk_rate[c][cand];
                     if (v > 0.0)
                     {
                         if (before + v > target) { cc = c; break; }
@@ -1169,27 +1308,36 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
                 int slot = -1;
                 if (cc >= 0)
                 {
+                    // the chunk's four pair rates: positions 4 cc + g, 4 cc + g + CAND_LANES, ... on lane g
                     gdx_env e{};
                     double hg_i = 0.0;
                     if constexpr (COAG_KERNEL == 3) e = GP->env[s];
                     if constexpr (PARTNER_HG) hg_i = GP->hgs[s];
                     const owner_terms ot = make_owner(e, me.x, m, hg_i);
-                    const int4 jq = reinterpret_cast<const int4 *>(GP->nbr + (long long)s*N_K)[cc];
-                    const int jr[4] = {jq.x, jq.y, jq.z, jq.w};
+                    pair_term mine[4 / CAND_LANES];
+#pragma unroll
+                    for (int j = 0; j < 4 / CAND_LANES; ++j)
+                    {
+                        const partner q = load_partner(GP->nbr[(long long)s*N_K + 4*cc + g + j*CAND_LANES], GP->res,
+                                                       GP->hgs);
+                        mine[j] = eval_pair_rate(ot, q, GP->lambda_0);
+                    }
                     double cumulative = before, last_size = 0.0, last_vrel = 0.0;
                     int last = -1;
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
                     {
-                        const partner q = load_partner(jr[r], GP->res, GP->hgs);
-                        const pair_term pt = eval_pair_rate(ot, q, GP->lambda_0);
-                        if (slot < 0 && pt.rate > 0.0)
+                        const pair_term &src = mine[r / CAND_LANES];
+                        const double rate = __shfl(src.rate, gbase + r % CAND_LANES, WAVE);
+                        const double sj = __shfl(src.size_j, gbase + r % CAND_LANES, WAVE);
+                        const double vr = __shfl(src.vrel, gbase + r % CAND_LANES, WAVE);
+                        if (slot < 0 && rate > 0.0)
                         {
-                            cumulative += pt.rate;
+                            cumulative += rate;
                             last = r;
-                            last_size = pt.size_j;
-                            last_vrel = pt.vrel;
-                            if (cumulative > target) { slot = r; size_j = pt.size_j; vrel = pt.vrel; }
+                            last_size = sj;
+                            last_vrel = vr;
+                            if (cumulative > target) { slot = r; size_j = sj; vrel = vr; }
                         }
                     }
                     extra_pairs += 4;
@@ -1210,8 +1358,8 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
                     if (ok)
                     {
                         // the K-sum at the new size and the next wait from the event
-                        const rate_sums r2 = owner_rate_sums<false>(GP, s, true, make_double2(size_new, numr_new), m,
-                                                                    nullptr);
+                        const rate_sums r2 = group_rate_sums<CAND_LANES, false>(
+                            GP, s, true, make_double2(size_new, numr_new), m, g, nullptr);
                         extra_pairs += N_K;
                         const double R2 = r2.total.r;
                         const double remaining = end - t_event;
@@ -1227,15 +1375,18 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
                                 if (!done) heavy = R2*remaining >= HEAVY_EVENTS;
                             }
                         }
-                        if (GP->evw)
+                        if (g == 0)
                         {
-                            double log_mass;
-                            const int cat = outcome_category(size_i, size_j, high_speed, sample, size_new, log_mass);
-                            GP->evw[s].count[cat] += 1ull;
-                            GP->evw[s].log_mass[cat] += log_mass;
+                            if (GP->evw)
+                            {
+                                double log_mass;
+                                const int cat = outcome_category(size_i, size_j, high_speed, sample, size_new, log_mass);
+                                GP->evw[s].count[cat] += 1ull;
+                                GP->evw[s].log_mass[cat] += log_mass;
+                            }
+                            GP->st[s] = make_double2(size_new, numr_new);
+                            GP->dirty[s] = 1;
                         }
-                        GP->st[s] = make_double2(size_new, numr_new);
-                        GP->dirty[s] = 1;
                         // finished: the wait draw is committed; else the chain continues after the event
                         // and repeats it
                         ctr = done ? k2 : k;
@@ -1246,11 +1397,12 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
                 }
             }
         }
-        GP->rng[s] = ctr;
+        if (g == 0) GP->rng[s] = ctr;
     }
-    // queue appends (heavy owners from the region's start, the others from its end): wavefront offsets
-    // from LDS, one returning global atomic per block and list
-    const unsigned long long bl = __ballot(chain && !heavy), bh = __ballot(chain && heavy);
+    // queue appends (heavy owners from the region's start, the others from its end) by each candidate's
+    // first lane: wavefront offsets from LDS, one returning global atomic per block and list
+    const bool lead = g == 0;
+    const unsigned long long bl = __ballot(lead && chain && !heavy), bh = __ballot(lead && chain && heavy);
     const int wv = threadIdx.x / WAVE;
     if (lane == 0)
     {
@@ -1261,7 +1413,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
     if (threadIdx.x == 0) block_base[0] = block_tot[0] ? atomicAdd(GP->qcount, block_tot[0]) : 0;
     if (threadIdx.x == WAVE) block_base[1] = block_tot[1] ? atomicAdd(GP->hcount, block_tot[1]) : 0;
     __syncthreads();
-    if (chain)
+    if (lead && chain)
     {
         const unsigned long long below = (1ull << lane) - 1ull;
         const int q = heavy ? beg + block_base[1] + block_off[1][wv] + __popcll(bh & below)
@@ -1269,10 +1421,10 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
         GP->queue[q] = s;
         GP->qstart[q] = qs;
     }
-    const unsigned nc = __popcll(__ballot(own)), nk = __popcll(__ballot(check));
-    const unsigned nv = __popcll(__ballot(viol)), na = __popcll(__ballot(accepted));
-    const unsigned nf = __popcll(__ballot(finished));
-    unsigned np = extra_pairs;
+    const unsigned nc = __popcll(__ballot(lead && own)), nk = __popcll(__ballot(lead && check));
+    const unsigned nv = __popcll(__ballot(lead && viol)), na = __popcll(__ballot(lead && accepted));
+    const unsigned nf = __popcll(__ballot(lead && finished));
+    unsigned np = lead ? extra_pairs : 0u;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) np += __shfl_xor(np, o, WAVE);
     if (lane == 0)
@@ -1316,7 +1468,7 @@ __global__ __launch_bounds__(BLOCK) void k_candidates(gdx_wave w, const int *vst
             atomicAdd(GP->wave_ctr + GDX_WAVE_CHAINED, (unsigned long long)tally[5]);
         }
     }
-    // the next candidate block resets the block's shared counters
+    // the next candidate block resets the block's shared counters and chunk partials
     __syncthreads();
     }
 }
@@ -1818,9 +1970,15 @@ void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb)
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks)
 {
     if (!blocks) return;
-    if (w.learned) k_bath_rate<true><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
-    else k_bath_rate<false><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    k_bath_rate<1><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch bath rate");
+}
+
+void launch_bath_rate_sub(hipStream_t st, const gdx_wave &w, int blocks)
+{
+    if (!blocks) return;
+    k_bath_rate<RATE_LANES><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    hip_check(hipGetLastError(), "launch subsample rate");
 }
 
 __global__ __launch_bounds__(256) void k_event_sum(const gdx_event_work *w, int n, gdx_event_work *sum)
