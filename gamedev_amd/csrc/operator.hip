@@ -197,17 +197,28 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
     if (threadIdx.x == 0) count = 0;
     __syncthreads();
     unsigned long long lo = ~0ull, hi = 0ull;
-    for (int s = s0 + threadIdx.x; s < s1; s += BLOCK)
+    // every round's record, flag and replaced record in one round of loads, before any store (a flat
+    // pointer could alias the LDS list, which would keep each round's loads behind the last store)
+    constexpr int ROUNDS = PUBLISH_OWNERS / BLOCK;
+    double2 vs[ROUNDS], rs[ROUNDS];
+    unsigned char ds[ROUNDS];
+#pragma unroll
+    for (int k = 0; k < ROUNDS; ++k)
     {
-        const double2 v = st[s];
-        const bool changed_here = dirty[s];
-        double was_x = v.x, was_y = v.y;
-        if (changed_here && track)
-        {
-            const double2 r = res[s];
-            was_x = r.x;
-            was_y = r.y;
-        }
+        const int s = s0 + k*BLOCK + int(threadIdx.x);
+        const int sv = s < s1 ? s : s0;
+        vs[k] = st[sv];
+        ds[k] = dirty[sv];
+        rs[k] = res[sv];
+    }
+#pragma unroll
+    for (int k = 0; k < ROUNDS; ++k)
+    {
+        const int s = s0 + k*BLOCK + int(threadIdx.x);
+        if (s >= s1) break;
+        const double2 v = vs[k];
+        const bool changed_here = ds[k];
+        const double was_x = changed_here ? rs[k].x : v.x, was_y = changed_here ? rs[k].y : v.y;
         res[s] = v;
         res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
         // positive doubles order like their bit patterns (col_size_scan.cu:20-24)
@@ -801,25 +812,42 @@ __global__ __launch_bounds__(BLOCK) void k_majorant(gdx_wave w)
     __shared__ float feat[BLOCK][mlp::F + 1];
     __shared__ float ys[BLOCK];
     __shared__ int list[MAJ_SLOTS];
-    __shared__ int count;
+    __shared__ int wave_tot[WAVES];
     int gi, s0, s1;
     if (!due_block(w, MAJ_SLOTS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
-    if (threadIdx.x == 0) count = 0;
-    __syncthreads();
     const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
-    for (int t = 0; t < MAJ_SLOTS; t += BLOCK)
+    // the stale slots in slot order: each thread reads the flags of SPAN consecutive slots in one round
+    // of loads (no LDS store in between, which a flat pointer could alias), then a block-wide prefix
+    // of the per-thread counts places them in the list
+    constexpr int SPAN = MAJ_SLOTS / BLOCK;
+    const int t0 = s0 + int(threadIdx.x)*SPAN;
+    const unsigned char *stale_flags = (G.bound_reuse && G.stale) ? G.stale : nullptr;
+    unsigned char flag[SPAN];
+#pragma unroll
+    for (int k = 0; k < SPAN; ++k) flag[k] = (t0 + k < s1) ? (stale_flags ? stale_flags[t0 + k] : 1) : 0;
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < SPAN; ++k) mine += flag[k] ? 1 : 0;
+    int incl = mine;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1)
     {
-        const int s = s0 + t + int(threadIdx.x);
-        const bool stale = s < s1 && (!G.bound_reuse || !G.stale || G.stale[s]);
-        const unsigned long long ball = __ballot(stale);
-        int b = 0;
-        if (lane == 0 && ball) b = atomicAdd(&count, __popcll(ball));
-        b = __shfl(b, 0, WAVE);
-        if (stale) list[b + __popcll(ball & ((1ull << lane) - 1ull))] = s;
+        const int v = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl += v;
     }
+    if (lane == WAVE - 1) wave_tot[wv] = incl;
     __syncthreads();
-    const int n = count;
+    int at = incl - mine, n = 0;
+    for (int k = 0; k < WAVES; ++k)
+    {
+        if (k < wv) at += wave_tot[k];
+        n += wave_tot[k];
+    }
+#pragma unroll
+    for (int k = 0; k < SPAN; ++k)
+        if (flag[k]) list[at++] = t0 + k;
+    __syncthreads();
     if (threadIdx.x == 0 && n > 0)
         atomicAdd(G.counters + GDX_CTR_BOUND_EVALS + blockIdx.x % GDX_CTR_SPREAD, (unsigned long long)n);
     const mlp::pack *P = static_cast<const mlp::pack *>(G.mlp);
@@ -965,11 +993,16 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     int gi, s0, s1;
     if (!due_block(w, SCREEN_OWNERS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
-    const double4 *cached = G.cached;
-    const double *measure = G.measure;
-    const double2 *res = G.res;
-    const int *slot_pidx = G.slot_pidx;
-    uint64_t *rng = G.rng;
+    // the group's arrays as distinct pointers, so that the rounds' loads can run ahead of the earlier
+    // rounds' counter stores
+    const double4 *__restrict__ cached = G.cached;
+    const double *__restrict__ measure = G.measure;
+    const double2 *__restrict__ res = G.res;
+    const int *__restrict__ slot_pidx = G.slot_pidx;
+    uint64_t *__restrict__ rng = G.rng;
+    const double *__restrict__ lam = G.lam;
+    const double *__restrict__ lam_add = G.lam_add;
+    const double safety = G.safety;
     const uint64_t seed = G.seed;
     const double end = w.dt[gi];
     const double lo = *G.lower, up = *G.upper;
@@ -987,6 +1020,28 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     int list[SCREEN_ROUNDS], off[SCREEN_ROUNDS];
     double2 cand[SCREEN_ROUNDS];
     unsigned n_viol = 0;
+    // every round's inputs first, in one round of loads without branches (a branch around a load makes
+    // the compiler wait for it there): an owner of the exact subsample (or of a plain swarm) reads its
+    // cached sums and its record, a thinned owner its bound (lam, lam_add); the loads an owner does not
+    // need read the block's first slot instead, one cache line for the whole block
+    double4 in[SCREEN_ROUNDS];
+    double2 bnd[SCREEN_ROUNDS], rec[SCREEN_ROUNDS];
+    double mes[SCREEN_ROUNDS];
+    uint64_t ctr0[SCREEN_ROUNDS];
+    int pid[SCREEN_ROUNDS];
+#pragma unroll
+    for (int k = 0; k < SCREEN_ROUNDS; ++k)
+    {
+        const int s = s0 + k*BLOCK + int(threadIdx.x);
+        const int sv = s < s1 ? s : s0;
+        const bool thinned = learned && !exact_owner(w, gi, sv);
+        mes[k] = measure[sv];
+        ctr0[k] = rng[sv];
+        pid[k] = slot_pidx[sv];
+        in[k] = cached[thinned ? s0 : sv];
+        rec[k] = res[thinned ? s0 : sv];
+        bnd[k] = make_double2(lam[thinned ? sv : s0], lam_add[thinned ? sv : s0]);
+    }
 #pragma unroll
     for (int k = 0; k < SCREEN_ROUNDS; ++k)
     {
@@ -1001,8 +1056,8 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         double2 old = make_double2(1.0, 0.0);
         if (valid && learned && !exact_owner(w, gi, s))
         {
-            const bool thin = measure[s] > 0.0;
-            const double L = thin ? G.safety*G.lam[s] + G.lam_add[s] : 0.0;
+            const bool thin = mes[k] > 0.0;
+            const double L = thin ? safety*bnd[k].x + bnd[k].y : 0.0;
             if (thin && (!isfinite(L) || L < 0.0))
             {
                 // unusable bound: the owner takes the plain chain from the bath start
@@ -1010,8 +1065,8 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
             }
             else if (thin && L > 0.0)
             {
-                uint64_t trial = rng[s];
-                const double t1 = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / L;
+                uint64_t trial = ctr0[k];
+                const double t1 = -log(col_uniform(uint32_t(pid[k]), seed, trial)) / L;
                 rng[s] = trial;
                 if (t1 < end)
                 {
@@ -1022,8 +1077,8 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         }
         else if (valid)
         {
-            double4 c = cached[s];
-            double m = measure[s];
+            const double4 c = in[k];
+            const double m = mes[k];
             if (!(m > 0.0))
             {
                 done = true;
@@ -1036,8 +1091,8 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
                 }
                 else
                 {
-                    uint64_t trial = rng[s];
-                    double wait = -log(col_uniform(uint32_t(slot_pidx[s]), seed, trial)) / c.x;
+                    uint64_t trial = ctr0[k];
+                    double wait = -log(col_uniform(uint32_t(pid[k]), seed, trial)) / c.x;
                     double remaining = end;
                     if (isfinite(wait) && wait > 0.0 && wait >= remaining)
                     {
@@ -1055,7 +1110,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
             heavy = chain && isfinite(c.x) && c.x*end >= HEAVY_EVENTS;
             if (done)
             {
-                old = res[s];
+                old = rec[k];
                 rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
             }
         }
