@@ -195,11 +195,9 @@ struct run_buffers
         counters.alloc(size_t(swarms)*GDX_CTRS);
         h_counters.resize(size_t(swarms)*GDX_CTRS);
     }
-    // device-to-host copy of the read-back bytes [from, to) (offsets above)
-    void read_back(size_t from, size_t to, hipStream_t st)
-    {
-        hip_check(hipMemcpyAsync(back_h.p + from, back_d.p + from, to - from, hipMemcpyDeviceToHost, st), "read back");
-    }
+    // the read-back bytes [from, to) (offsets above, multiples of 256) written into the page-locked mirror
+    // by a kernel on the stream (no copy engine to synchronise with)
+    void read_back(size_t from, size_t to, hipStream_t st);
     run_buffers() = default;
     run_buffers(const run_buffers &) = delete;
     run_buffers &operator=(const run_buffers &) = delete;
@@ -372,6 +370,8 @@ void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks, int *vstar
 void launch_queue_prefix(hipStream_t st, const gdx_wave &w, int *qprefix);
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);
 void launch_zero(hipStream_t st, const zero_ranges &z);
+// bytes (a multiple of 16) from device memory into page-locked host memory by a kernel on the stream
+void launch_to_host(hipStream_t st, void *host_dst, const void *src, size_t bytes);
 // the ranges cleared and `bytes` of host data (multiple of 4) written to dst by one launch: the data ride
 // in the kernel's arguments when they fit (STAGE_INLINE_BYTES), else a zero launch and a copy
 constexpr int STAGE_INLINE_BYTES = 3072;

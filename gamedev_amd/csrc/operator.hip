@@ -2087,6 +2087,25 @@ void launch_zero(hipStream_t st, const zero_ranges &z)
     hip_check(hipGetLastError(), "launch zero");
 }
 
+__global__ __launch_bounds__(256) void k_to_host(uint4 *dst, const uint4 *src, int n)
+{
+    for (int i = blockIdx.x*blockDim.x + threadIdx.x; i < n; i += gridDim.x*blockDim.x) dst[i] = src[i];
+}
+
+void launch_to_host(hipStream_t st, void *host_dst, const void *src, size_t bytes)
+{
+    const int n = int(bytes / 16);
+    if (!n) return;
+    k_to_host<<<std::min(16, (n + 255) / 256), 256, 0, st>>>(static_cast<uint4 *>(host_dst),
+                                                             static_cast<const uint4 *>(src), n);
+    hip_check(hipGetLastError(), "launch read-back");
+}
+
+void run_buffers::read_back(size_t from, size_t to, hipStream_t st)
+{
+    launch_to_host(st, back_h.p + from, back_d.p + from, to - from);
+}
+
 // a phase's due list (ints and bath ends) in the launch's arguments, written by the first block
 struct stage_data
 {
