@@ -1581,6 +1581,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
     __shared__ int qp[2*(CHAIN_PREFIX_LDS + 1)];
     __shared__ unsigned long long end_key, end_sum[5];
     __shared__ unsigned long long *end_wave;
+    __shared__ int lpool;                     // the block's next light position
     if (threadIdx.x == 0) end_key = 0ull;
     if (threadIdx.x < 5) end_sum[threadIdx.x] = 0ull;
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
@@ -1650,26 +1651,47 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
         n_events = n_chain = n_pairs = n_blocks = 0;
         longest = 0;
     };
-    // this wavefront's owners: heavy positions hn, hn + nwaves, ... below heavy, then the light positions
-    // [lc, le) (wavefront-uniform cursors)
+    // Owners are dealt without global atomics. Heavy positions: while they are at most half the
+    // wavefronts, heavy owner h runs alone on the first segment of wavefront h * (nwaves / heavy) (a light
+    // owner sharing its wavefront would stall the long chain with its loads and branches at every
+    // refill), else wavefront gw takes gw, gw + nwaves, ... on any free segment. Light positions: one
+    // contiguous range per block, pulled by the block's free segments through an LDS counter, so that
+    // the wavefronts of a block balance their light work among themselves (a wavefront that ran a solo
+    // heavy owner joins in once its chain is done).
     const int nwaves = int(gridDim.x)*WAVES;
+    const int gw = int(blockIdx.x)*WAVES + wave;
     const int light = total - heavy;
-    int hn = int(blockIdx.x)*WAVES + wave;
-    int lc = int((long long)light*hn / nwaves), le = int((long long)light*(hn + 1) / nwaves);
+    const bool solo = heavy <= nwaves / 2;
+    const int hstride = solo && heavy > 0 ? nwaves / heavy : nwaves;
+    int hn = solo ? ((gw % hstride == 0 && gw / hstride < heavy) ? gw / hstride : heavy) : gw;
+    const int hstep = solo ? heavy : nwaves;  // a solo wavefront's one heavy position, then none
+    const int lb1 = int((long long)light*(blockIdx.x + 1) / gridDim.x);
+    if (threadIdx.x == 0) lpool = int((long long)light*blockIdx.x / gridDim.x);
+    __syncthreads();
     while (true)
     {
-        const unsigned long long need = __ballot(!active) & seg_lead;
+        unsigned long long need = __ballot(!active) & seg_lead;
+        // a solo heavy owner runs alone: its wavefront takes nothing while it runs, and only that owner
+        // (on the first segment) at its first refill
+        if (solo && __ballot(active && heavy_owner)) need = 0ull;
+        if (solo && hn < heavy) need &= 1ull;
         if (need)
         {
             // free segments, in lane order, take the next heavy positions, then the next light ones
             const int k = __popcll(need);
-            const int th = min(k, hn < heavy ? (heavy - 1 - hn) / nwaves + 1 : 0);
-            const int tl = min(k - th, le - lc);
+            const int th = min(k, hn < heavy ? (heavy - 1 - hn) / hstep + 1 : 0);
+            int tl = 0, lc = 0;
+            if (k > th)
+            {
+                if (lane == 0) lc = atomicAdd(&lpool, k - th);
+                lc = __builtin_amdgcn_readfirstlane(lc);
+                tl = max(0, min(k - th, lb1 - lc));
+            }
             const int rank = __popcll(need & ((1ull << segbase) - 1ull));
             if (!active && rank < th + tl)
             {
                 const bool hv = rank < th;
-                const int v = hv ? hn + rank*nwaves : lc + (rank - th);
+                const int v = hv ? hn + rank*hstep : lc + (rank - th);
                 const int *prefix = hv ? hprefix : lprefix;
                 // due group of run position v: last entry with prefix <= v
                 int lo = 0, hi = w.n - 1;
@@ -1721,8 +1743,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                 heavy_owner = hv;
                 active = true;
             }
-            hn += th*nwaves;
-            lc += tl;
+            hn += th*hstep;
         }
         // no segment running after the refill: the wavefront's positions are done
         if (!__ballot(active)) break;
