@@ -753,7 +753,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             if (pw.blocks256)
                 timer.time(GDX_COL_KERNEL_PREFIX, pw.w256.total, [&] { launch_queue_prefix(st, pw.w256, rb.qprefix.p); });
             timer.time(GDX_COL_KERNEL_CHAIN, pw.w256.total, [&] { launch_chain(st, pw.w256, rb.resident); });
-            timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w256, pw.blocks256); });
+            timer.time(GDX_COL_KERNEL_AUDIT, pw.w256.total, [&] { launch_chain_audit(st, pw.w1024, pw.blocks1024); });
             rb.read_back(rb.off_wave, rb.back_bytes, st);          // wave counters, errors, audit bins
             wait_stream(st, "chain pass");
             for (op_run *r : now) r->after_chain();

@@ -32,8 +32,6 @@ constexpr int WAVE = 64;
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / WAVE;
 constexpr int PUBLISH_OWNERS = 1024;
-constexpr int RATE_OWNERS = 256;
-constexpr int CHAIN_OWNERS = 256;
 // owners expected to take at least this many events in the bath (rate at the bath start times the
 // bath) join the heavy end of their group's queue region, which the chain takes first: long chains
 // then start with the launch instead of holding it open after the other owners are done
@@ -1919,12 +1917,17 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
     }
 }
 
-// audit of the chained owners, one lane per queue position of each due group (col_audit_bin.cu:13-60)
+// audit of the chained owners, one lane per queue position of each due group (col_audit_bin.cu:13-60),
+// AUDIT_ROUNDS positions per lane: a block covers 1024 slots of its group, and blocks without a queued
+// position (most of them, the queues being short) leave after reading two counters
+constexpr int AUDIT_ROUNDS = 4;
+static_assert(BLOCK*AUDIT_ROUNDS == PUBLISH_OWNERS, "the audit uses the 1024-owner block starts");
+
 __global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_wave w)
 {
     __shared__ audit_lds a;
     int gi, s0, s1;
-    if (!due_block(w, CHAIN_OWNERS, gi, s0, s1)) return;
+    if (!due_block(w, BLOCK*AUDIT_ROUNDS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
     // queued positions: the heavy queue [beg, h1) and the light queue [l0, beg + cnt)
     const int h1 = w.beg[gi] + *G.hcount;
@@ -1932,40 +1935,46 @@ __global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_wave w)
     if (s0 >= h1 && s1 <= l0) return;
     const double end = w.dt[gi];
     const double lo = *G.lower, up = *G.upper;
-    const int *queue = G.queue;
-    const double2 *st = G.st, *res = G.res;
-    const double4 *cached = G.cached, *qsum = G.qsum;
-    const int *qevents = G.qevents;
+    const int *__restrict__ queue = G.queue;
+    const double2 *__restrict__ st = G.st;
+    const double2 *__restrict__ res = G.res;
+    const double4 *__restrict__ cached = G.cached;
+    const double4 *__restrict__ qsum = G.qsum;
+    const int *__restrict__ qevents = G.qevents;
     clear_audit(a);
     __syncthreads();
-    const int pos = s0 + int(threadIdx.x);
-    // owners that ended in an error are left out: the operator call fails on them
-    bool take = pos < s1 && (pos < h1 || pos >= l0) && qevents[pos] >= 0;
-    // learned swarms audit their exact subsample only
-    if (take && w.learned) take = exact_owner(w, gi, queue[pos]);
-    double2 old = make_double2(1.0, 0.0);
-    double size_end = 1.0, rate0 = 0.0;
-    double4 sum = make_double4(0.0, 0.0, 0.0, 0.0);
-    int events = 0;
-    if (take)
+#pragma unroll 1
+    for (int k = 0; k < AUDIT_ROUNDS; ++k)
     {
-        int s = queue[pos];
-        old = res[s];
-        size_end = st[s].x;
-        rate0 = cached[s].x;
-        sum = qsum[pos];
-        events = qevents[pos];
+        const int pos = s0 + k*BLOCK + int(threadIdx.x);
+        // owners that ended in an error are left out: the operator call fails on them
+        bool take = pos < s1 && (pos < h1 || pos >= l0) && qevents[pos] >= 0;
+        // learned swarms audit their exact subsample only
+        if (take && w.learned) take = exact_owner(w, gi, queue[pos]);
+        double2 old = make_double2(1.0, 0.0);
+        double size_end = 1.0, rate0 = 0.0;
+        double4 sum = make_double4(0.0, 0.0, 0.0, 0.0);
+        int events = 0;
+        if (take)
+        {
+            int s = queue[pos];
+            old = res[s];
+            size_end = st[s].x;
+            rate0 = cached[s].x;
+            sum = qsum[pos];
+            events = qevents[pos];
+        }
+        int bs = take ? size_bin(old.x, lo, up) : -1;
+        int be = take ? size_bin(size_end, lo, up) : -1;
+        double weight = old.y*grain_mass(old.x);
+        double probability = -expm1(-rate0*end);
+        double ev = static_cast<double>(events);
+        bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(sum.x)
+            || sum.x < 0.0 || !isfinite(ev) || !isfinite(sum.y) || sum.y < 0.0 || !isfinite(sum.z) || sum.z < 0.0
+            || !isfinite(sum.w) || sum.w < 0.0 || !isfinite(size_end) || !(size_end > 0.0);
+        double growth = fabs(log(size_end / old.x));
+        audit_wave<true>(a, take, bs, be, weight, probability, sum.x, sum.y, sum.z, sum.w, ev, growth, bad);
     }
-    int bs = take ? size_bin(old.x, lo, up) : -1;
-    int be = take ? size_bin(size_end, lo, up) : -1;
-    double weight = old.y*grain_mass(old.x);
-    double probability = -expm1(-rate0*end);
-    double ev = static_cast<double>(events);
-    bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(sum.x) || sum.x < 0.0
-        || !isfinite(ev) || !isfinite(sum.y) || sum.y < 0.0 || !isfinite(sum.z) || sum.z < 0.0 || !isfinite(sum.w)
-        || sum.w < 0.0 || !isfinite(size_end) || !(size_end > 0.0);
-    double growth = fabs(log(size_end / old.x));
-    audit_wave<true>(a, take, bs, be, weight, probability, sum.x, sum.y, sum.z, sum.w, ev, growth, bad);
     __syncthreads();
     flush_audit(a, G.raw_audit);
 }
