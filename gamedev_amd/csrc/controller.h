@@ -189,9 +189,10 @@ struct schedule
     static constexpr int FINEST = 52;
     static constexpr int LINK_LEVELS = 4;
     static constexpr int FAR = 1 << 20;       // hop count of unreachable pairs
+    static constexpr int MAX_GROUPS = 64;     // one bit per group in the link masks
     double horizon;
     int n;
-    std::vector<std::vector<int>> adj;        // linked groups (an edge either way)
+    std::vector<uint64_t> adj;                // linked groups (an edge either way), one bit per group
     std::vector<int> level;
     std::vector<uint64_t> step, next;
     uint64_t end;
@@ -205,32 +206,36 @@ struct schedule
         throw status_error(10, too_fine);
     }
 
-    // hop counts from `src` along links whose interior groups satisfy `relay` (FAR if unreachable)
-    template <typename R>
-    std::vector<int> hops_from(int src, R &&relay) const
+    // hop counts d[0..n) from `src` along links whose interior groups are in the mask `relay` (FAR if
+    // unreachable): a breadth-first search one hop level at a time over the link masks (the host's
+    // share of every refresh wave, so it allocates nothing)
+    void hops_from(int src, uint64_t relay, int *d) const
     {
-        std::vector<int> d(n, FAR);
-        std::vector<int> frontier{src};
+        for (int i = 0; i < n; ++i) d[i] = FAR;
         d[src] = 0;
-        for (size_t k = 0; k < frontier.size(); ++k)
+        uint64_t seen = uint64_t(1) << src, frontier = seen;
+        const uint64_t pass = relay | seen;
+        for (int depth = 1; frontier; ++depth)
         {
-            const int u = frontier[k];
-            if (u != src && !relay(u)) continue;
-            for (int v : adj[u])
-                if (d[v] == FAR) { d[v] = d[u] + 1; frontier.push_back(v); }
+            uint64_t reach = 0;
+            for (uint64_t f = frontier & pass; f; f &= f - 1) reach |= adj[size_t(__builtin_ctzll(f))];
+            reach &= ~seen;
+            seen |= reach;
+            for (uint64_t m = reach; m; m &= m - 1) d[__builtin_ctzll(m)] = depth;
+            frontier = reach;
         }
-        return d;
     }
 
     // the finest level any group forces on c through the 16:1 links: max over e of lv[e] - 4 hops(c, e),
-    // relayed only through groups in `mutable_set` (the others keep lv and pass nothing on)
-    std::vector<int> raise_for_links(const std::vector<int> &lv, const std::vector<bool> &mutable_set) const
+    // relayed only through groups in the mask `mutable_set` (the others keep lv and pass nothing on)
+    std::vector<int> raise_for_links(const std::vector<int> &lv, uint64_t mutable_set) const
     {
         std::vector<int> out = lv;
+        int d[MAX_GROUPS];
         for (int c = 0; c < n; ++c)
         {
-            if (!mutable_set[c]) continue;
-            const std::vector<int> d = hops_from(c, [&](int u) { return bool(mutable_set[u]); });
+            if (!((mutable_set >> c) & 1u)) continue;
+            hops_from(c, mutable_set, d);
             for (int e = 0; e < n; ++e)
                 if (d[e] != FAR) out[c] = std::max(out[c], lv[e] - LINK_LEVELS*d[e]);
         }
@@ -238,20 +243,21 @@ struct schedule
     }
 
     schedule(double h, const std::vector<double> &requested, const std::vector<uint32_t> &edges)
-        : horizon(h), n(int(requested.size())), adj(requested.size()), level(requested.size(), 0),
+        : horizon(h), n(int(requested.size())), adj(requested.size(), 0), level(requested.size(), 0),
           step(requested.size()), next(requested.size(), 0)
     {
         if (!(h > 0) || !std::isfinite(h)) throw status_error(10, "invalid local collision horizon");
+        if (n > MAX_GROUPS) throw status_error(GDX_COL_ERR_ARG, "the scheduler takes at most 64 groups");
         const int words = (n + 31) / 32;
         for (int c = 0; c < n; ++c)
             for (int d = 0; d < n; ++d)
                 if (c != d && (((edges[size_t(c)*words + d / 32] >> (d % 32)) & 1u)
                                || ((edges[size_t(d)*words + c / 32] >> (c % 32)) & 1u)))
-                    adj[c].push_back(d);
+                    adj[size_t(c)] |= uint64_t(1) << d;
         for (int c = 0; c < n; ++c)
             level[c] = level_for(requested[c], "invalid local collision timestep",
                                  "local collision timestep exceeds time resolution");
-        level = raise_for_links(level, std::vector<bool>(n, true));
+        level = raise_for_links(level, n == 64 ? ~uint64_t(0) : (uint64_t(1) << n) - 1);
         end = uint64_t(1) << FINEST;
         for (int c = 0; c < n; ++c) step[c] = uint64_t(1) << (FINEST - level[c]);
     }
@@ -266,16 +272,17 @@ struct schedule
     void adapt(uint64_t tick, const std::vector<int> &groups, const std::vector<double> &requested,
                const std::vector<bool> &passed)
     {
-        std::vector<bool> is_due(n, false);
-        for (int c : groups) is_due[c] = true;
+        uint64_t is_due = 0;
+        for (int c : groups) is_due |= uint64_t(1) << c;
         // finest level that still leaves every pending endpoint of a group that is not due in place:
         // min over such groups e of level[e] + 4 hops(c, e), through due groups only
         std::vector<int> cap(n, FINEST);
+        int d[MAX_GROUPS];
         for (int c : groups)
         {
-            const std::vector<int> d = hops_from(c, [&](int u) { return bool(is_due[u]); });
+            hops_from(c, is_due, d);
             for (int e = 0; e < n; ++e)
-                if (!is_due[e] && d[e] != FAR) cap[c] = std::min(cap[c], level[e] + LINK_LEVELS*d[e]);
+                if (!((is_due >> e) & 1u) && d[e] != FAR) cap[c] = std::min(cap[c], level[e] + LINK_LEVELS*d[e]);
         }
         std::vector<int> cand = level;
         for (int c : groups)
