@@ -36,6 +36,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "collision updates/s at 1e7 reps, K=32: learned+batched vs plain, same GPU"
 HBM_PEAK_GBS = 8000.0
+# vector-ALU issue peak of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each (FP64 FMA
+# included, at full rate), 2.4 GHz
+VALU_PEAK_WAVE_INSTR = 256*4*2.4e9/4
 K = 32
 # algorithmic bytes (K = 32) of each timed kernel, and what limits it (measured: SQ counters and traces)
 #   publish:      per due owner: state (s, N) read 16 + reservoir publish 16
@@ -57,9 +60,10 @@ RATED_BYTES = 20*K + 29
 CAND_BYTES, CAND_PAIR_BYTES = 40, 20
 CHAIN_OWNER_BYTES, CHAIN_PAIR_BYTES, AUDIT_OWNER_BYTES = 140, 20, 64
 MAJORANT_DUE_BYTES, MAJORANT_EVAL_BYTES = 1, 12*K + 72
-LIMIT = {"publish": "hbm", "mark_stale": "latency", "bath_rate": "fp64-valu", "screen": "latency",
-         "candidates": "fp64-valu", "queue_prefix": "latency", "chain": "fp64-valu + serial event latency",
-         "chain_audit": "latency", "majorant": "valu (features) + gather latency"}
+LIMIT = {"publish": "hbm", "mark_stale": "fp64-valu + gather latency", "bath_rate": "fp64-valu",
+         "screen": "valu (draws) + latency", "candidates": "fp64-valu", "queue_prefix": "latency",
+         "chain": "fp64-valu + serial event latency", "chain_audit": "latency",
+         "majorant": "valu (features) + gather latency"}
 
 
 def kernel_bytes(kernel: str, rec: dict, tot: dict) -> float:
@@ -469,7 +473,7 @@ def kernel_table(kt: dict, tot: dict, steps: int) -> dict:
     return out
 
 
-def roofline_of(kt: dict, tot: dict, dom: str | None = None) -> dict:
+def roofline_of(kt: dict, tot: dict, dom: str | None = None, learned: bool = False) -> dict:
     dom = dom or max(kt, key=lambda k: kt[k]["ms"])
     rec = kt[dom]
     bytes_per_launch = kernel_bytes(dom, rec, tot)/max(1, rec["launches"])
@@ -482,9 +486,20 @@ def roofline_of(kt: dict, tot: dict, dom: str | None = None) -> dict:
         per_owner = json.loads(tfile.read_text()).get("bytes_per_owner")
         if per_owner:
             traffic = per_owner*rec["owners"]/max(1, rec["launches"])
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom, "limit": LIMIT.get(dom, ""),
-            "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved/HBM_PEAK_GBS, "traffic": traffic, "kernel": dom, "limit": LIMIT.get(dom, ""),
+           "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s*1e3, "launches": rec["launches"]}
+    # learned-run kernels limited by the vector ALUs: their VALU instructions per launch (SQ_INSTS_VALU of the
+    # committed counter run of the learned operator) over this run's launch time, against the issue peak
+    cfile = ROOT/"profiles"/"counters_learned_kernels_fiducial_3d_1e7.json"
+    if learned and "valu" in LIMIT.get(dom, "") and cfile.exists() and avg_s > 0:
+        k = json.loads(cfile.read_text())["kernels"].get(dom)
+        if k:
+            rate = k["valu_instructions_per_launch"]/avg_s
+            out["valu"] = {"achieved": rate, "peak": VALU_PEAK_WAVE_INSTR, "unit": "wave64 VALU instructions/s",
+                           "frac": rate/VALU_PEAK_WAVE_INSTR, "busy_in_counter_run": k["valu_busy"],
+                           "source": "profiles/counters_learned_kernels_fiducial_3d_1e7.json"}
+    return out
 
 
 def main() -> int:
@@ -615,7 +630,7 @@ def main() -> int:
                    "longest_chain_events": l_tot["longest_chain"],
                    "kernel_ms_per_step": {k: v["ms"]/max(1, lp_steps) for k, v in l_kt.items()},
                    "kernels": kernel_table(l_kt, lp_tot, lp_steps),
-                   "roofline": roofline_of(l_kt, lp_tot), "totals": l_tot}
+                   "roofline": roofline_of(l_kt, lp_tot, learned=True), "totals": l_tot}
         op.use_majorant(None)
 
     batched = batched_fid = None
