@@ -648,12 +648,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         if (lane == 0 && nv) atomicAdd(&checked[1], nv);
     }
     // controller-bin sums: each owner adds to the block's LDS bins (native FP64 LDS atomics)
-    int sb = lead ? size_bin(me.x, lo, up) : -1;
-    double weight = me.y*grain_mass(me.x);
-    bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(R) || R < 0.0 || !isfinite(J1) || J1 < 0.0
-        || !isfinite(J2) || J2 < 0.0;
     if (lead)
     {
+        const int sb = size_bin(me.x, lo, up);
+        const double weight = me.y*grain_mass(me.x);
+        const bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(R) || R < 0.0 || !isfinite(J1)
+            || J1 < 0.0 || !isfinite(J2) || J2 < 0.0;
         atomicAdd(&acci[sb][0], 1);
         atomicAdd(&acci[sb][1], 1);
         if (bad)
@@ -1112,13 +1112,18 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
                 rate0 = (isfinite(c.x) && c.x >= 0.0) ? c.x : 0.0;
             }
         }
-        int bs = done ? size_bin(old.x, lo, up) : -1;
-        double weight = old.y*grain_mass(old.x);
-        double probability = -expm1(-rate0*end);
-        bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard)
-            || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
-            || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
-        audit_wave<false>(a, done, bs, bs, weight, probability, hazard, j1, j2, jmax, 0.0, 0.0, bad);
+        // audit terms of a finished owner only: in a learned swarm most wavefronts hold no owner of the
+        // exact subsample and skip them (the bin, the mass and expm1 cost more than the draw)
+        if (done)
+        {
+            const int bs = size_bin(old.x, lo, up);
+            const double weight = old.y*grain_mass(old.x);
+            const double probability = -expm1(-rate0*end);
+            const bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(hazard)
+                || hazard < 0.0 || !isfinite(j1) || j1 < 0.0 || !isfinite(j2) || j2 < 0.0 || !isfinite(jmax)
+                || jmax < 0.0 || !isfinite(old.x) || !(old.x > 0.0);
+            audit_wave<false>(a, true, bs, bs, weight, probability, hazard, j1, j2, jmax, 0.0, 0.0, bad);
+        }
         // owners with an event join the group's queue (heavy ones from the region's start, the others
         // from its end) and thinning candidates its candidate list: offsets within the block's share
         // from one LDS atomic per wavefront and list
@@ -1964,16 +1969,19 @@ __global__ __launch_bounds__(BLOCK) void k_chain_audit(gdx_wave w)
             sum = qsum[pos];
             events = qevents[pos];
         }
-        int bs = take ? size_bin(old.x, lo, up) : -1;
-        int be = take ? size_bin(size_end, lo, up) : -1;
-        double weight = old.y*grain_mass(old.x);
-        double probability = -expm1(-rate0*end);
-        double ev = static_cast<double>(events);
-        bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(sum.x)
-            || sum.x < 0.0 || !isfinite(ev) || !isfinite(sum.y) || sum.y < 0.0 || !isfinite(sum.z) || sum.z < 0.0
-            || !isfinite(sum.w) || sum.w < 0.0 || !isfinite(size_end) || !(size_end > 0.0);
-        double growth = fabs(log(size_end / old.x));
-        audit_wave<true>(a, take, bs, be, weight, probability, sum.x, sum.y, sum.z, sum.w, ev, growth, bad);
+        if (take)
+        {
+            const int bs = size_bin(old.x, lo, up);
+            const int be = size_bin(size_end, lo, up);
+            const double weight = old.y*grain_mass(old.x);
+            const double probability = -expm1(-rate0*end);
+            const double ev = static_cast<double>(events);
+            const bool bad = !isfinite(weight) || !(weight > 0.0) || !isfinite(probability) || !isfinite(sum.x)
+                || sum.x < 0.0 || !isfinite(ev) || !isfinite(sum.y) || sum.y < 0.0 || !isfinite(sum.z)
+                || sum.z < 0.0 || !isfinite(sum.w) || sum.w < 0.0 || !isfinite(size_end) || !(size_end > 0.0);
+            const double growth = fabs(log(size_end / old.x));
+            audit_wave<true>(a, true, bs, be, weight, probability, sum.x, sum.y, sum.z, sum.w, ev, growth, bad);
+        }
     }
     __syncthreads();
     flush_audit(a, G.raw_audit);
