@@ -562,7 +562,10 @@ def main() -> int:
         log(f"autotuned Morton leaf target {tune['best']}: " + ", ".join(f"{k}: {v:.1f} ms" for k, v in tune["ms"].items()))
         tile = op.autotune_tile((0, 8, 16, 32), repeats=1)
         log(f"autotuned top-K tile {tile['best']}: " + ", ".join(f"{k}: {v:.1f} ms" for k, v in tile["ms"].items()))
-        tune = {"best": tune["best"], "ms": tune["ms"], "tile": tile}
+        block = op.autotune_block((1, 2, 4), repeats=1)
+        log(f"autotuned top-K block {block['best']} wavefronts: "
+            + ", ".join(f"{k}: {v:.1f} ms" for k, v in block["ms"].items()))
+        tune = {"best": tune["best"], "ms": tune["ms"], "tile": tile, "block": block}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     op.geometry()

@@ -125,6 +125,8 @@ _SIGS = {
     "gdx_col_autotune": [P, P, I, I, P, P],
     "gdx_col_set_topk_tile": [P, I],
     "gdx_col_autotune_tile": [P, P, I, I, P, P],
+    "gdx_col_set_topk_block": [P, I],
+    "gdx_col_autotune_block": [P, P, I, I, P, P],
 }
 EXPORTS = tuple(_SIGS) + ("gdx_col_last_error", "gdx_col_majorant_last_error")
 

@@ -252,6 +252,19 @@ class CollisionOperator:
         self._call(self.lib.gdx_col_autotune_tile, _np_ptr(c), len(c), int(repeats), C.byref(best), _np_ptr(ms))
         return {"best": best.value, "ms": {int(k): float(v) for k, v in zip(c, ms)}}
 
+    def set_topk_block(self, waves: int) -> None:
+        """Block shape of the Morton search: 1, 2 or 4 wavefronts per block of the top-K kernel."""
+        self._call(self.lib.gdx_col_set_topk_block, int(waves))
+
+    def autotune_block(self, candidates=(1, 2, 4), repeats: int = 2) -> dict:
+        """Time the geometry epoch per top-K block shape at the current leaf target and tile, keep the
+        fastest."""
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c), np.float64)
+        best = C.c_int()
+        self._call(self.lib.gdx_col_autotune_block, _np_ptr(c), len(c), int(repeats), C.byref(best), _np_ptr(ms))
+        return {"best": best.value, "ms": {int(k): float(v) for k, v in zip(c, ms)}}
+
     # ---- learned majorant ----
     def majorant_sample(self):
         """Features (n x 8, float32), partner scale S and exact bath-start rate R per particle at the

@@ -140,6 +140,38 @@ struct phase_wave
     int blocks1024 = 0, blocks256 = 0, blocks_sub = 0, blocks_maj = 0, blocks_lsub = 0;
 };
 
+// geometry epochs timed for each candidate value of one top-K launch parameter (`field`, the tile or the
+// block shape), the fastest kept: the epoch's results do not depend on either, only its time does
+void autotune_epoch(context &c, int &field, const int *candidates, int count, int repeats, int *best, double *ms)
+{
+    require(c.user.bound, GDX_COL_ERR_STATE, "no particle state is bound");
+    if (c.state_loaded) store_state(c);
+    hip_check(hipStreamSynchronize(c.stream), "autotune");
+    int pick = field;
+    double fastest = 0.0;
+    for (int i = 0; i < count; ++i)
+    {
+        field = candidates[i];
+        geometry_epoch(c);
+        double t = 0.0;
+        for (int r = 0; r < repeats; ++r)
+        {
+            hip_check(hipStreamSynchronize(c.stream), "autotune");
+            auto t0 = std::chrono::steady_clock::now();
+            geometry_epoch(c);
+            hip_check(hipStreamSynchronize(c.stream), "autotune");
+            double v = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            t = r ? std::min(t, v) : v;
+        }
+        if (ms) ms[i] = t;
+        if (i == 0 || t < fastest) { fastest = t; pick = candidates[i]; }
+    }
+    field = pick;
+    geometry_epoch(c);
+    hip_check(hipStreamSynchronize(c.stream), "autotune");
+    if (best) *best = pick;
+}
+
 // the host's wait for the end of a phase: polling the stream rather than sleeping in the runtime's
 // wait, since the next phase's launches depend on what the host reads back (every microsecond of
 // wake-up is a microsecond of idle GPU, several times per operator call)
@@ -1444,37 +1476,33 @@ int gdx_col_autotune_tile(gdx_col_handle h, const int *candidates, int count, in
 {
     if (!h) return GDX_COL_ERR_ARG;
     return guard(&h->c, [&] {
-        context &c = h->c;
         require(candidates && count > 0 && repeats > 0, GDX_COL_ERR_ARG, "invalid autotuning arguments");
-        require(c.user.bound, GDX_COL_ERR_STATE, "no particle state is bound");
         for (int i = 0; i < count; ++i)
             require(candidates[i] == 0 || candidates[i] == 8 || candidates[i] == 16 || candidates[i] == 32,
                     GDX_COL_ERR_ARG, "top-K tile must be 0, 8, 16 or 32");
-        if (c.state_loaded) store_state(c);
-        hip_check(hipStreamSynchronize(c.stream), "autotune");
-        int pick = c.topk_tile;
-        double fastest = 0.0;
+        autotune_epoch(h->c, h->c.topk_tile, candidates, count, repeats, best, ms);
+    });
+}
+
+int gdx_col_set_topk_block(gdx_col_handle h, int waves)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(waves == 1 || waves == 2 || waves == 4, GDX_COL_ERR_ARG, "top-K block must be 1, 2 or 4 wavefronts");
+        h->c.topk_waves = waves;
+        h->c.geometry_valid = h->c.state_loaded = false;
+    });
+}
+
+int gdx_col_autotune_block(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(candidates && count > 0 && repeats > 0, GDX_COL_ERR_ARG, "invalid autotuning arguments");
         for (int i = 0; i < count; ++i)
-        {
-            c.topk_tile = candidates[i];
-            geometry_epoch(c);
-            double t = 0.0;
-            for (int r = 0; r < repeats; ++r)
-            {
-                hip_check(hipStreamSynchronize(c.stream), "autotune");
-                auto t0 = std::chrono::steady_clock::now();
-                geometry_epoch(c);
-                hip_check(hipStreamSynchronize(c.stream), "autotune");
-                double v = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                t = r ? std::min(t, v) : v;
-            }
-            if (ms) ms[i] = t;
-            if (i == 0 || t < fastest) { fastest = t; pick = candidates[i]; }
-        }
-        c.topk_tile = pick;
-        geometry_epoch(c);
-        hip_check(hipStreamSynchronize(c.stream), "autotune");
-        if (best) *best = pick;
+            require(candidates[i] == 1 || candidates[i] == 2 || candidates[i] == 4, GDX_COL_ERR_ARG,
+                    "top-K block must be 1, 2 or 4 wavefronts");
+        autotune_epoch(h->c, h->c.topk_waves, candidates, count, repeats, best, ms);
     });
 }
 

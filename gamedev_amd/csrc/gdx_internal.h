@@ -253,6 +253,7 @@ struct context
     int leaf_target = MORTON_LEAF_TARGET;   // Morton refinement threshold (gdx_col_set_leaf_target, autotuner)
     int last_waves = 0;        // refresh waves of the last operator call (ensemble load balance)
     int topk_tile = 16;        // top-K merge tile: 0 insertion, 8/16/32 keys per merge (gdx_col_set_topk_tile, autotuner)
+    int topk_waves = 1;        // top-K block shape: 1, 2 or 4 wavefronts per block (gdx_col_set_topk_block, autotuner)
     hipStream_t stream = nullptr;
     bound_state user;
 

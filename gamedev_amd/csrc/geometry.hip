@@ -695,9 +695,11 @@ __device__ __forceinline__ void wave_centre(float qx, float qy, float qz, bool q
 
 // col_cache_get (inc/swarm/_col_cache.cuh:73-159) over Morton-ordered queries
 // one query per physical record; DEDUP when two images of one particle can share a search ball
-// TILE = 0: the insertion scheme; TILE = B > 0: the tiled scheme with B-key merges (no image dedup)
-template <bool DEDUP, int TILE>
-__global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *__restrict__ rec,
+// TILE = 0: the insertion scheme; TILE = B > 0: the tiled scheme with B-key merges (no image dedup);
+// WPB wavefronts per block (the block shape: every wavefront runs its own queries with its own LDS
+// stack and tile, so the shape changes only how the wavefronts are packed onto the CUs)
+template <bool DEDUP, int TILE, int WPB>
+__global__ __launch_bounds__(WPB*WAVE) void k_col_topk(const float4 *__restrict__ rec,
                                                         const gdx_node *__restrict__ nodes, int n, int dim,
                                                         int max_level, const int *__restrict__ sidx,
                                                         const float4 *__restrict__ pts,
@@ -711,10 +713,10 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_col_topk(const float4 *__restric
 {
     static_assert(!(DEDUP && TILE), "image dedup runs on the insertion scheme");
     static_assert(TILE == 0 || TILE >= TILE_GROUP, "a merge tile holds at least one record group");
-    __shared__ int stacks[TOPK_BLOCK / WAVE][TOPK_STACK];
-    __shared__ uint64_t tiles[TOPK_BLOCK / WAVE][TILE > 0 ? TILE : 1][WAVE];
+    __shared__ int stacks[WPB][TOPK_STACK];
+    __shared__ uint64_t tiles[WPB][TILE > 0 ? TILE : 1][WAVE];
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    long long m = ((long long)xcd_block()*(TOPK_BLOCK / WAVE) + wave)*WAVE + lane;
+    long long m = ((long long)xcd_block()*WPB + wave)*WAVE + lane;
     bool valid = m < n;
     float4 me = valid ? rec[m] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
     int code = __float_as_int(me.w);
@@ -1166,12 +1168,18 @@ void geometry_epoch(context &c)
     else
     {
         int waves = blocks_for(nr, WAVE);
-        dim3 grid = xcd_grid(blocks_for(waves, TOPK_BLOCK / WAVE));
         if (X_WEDGE) c.nbr_image.alloc(na*N_K);
-#define GDX_TOPK_LAUNCH(DEDUP, TILE)                                                                         \
-    k_col_topk<DEDUP, TILE><<<grid, TOPK_BLOCK, 0, st>>>(                                                     \
+#define GDX_TOPK_SHAPED(DEDUP, TILE, WPB)                                                                    \
+    k_col_topk<DEDUP, TILE, WPB><<<xcd_grid(blocks_for(waves, WPB)), WPB*WAVE, 0, st>>>(                     \
         c.index.rec.p, c.index.nodes.p, nr, DIM, MORTON_MAX_LEVEL, c.index.idx.p, c.pts.p, c.pidx_slot.p,     \
         c.group.p, c.user, c.nbr.p, c.measure.p, c.graph.p, c.overflow.p, c.nbr_image.p)
+#define GDX_TOPK_LAUNCH(DEDUP, TILE)                                                                         \
+    do                                                                                                        \
+    {                                                                                                         \
+        if (c.topk_waves == 1) GDX_TOPK_SHAPED(DEDUP, TILE, 1);                                               \
+        else if (c.topk_waves == 2) GDX_TOPK_SHAPED(DEDUP, TILE, 2);                                          \
+        else GDX_TOPK_SHAPED(DEDUP, TILE, 4);                                                                 \
+    } while (0)
         if (!c.unique_ids && X_WEDGE)
             GDX_TOPK_LAUNCH(bool(X_WEDGE), 0);
         else if (c.topk_tile == 8)
@@ -1183,6 +1191,7 @@ void geometry_epoch(context &c)
         else
             GDX_TOPK_LAUNCH(false, 0);
 #undef GDX_TOPK_LAUNCH
+#undef GDX_TOPK_SHAPED
         hip_check(hipGetLastError(), "launch top-K");
     }
     unsigned int ovf = 0;

@@ -143,6 +143,12 @@ int gdx_col_autotune(gdx_col_handle h, const int *candidates, int count, int rep
  * keeps the fastest. */
 int gdx_col_set_topk_tile(gdx_col_handle h, int tile);
 int gdx_col_autotune_tile(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms);
+/* block shape of the Morton search, no reference counterpart (the reference launches one block per
+ * 64-query tile): 1, 2 or 4 wavefronts per block of the top-K kernel, each wavefront with its own LDS
+ * stack and tile. Rows are identical for every shape; the autotuner times the epoch at each candidate (at
+ * the current leaf target and tile) and keeps the fastest. */
+int gdx_col_set_topk_block(gdx_col_handle h, int waves);
+int gdx_col_autotune_block(gdx_col_handle h, const int *candidates, int count, int repeats, int *best, double *ms);
 
 /* one operator call over `duration`: evolve_local_collisions (inc/swarm/_col_chain.cuh:108-539) */
 int gdx_col_operator(gdx_col_handle h, double duration, gdx_col_stats *out);

@@ -1,13 +1,13 @@
-"""The geometry autotuner: the Morton leaf target and the top-K merge tile change only the search time.
-Neighbour rows and a following operator call are bit-identical for every leaf target and tile; the tuner
-keeps the fastest."""
+"""The geometry autotuner: the Morton leaf target, the top-K merge tile and the top-K block shape change
+only the search time. Neighbour rows and a following operator call are bit-identical for every leaf
+target, tile and shape; the tuner keeps the fastest."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def _run(st, gpu, leaf, tile=None, model="fiducial_3d"):
+def _run(st, gpu, leaf, tile=None, model="fiducial_3d", block=None):
     from gamedev_amd import CollisionOperator, synth
     d = synth.to_device(st, gpu)
     op = CollisionOperator(model, len(st["x"]), st["total_mass"], seed=3)
@@ -16,6 +16,8 @@ def _run(st, gpu, leaf, tile=None, model="fiducial_3d"):
         op.set_leaf_target(leaf)
     if tile is not None:
         op.set_topk_tile(tile)
+    if block is not None:
+        op.set_topk_block(block)
     op.geometry()
     codes, meas = op.neighbors()
     s = op.evolve(0.01)
@@ -88,3 +90,36 @@ def test_autotune_tile_keeps_the_fastest(gpu):
     np.testing.assert_array_equal(codes0, codes1)
     with pytest.raises(GdxError):
         op.set_topk_tile(12)
+
+
+@pytest.mark.parametrize("model", ["fiducial_3d", "plain_2d"])
+def test_topk_block_does_not_change_results(gpu, model):
+    """One, two and four wavefronts per top-K block give the same rows, measures and operator results
+    (tiled and insertion selection)."""
+    from gamedev_amd import synth
+    st = synth.fiducial_3d_state(30000, seed=28) if model == "fiducial_3d" else synth.plain_2d_state(30000, seed=28)
+    ref = _run(st, gpu, None, 16, model, 4)
+    for tile, block in ((16, 1), (16, 2), (0, 1), (32, 2)):
+        print(f"[autotune] {model} tile {tile} block {block}", flush=True)
+        r = _run(st, gpu, None, tile, model, block)
+        for a, b in zip(r[:5], ref[:5]):
+            np.testing.assert_array_equal(a, b)
+        assert r[5]["events"] == ref[5]["events"] and r[5]["waves"] == ref[5]["waves"]
+
+
+def test_autotune_block_keeps_the_fastest(gpu):
+    from gamedev_amd import CollisionOperator, GdxError, synth
+    st = synth.fiducial_3d_state(50000, seed=29)
+    d = synth.to_device(st, gpu)
+    op = CollisionOperator("fiducial_3d", 50000, st["total_mass"], seed=3)
+    op.bind_soa(d["x"], d["y"], d["z"], d["size"], d["numr"])
+    op.geometry()
+    codes0, _ = op.neighbors()
+    res = op.autotune_block((1, 2, 4), repeats=2)
+    print("[autotune] block", res, flush=True)
+    assert res["best"] in (1, 2, 4)
+    assert res["ms"][res["best"]] == min(res["ms"].values())
+    codes1, _ = op.neighbors()
+    np.testing.assert_array_equal(codes0, codes1)
+    with pytest.raises(GdxError):
+        op.set_topk_block(3)
