@@ -231,6 +231,12 @@ class CollisionOperator:
         """Test access: every bath of later calls lasts min(duration, horizon); 0 restores the controller."""
         self._call(self.lib.gdx_col_set_fixed_bath, float(duration))
 
+    def set_narrow_waves(self, owners: int) -> None:
+        """Refresh waves of at most `owners` due owners take the narrow launch shapes (four lanes per
+        plain owner in the rate pass, one owner per lane in the screen); 0 always takes the wide ones.
+        Results do not depend on it."""
+        self._call(self.lib.gdx_col_set_narrow_waves, int(owners))
+
     def autotune(self, candidates=(16, 32, 64, 128, 256), repeats: int = 2) -> dict:
         """Time the geometry epoch per Morton leaf target, keep the fastest (a valid epoch is left built
         with it); returns {"best": leaf, "ms": {leaf: ms}}."""

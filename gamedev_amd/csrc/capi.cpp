@@ -131,13 +131,15 @@ struct phase_entry
     int sub;              // learned swarm: exact-subsample phase of this bath; -1 plain
 };
 
-// due entries of one phase in device memory; launch shapes: 1024 and 256 owners per block, the plain
-// groups' owners (256 per block), the learned groups' owners (MAJ_SLOTS per block) and the learned
-// groups' exact subsamples (RATE_SUB_OWNERS per block)
+// due entries of one phase in device memory; launch shapes: 1024 and 256 owners per block, the groups
+// rated one lane per owner (256 per block), the learned groups' owners (MAJ_SLOTS per block) and the
+// groups rated by lane groups (RATE_SUB_OWNERS per block): learned groups' exact subsamples, and plain
+// groups when the phase is narrow (at most narrow_owners due owners)
 struct phase_wave
 {
     gdx_wave w1024, w256, wsub, wmaj, wlsub;
     int blocks1024 = 0, blocks256 = 0, blocks_sub = 0, blocks_maj = 0, blocks_lsub = 0;
+    bool narrow = false;
 };
 
 // geometry epochs timed for each candidate value of one top-K launch parameter (`field`, the tile or the
@@ -183,13 +185,17 @@ void wait_stream(hipStream_t st, const char *what)
 }
 
 // fills the phase's due list and stages it on the device together with clearing the ranges `clear`
-phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipStream_t st, const zero_ranges &clear)
+phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipStream_t st, const zero_ranges &clear,
+                        int narrow_owners)
 {
     const int n = int(e.size());
     rb.reserve(n);
     int *ref = rb.h_int, *beg = ref + n, *cnt = beg + n, *sub = cnt + n, *b1024 = sub + n, *b256 = b1024 + n + 1;
     int *bsub = b256 + n + 1, *bmaj = bsub + n + 1, *blsub = bmaj + n + 1;
     int total = 0, x = 0, y = 0, z = 0, u = 0, v = 0, learned = 0;
+    long long due = 0;
+    for (const phase_entry &p : e) due += p.cnt;
+    const bool narrow = due <= narrow_owners;
     for (int i = 0; i < n; ++i)
     {
         ref[i] = e[i].ref;
@@ -204,9 +210,11 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
         blsub[i] = v;
         x += (e[i].cnt + 1023) / 1024;
         y += (e[i].cnt + 255) / 256;
-        // plain groups: every owner, 256 per block; learned groups: the exact subsample (runs sub, sub +
-        // stride, ... of THIN_RUN slots, as exact_owner on the device), RATE_SUB_OWNERS per block
-        if (e[i].sub < 0) z += (e[i].cnt + 255) / 256;
+        // plain groups: every owner, 256 per block (RATE_SUB_OWNERS per block in a narrow phase); learned
+        // groups: the exact subsample (runs sub, sub + stride, ... of THIN_RUN slots, as exact_owner on the
+        // device), RATE_SUB_OWNERS per block
+        if (e[i].sub < 0 && !narrow) z += (e[i].cnt + 255) / 256;
+        else if (e[i].sub < 0) v += (e[i].cnt + RATE_SUB_OWNERS - 1) / RATE_SUB_OWNERS;
         else v += (thin_rated(e[i].cnt, e[i].sub, rb.h_table[size_t(e[i].ref)].thin_stride) + RATE_SUB_OWNERS - 1)
                   / RATE_SUB_OWNERS;
         u += e[i].sub < 0 ? 0 : (e[i].cnt + MAJ_SLOTS - 1) / MAJ_SLOTS;
@@ -235,6 +243,7 @@ phase_wave upload_phase(run_buffers &rb, const std::vector<phase_entry> &e, hipS
     pw.blocks_sub = z;
     pw.blocks_maj = u;
     pw.blocks_lsub = v;
+    pw.narrow = narrow;
     return pw;
 }
 
@@ -692,6 +701,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
 {
     const int nr = int(runs.size());
     const int ng = nr*LOCAL_GROUPS;
+    const int narrow_owners = runs[0]->c.narrow_owners;
     rb.size_run(ng, nr);
     rb.h_table.clear();
     for (op_run *r : runs)
@@ -748,7 +758,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.raw_rate.p, sizeof(gdx_raw_rate)*COL_BIN_S*ng);
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
             z.add(rb.nchanged.p, sizeof(int)*CHANGED_LISTS);
-            phase_wave pw = upload_phase(rb, e, st, z);
+            phase_wave pw = upload_phase(rb, e, st, z, narrow_owners);
             timer.time(GDX_COL_KERNEL_PUBLISH, pw.w256.total, [&] { launch_publish(st, pw.w1024, pw.blocks1024, rb); });
             if (track && pw.blocks1024)
                 timer.time(GDX_COL_KERNEL_STALE, pw.w256.total, [&] { launch_mark_stale(st, pw.w1024, rb); });
@@ -756,7 +766,7 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
                 timer.time(GDX_COL_KERNEL_MAJORANT, pw.w256.total, [&] { launch_majorant(st, pw.wmaj, pw.blocks_maj); });
             timer.time(GDX_COL_KERNEL_RATE, pw.w256.total, [&] {
                 launch_bath_rate(st, pw.wsub, pw.blocks_sub);
-                launch_bath_rate_sub(st, pw.wlsub, pw.blocks_lsub);
+                launch_bath_rate_grouped(st, pw.wlsub, pw.blocks_lsub);
             });
             rb.read_back(rb.off_rate, rb.off_err, st);             // rate bins and wave counters
             wait_stream(st, "rating pass");
@@ -778,8 +788,11 @@ void run_swarms(std::vector<op_run *> &runs, run_buffers &rb, hipStream_t st, ke
             z.add(rb.hcount.p, sizeof(int)*ng);
             z.add(rb.ccount.p, sizeof(int)*ng);
             z.add(rb.wave_ctr.p, sizeof(unsigned long long)*GDX_WAVE_CTRS*nr);
-            phase_wave pw = upload_phase(rb, e, st, z);
-            timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] { launch_screen(st, pw.w1024, pw.blocks1024); });
+            phase_wave pw = upload_phase(rb, e, st, z, narrow_owners);
+            timer.time(GDX_COL_KERNEL_SCREEN, pw.w256.total, [&] {
+                if (pw.narrow) launch_screen(st, pw.w256, pw.blocks256, true);
+                else launch_screen(st, pw.w1024, pw.blocks1024, false);
+            });
             if (pw.w256.learned && pw.blocks256)
                 timer.time(GDX_COL_KERNEL_CANDIDATES, pw.w256.total, [&] { launch_candidates(st, pw.w256, pw.blocks256, rb.cprefix.p); });
             if (pw.blocks256)
@@ -1374,10 +1387,11 @@ int gdx_col_rate_pass(gdx_col_handle h)
         zero_ranges z;
         z.add(rb.raw_rate.p, sizeof(gdx_raw_rate)*RAW_BINS);
         z.add(rb.nchanged.p, sizeof(int)*CHANGED_LISTS);
-        phase_wave pw = upload_phase(rb, e, c.stream, z);
+        phase_wave pw = upload_phase(rb, e, c.stream, z, c.narrow_owners);
         launch_publish(c.stream, pw.w1024, pw.blocks1024, rb);
         if (c.model && c.rev_valid && pw.blocks1024) launch_mark_stale(c.stream, pw.w1024, rb);
         launch_bath_rate(c.stream, pw.wsub, pw.blocks_sub);
+        launch_bath_rate_grouped(c.stream, pw.wlsub, pw.blocks_lsub);
         hip_check(hipStreamSynchronize(c.stream), "rating pass");
     });
 }
@@ -1530,6 +1544,15 @@ int gdx_col_set_fixed_bath(gdx_col_handle h, double duration)
     return guard(&h->c, [&] {
         require(std::isfinite(duration) && duration >= 0.0, GDX_COL_ERR_ARG, "fixed bath duration must be >= 0");
         h->c.fixed_bath = duration;
+    });
+}
+
+int gdx_col_set_narrow_waves(gdx_col_handle h, int owners)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] {
+        require(owners >= 0, GDX_COL_ERR_ARG, "narrow-wave owner limit must be >= 0");
+        h->c.narrow_owners = owners;
     });
 }
 

@@ -254,6 +254,8 @@ struct context
     int last_waves = 0;        // refresh waves of the last operator call (ensemble load balance)
     int topk_tile = 16;        // top-K merge tile: 0 insertion, 8/16/32 keys per merge (gdx_col_set_topk_tile, autotuner)
     int topk_waves = 1;        // top-K block shape: 1, 2 or 4 wavefronts per block (gdx_col_set_topk_block, autotuner)
+    int narrow_owners = GDX_COL_NARROW_OWNERS;   // waves of at most this many owners take the narrow
+                                                 // rate and screen launches (gdx_col_set_narrow_waves)
     hipStream_t stream = nullptr;
     bound_state user;
 
@@ -361,12 +363,15 @@ void kd_query(context &c);
 void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng);
 void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb);
 void launch_mark_stale(hipStream_t st, const gdx_wave &w, run_buffers &rb);
-// plain swarms' groups (block starts of 256 owners), learned swarms' exact subsamples (block starts of
-// RATE_SUB_OWNERS owners)
+// rate pass: one lane per owner (block starts of 256 owners) for plain swarms' groups in large waves; a
+// group of lanes per owner (block starts of RATE_SUB_OWNERS owners) for learned swarms' exact subsamples
+// and for plain groups in waves of at most context::narrow_owners plain owners
 void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks);
 constexpr int RATE_SUB_OWNERS = 64;
-void launch_bath_rate_sub(hipStream_t st, const gdx_wave &w, int blocks);
-void launch_screen(hipStream_t st, const gdx_wave &w, int blocks);
+void launch_bath_rate_grouped(hipStream_t st, const gdx_wave &w, int blocks);
+// narrow: one owner per lane over block starts of 256 owners (waves of at most context::narrow_owners
+// owners), else four per lane over block starts of 1024
+void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, bool narrow);
 void launch_candidates(hipStream_t st, const gdx_wave &w, int blocks, int *vstart);
 void launch_queue_prefix(hipStream_t st, const gdx_wave &w, int *qprefix);
 void launch_majorant(hipStream_t st, const gdx_wave &w, int blocks);

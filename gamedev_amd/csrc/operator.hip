@@ -569,17 +569,18 @@ __device__ __forceinline__ rate_sums group_rate_sums(const gdx_gref *GP, int s, 
     return rate_sums{total, JM, V};
 }
 
-// LANES = 1: the owners of plain swarms' groups, one lane each. LANES = RATE_LANES: the exact subsample
-// of learned swarms' groups, a lane group per owner (the subsample is a small fraction of the due
-// owners, too few wavefronts to hide the serial pair loop of one lane per owner). Each launch has its
-// own block starts, so that a group is rated by exactly one of the two.
+// LANES = 1: the owners of plain swarms' groups, one lane each. LANES = RATE_LANES: a lane group per
+// owner, for the exact subsample of learned swarms' groups and for plain groups in waves with few plain
+// owners (in both cases too few wavefronts to hide the serial pair loop of one lane per owner; the
+// group's lanes fold their positions in the same order, so the sums are the same bits). Each launch has
+// its own block starts, so that a group is rated by exactly one of the two.
 constexpr int RATE_LANES = 4;
-static_assert(BLOCK / RATE_LANES == RATE_SUB_OWNERS, "host block starts of the subsample launch");
+static_assert(BLOCK / RATE_LANES == RATE_SUB_OWNERS, "host block starts of the lane-group launch");
 
 template <int LANES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_bath_rate(gdx_wave w)
 {
-    constexpr bool SUB = LANES > 1;
+    constexpr bool GROUPED = LANES > 1;
     constexpr int OWNERS = BLOCK / LANES;
     __shared__ double accd[COL_BIN_S][4];     // mass, weighted rate, change, second
     __shared__ int acci[COL_BIN_S][3];        // count, owner count, invalid count
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     const int lane = threadIdx.x % WAVE;
     const int g = lane % LANES;                         // lane within the owner's group
     const int i = (b - w.bstart[gi])*OWNERS + int(threadIdx.x) / LANES;
-    const int sub = SUB ? w.sub[gi] : -1;
+    const int sub = GROUPED ? w.sub[gi] : -1;
     const gdx_gref *GP = w.table + w.ref[gi];
     const double2 *st = GP->st;
     const double *measure = GP->measure;
@@ -639,9 +640,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         if (threadIdx.x == 0 && xcd_block() == w.bstart[gi]) { *G.lower = lo; *G.upper = up; }
     }
 
-    if constexpr (SUB)
+    if constexpr (GROUPED)
     {
-        const bool check = lead && live && V;
+        // only a learned group's subsample has a bound to check
+        const bool check = sub >= 0 && lead && live && V;
         const bool viol = check && !(R <= GP->safety*GP->lam[s] + GP->lam_add[s]);
         const unsigned nc = __popcll(__ballot(check)), nv = __popcll(__ballot(viol));
         if (lane == 0 && nc) atomicAdd(&checked[0], nc);
@@ -669,7 +671,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         }
     }
     __syncthreads();
-    if constexpr (SUB)
+    if constexpr (GROUPED)
     {
         if (threadIdx.x == 255 && checked[0])
         {
@@ -975,21 +977,23 @@ __device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int 
     }
 }
 
-// screen (col_skip_scan.cu:13-59): one lane per owner, SCREEN_ROUNDS owners per lane (a block covers
-// SCREEN_OWNERS slots, so that the block's fixed latencies - group search, counter atomics, audit flush
-// - are paid once per 1024 owners). Owners with no event in the bath finish here and enter the audit
-// sums; the rest are appended to their group's queue region [beg, beg + cnt).
+// screen (col_skip_scan.cu:13-59): one lane per owner, ROUNDS owners per lane. In large waves a block
+// covers SCREEN_OWNERS slots (SCREEN_ROUNDS per lane), so that the block's fixed latencies - group
+// search, counter atomics, audit flush - are paid once per 1024 owners; in waves with few owners one
+// round per lane spreads them over four times the blocks. Owners with no event in the bath finish here
+// and enter the audit sums; the rest are appended to their group's queue region [beg, beg + cnt).
 constexpr int SCREEN_ROUNDS = 4;
 constexpr int SCREEN_OWNERS = BLOCK*SCREEN_ROUNDS;
 static_assert(CHANGED_LISTS == XCDS, "one changed-slot list per XCD");
 static_assert(SCREEN_OWNERS == PUBLISH_OWNERS, "the screen and the publish share the 1024-owner block starts");
 
+template <int ROUNDS>
 __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
 {
     __shared__ audit_lds a;
     __shared__ int block_tot[3], block_base[3];
     int gi, s0, s1;
-    if (!due_block(w, SCREEN_OWNERS, gi, s0, s1)) return;
+    if (!due_block(w, BLOCK*ROUNDS, gi, s0, s1)) return;
     const gdx_gref &G = w.table[w.ref[gi]];
     // the group's arrays as distinct pointers, so that the rounds' loads can run ahead of the earlier
     // rounds' counter stores
@@ -1015,20 +1019,20 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     const bool learned = w.learned && w.sub[gi] >= 0;
     // per round: the owner's list (0 light queue, 1 candidates, 2 heavy queue, -1 none) and its offset
     // within the block's share of that list, and a candidate's {time, bound}
-    int list[SCREEN_ROUNDS], off[SCREEN_ROUNDS];
-    double2 cand[SCREEN_ROUNDS];
+    int list[ROUNDS], off[ROUNDS];
+    double2 cand[ROUNDS];
     unsigned n_viol = 0;
     // every round's inputs first, in one round of loads without branches (a branch around a load makes
     // the compiler wait for it there): an owner of the exact subsample (or of a plain swarm) reads its
     // cached sums and its record, a thinned owner its bound (lam, lam_add); the loads an owner does not
     // need read the block's first slot instead, one cache line for the whole block
-    double4 in[SCREEN_ROUNDS];
-    double2 bnd[SCREEN_ROUNDS], rec[SCREEN_ROUNDS];
-    double mes[SCREEN_ROUNDS];
-    uint64_t ctr0[SCREEN_ROUNDS];
-    int pid[SCREEN_ROUNDS];
+    double4 in[ROUNDS];
+    double2 bnd[ROUNDS], rec[ROUNDS];
+    double mes[ROUNDS];
+    uint64_t ctr0[ROUNDS];
+    int pid[ROUNDS];
 #pragma unroll
-    for (int k = 0; k < SCREEN_ROUNDS; ++k)
+    for (int k = 0; k < ROUNDS; ++k)
     {
         const int s = s0 + k*BLOCK + int(threadIdx.x);
         const int sv = s < s1 ? s : s0;
@@ -1041,7 +1045,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
         bnd[k] = make_double2(lam[thinned ? sv : s0], lam_add[thinned ? sv : s0]);
     }
 #pragma unroll
-    for (int k = 0; k < SCREEN_ROUNDS; ++k)
+    for (int k = 0; k < ROUNDS; ++k)
     {
         const int s = s0 + k*BLOCK + int(threadIdx.x);
         const bool valid = s < s1;
@@ -1152,7 +1156,7 @@ __global__ __launch_bounds__(BLOCK) void k_screen(gdx_wave w)
     if (threadIdx.x == 2*WAVE) block_base[2] = block_tot[2] ? atomicAdd(G.hcount, block_tot[2]) : 0;
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < SCREEN_ROUNDS; ++k)
+    for (int k = 0; k < ROUNDS; ++k)
     {
         const int s = s0 + k*BLOCK + int(threadIdx.x);
         if (list[k] == 0 || list[k] == 2)
@@ -2067,11 +2071,11 @@ void launch_bath_rate(hipStream_t st, const gdx_wave &w, int blocks)
     hip_check(hipGetLastError(), "launch bath rate");
 }
 
-void launch_bath_rate_sub(hipStream_t st, const gdx_wave &w, int blocks)
+void launch_bath_rate_grouped(hipStream_t st, const gdx_wave &w, int blocks)
 {
     if (!blocks) return;
     k_bath_rate<RATE_LANES><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
-    hip_check(hipGetLastError(), "launch subsample rate");
+    hip_check(hipGetLastError(), "launch lane-group rate");
 }
 
 __global__ __launch_bounds__(256) void k_event_sum(const gdx_event_work *w, int n, gdx_event_work *sum)
@@ -2199,10 +2203,11 @@ void launch_sample(context &c, const gdx_gref &G, float *feat, double *scale, do
     hip_check(hipGetLastError(), "launch sample");
 }
 
-void launch_screen(hipStream_t st, const gdx_wave &w, int blocks)
+void launch_screen(hipStream_t st, const gdx_wave &w, int blocks, bool narrow)
 {
     if (!blocks) return;
-    k_screen<<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    if (narrow) k_screen<1><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
+    else k_screen<SCREEN_ROUNDS><<<xcd_grid(blocks), BLOCK, 0, st>>>(w);
     hip_check(hipGetLastError(), "launch screen");
 }
 

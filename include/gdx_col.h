@@ -175,6 +175,13 @@ int gdx_col_reset_controller(gdx_col_handle h);
  * min(duration, horizon) instead of the controller's choice (audits still run); 0 restores the
  * controller. Two operators with the same fixed bath follow the same schedule whatever their rates. */
 int gdx_col_set_fixed_bath(gdx_col_handle h, double duration);
+/* launch shapes, no reference counterpart: refresh waves with at most `owners` due owners rate each
+ * plain owner with a group of four lanes and screen one owner per lane, so that the few owners of a small
+ * wave spread over the whole GPU; larger waves take one lane per owner and four owners per lane
+ * (default GDX_COL_NARROW_OWNERS; 0 always takes the wide shapes, < 0 is rejected). In a batch or an
+ * ensemble the first handle's value applies. Results do not depend on it. */
+#define GDX_COL_NARROW_OWNERS 65536
+int gdx_col_set_narrow_waves(gdx_col_handle h, int owners);
 int gdx_col_synchronize(gdx_col_handle h);
 
 /* ---- diagnostics (handles created with GDX_COL_DIAGNOSTICS) ---- */
