@@ -1810,29 +1810,34 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                     j1 += J1*wait;
                     j2 += J2*wait;
                     jmax = fmax(jmax, JM);
-                    // partner by the cumulative walk in slot order, last positive slot on roundoff
+                    // partner by the cumulative walk in slot order, last positive slot on roundoff. Lane
+                    // kl of the segment adds the rates of slots 0..kl in slot order and zeros for the
+                    // later slots: the walk's own additions, since every rate here is finite and >= 0
+                    // (an invalid one ended the chain above) and adding a zero to the non-negative
+                    // cumulative changes no bit. Its partial is the walk's cumulative at slot kl; the
+                    // walk stops at the first positive slot whose partial exceeds the target. Only the
+                    // additions are serial; the per-slot tests are one ballot instead of a dependent
+                    // compare and select at every slot.
                     double target = draw_u()*R;
                     walk[wave][seg][slot_k] = p.rate;
                     __builtin_amdgcn_wave_barrier();
                     const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
+                    const double own_rate = walk[wave][seg][kl];
+                    // the lane's slot, opaque here: each slot's lane test stays one compare inside the
+                    // walk instead of a mask kept across the event loop
+                    int upto = kl;
+                    asm volatile("" : "+v"(upto));
                     double cumulative = 0.0;
-                    int slot = -1;
-#pragma unroll 4
+#pragma unroll
                     for (int k = 0; k < N_K / 2; ++k)
                     {
-                        double2 rk = wr[k];
-                        if (rk.x > 0.0)
-                        {
-                            cumulative += rk.x;
-                            if (slot < 0 && cumulative > target) slot = 2*k;
-                        }
-                        if (rk.y > 0.0)
-                        {
-                            cumulative += rk.y;
-                            if (slot < 0 && cumulative > target) slot = 2*k + 1;
-                        }
+                        const double2 rk = wr[k];
+                        cumulative += 2*k <= upto ? rk.x : 0.0;
+                        cumulative += 2*k + 1 <= upto ? rk.y : 0.0;
                     }
                     __builtin_amdgcn_wave_barrier();
+                    const unsigned long long hit = __ballot(own_rate > 0.0 && cumulative > target) & seg_mask;
+                    int slot = hit ? int(__builtin_ctzll(hit)) - segbase : -1;
                     if (slot < 0) slot = LAST;
                     if (slot < 0) { code = 5; fin = true; }
                     else
