@@ -468,9 +468,9 @@ GDX_FN int outcome_category(double si, double sj, bool high_speed, double u, dou
     return 4;
 }
 
-GDX_FN double sample_outcome(double si, double sj, bool high_speed, double u)
+// the new owner size of one event from the pair's mass ratio q = (sj / si)^3
+GDX_FN double sample_outcome_q(double si, double sj, double q, bool high_speed, double u)
 {
-    double q = mass_ratio(si, sj);
     double Gp = sticking_packet(q);
     if (!high_speed) return cbrt(si*si*si + Gp*sj*sj*sj);
     if (q <= 0.1)
@@ -482,6 +482,11 @@ GDX_FN double sample_outcome(double si, double sj, bool high_speed, double u)
     double lower = sqrt(INIT_SMIN);
     double root = lower + u*(sqrt(si) - lower);
     return fmin(si, fmax(INIT_SMIN, root*root));
+}
+
+GDX_FN double sample_outcome(double si, double sj, bool high_speed, double u)
+{
+    return sample_outcome_q(si, sj, mass_ratio(si, sj), high_speed, u);
 }
 
 // RN(a/b) from y = RN(1/b): q = a y, r = a - b q (exact with one FMA), q' = RN(q + r y) is the
@@ -558,6 +563,14 @@ GDX_FN owner_terms make_owner(const gdx_env &e, double size_i, double m, double 
     return t;
 }
 
+// sample_outcome with the mass ratio's division by the owner's size taken through its held reciprocal
+// (the correctly rounded quotient, as div_rn gives it): the same size bit for bit
+GDX_FN double sample_outcome_owner(const owner_terms &t, double sj, bool high_speed, double u)
+{
+    double ratio = div_rn(sj, t.size_i, t.inv_size);
+    return sample_outcome_q(t.size_i, sj, ratio*ratio*ratio, high_speed, u);
+}
+
 
 GDX_FN double vrel_pair_terms(const owner_terms &t, double size_j)
 {
@@ -631,6 +644,14 @@ struct cat_moments
     GDX_FN void get(double &a, double &b, double &c) const { a = mean; b = second; c = maximum; }
 };
 
+// zeros in place of the catastrophic moments: a caller that fills them in afterwards for the pairs that
+// took the catastrophic branch (catastrophic_pair), where the terms are rate x moment with the rate
+// unchanged (the branch's rate factor is 1)
+struct cat_moments_deferred
+{
+    GDX_FN void get(double &a, double &b, double &c) const { a = 0.0; b = 0.0; c = 0.0; }
+};
+
 // the same moments read from shared memory only where a lane needs them (volatile: neither
 // forwarded into registers nor hoisted out of the pair loop)
 struct cat_moments_lds
@@ -657,6 +678,15 @@ GDX_FN cat_moments catastrophic_moments(double si)
     }
     c.maximum = 2.0*L;
     return c;
+}
+
+// whether outcome_moments_terms takes its catastrophic branch for this partner: the same ratio, the
+// same tests in the same order (a NaN ratio falls through to it as there)
+GDX_FN bool catastrophic_pair(const owner_terms &t, double sj, bool high_speed)
+{
+    double ratio = div_rn(sj, t.size_i, t.inv_size);
+    double q = ratio*ratio*ratio;
+    return high_speed && !(q <= 0.1);
 }
 
 // _erosion_outcome_moments with owner-uniform and constant divisors (_col_event.cuh:31-74). The

@@ -1625,7 +1625,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
     unsigned long long *ctr_of = nullptr;     // the swarm whose counters the accumulators hold
     unsigned long long *wave_of = nullptr;
     owner_terms t{};
-    cat_moments cm{0.0, 0.0, 0.0};
     partner q{0.0, 0.0, 0.0, false};
     partner_terms pt{0.0, 0.0, 0.0, 0.0};
     // the owner's next SEG draws, one per lane of the segment (lane kl: draw pool_base + kl, and its
@@ -1728,8 +1727,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                 t = make_owner(e, cur.x, G->measure[s], hg_i);
                 refill(ctr);
                 moments = exact_owner(w, gi, s);
-                // the catastrophic moments enter only the jump moments
-                if (moments) cm = catastrophic_moments(cur.x);
                 // the partners are the published states: fixed for the whole chain, and so are their
                 // partner-only factors
                 q = load_partner(G->nbr[(long long)s*N_K + kl], G->res, G->hgs);
@@ -1762,8 +1759,25 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
         // an event takes at most three draws
         if (pool_used > SEG - 3) refill(pool_base + uint64_t(pool_used));
         // owners outside a learned swarm's audited subsample keep no compensators: rates alone
-        const pair_term p = moments ? eval_pair_terms<cat_moments, HOLD_PARTNER>(t, cm, q, lambda_0, pt)
-                                    : eval_pair_rate<HOLD_PARTNER>(t, q, lambda_0, pt);
+        pair_term p = moments ? eval_pair_terms<cat_moments_deferred, HOLD_PARTNER>(t, cat_moments_deferred{}, q,
+                                                                                    lambda_0, pt)
+                              : eval_pair_rate<HOLD_PARTNER>(t, q, lambda_0, pt);
+        // the catastrophic moments (owner-size logarithm and exponentials) only at events where some
+        // partner of the owner is in the catastrophic regime, filled in as the pair evaluation forms them
+        if constexpr (CAN_FRAGMENT)
+        {
+            const bool cat = moments && q.valid && catastrophic_pair(t, q.size_j, p.vrel >= V_FRAG);
+            if (__ballot(cat) & seg_mask)
+            {
+                const cat_moments c = catastrophic_moments(t.size_i);
+                if (cat)
+                {
+                    p.j1 = p.rate*c.mean;
+                    p.j2 = p.rate*c.second;
+                    p.jmax = p.rate > 0.0 ? c.maximum : 0.0;
+                }
+            }
+        }
         const bool bad_rate = !isfinite(p.rate) || p.rate < 0.0;
         double R = seg_fold<false>(p.rate);
         // the jump moments only where a segment of the wavefront keeps compensators
@@ -1849,7 +1863,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                         double mass_before = numr_i*size_i*size_i*size_i;
                         bool high_speed = vrel >= V_FRAG;
                         double sample = high_speed ? draw_u() : 0.0;
-                        double size_new = sample_outcome(size_i, size_j, high_speed, sample);
+                        double size_new = sample_outcome_owner(t, size_j, high_speed, sample);
                         if (kl == 0 && G->evw)
                         {
                             double log_mass;
@@ -1863,7 +1877,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                         double mass_after = numr_i*size_new*size_new*size_new;
                         double mass_scale = fmax(fabs(mass_before), fabs(mass_after));
                         owner_resize(t, size_new);
-                        if (moments) cm = catastrophic_moments(size_new);
                         if (!isfinite(size_new) || !isfinite(numr_i) || !(size_new > 0.0) || !(numr_i > 0.0)
                             || fabs(mass_after - mass_before) > 2.0e-12*mass_scale)
                         { code = 6; fin = true; }

@@ -129,7 +129,7 @@ int main(int argc, char **argv)
     }
     long long trials = argc > 1 ? std::atoll(argv[1]) : 1000000;
     lcg g{argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1ull};
-    long long bad[7] = {0, 0, 0, 0, 0, 0, 0}, reg[6] = {0, 0, 0, 0, 0, 0}, branch[3] = {0, 0, 0};
+    long long bad[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, reg[6] = {0, 0, 0, 0, 0, 0}, branch[3] = {0, 0, 0};
     for (long long n = 0; n < trials; ++n)
     {
         double y = Y_MIN + (Y_MAX - Y_MIN)*g.next();
@@ -173,6 +173,34 @@ int main(int argc, char **argv)
                 bad[6]++;
             }
 
+        // the chain's deferred catastrophic moments: zeros in the evaluation, the moments filled in for
+        // the pairs that took the catastrophic branch
+        pair_term d = eval_pair_terms(t, cat_moments_deferred{}, q, lambda_0);
+        if (catastrophic_pair(t, size_j, d.vrel >= V_FRAG))
+        {
+            d.j1 = d.rate*cm.mean;
+            d.j2 = d.rate*cm.second;
+            d.jmax = d.rate > 0.0 ? cm.maximum : 0.0;
+        }
+        const double dv[5] = {d.rate, d.j1, d.j2, d.jmax, d.vrel}, dw[5] = {b.rate, b.j1, b.j2, b.jmax, b.vrel};
+        for (int f = 0; f < 5; ++f)
+            if (bits(dv[f]) != bits(dw[f]) && !(std::isnan(dv[f]) && std::isnan(dw[f])))
+            {
+                if (bad[7] < 3) std::printf("deferred-moment mismatch field %d: size_i %.17g size_j %.17g\n", f, size_i, size_j);
+                bad[7]++;
+            }
+        // the chain's outcome through the owner's held reciprocal: the same new size
+        const double u = g.next();
+        for (int hs = 0; hs < 2; ++hs)
+        {
+            const double s1 = sample_outcome(size_i, size_j, hs == 1, u), s2 = sample_outcome_owner(t, size_j, hs == 1, u);
+            if (bits(s1) != bits(s2) && !(std::isnan(s1) && std::isnan(s2)))
+            {
+                if (bad[8] < 3) std::printf("outcome mismatch: size_i %.17g size_j %.17g u %.17g\n", size_i, size_j, u);
+                bad[8]++;
+            }
+        }
+
         if constexpr (COAG_KERNEL == 3) reg[regime(e, size_i, size_j)]++;
         double ratio = size_j / size_i;
         bool high = a.vrel >= V_FRAG;
@@ -192,7 +220,7 @@ int main(int argc, char **argv)
     std::printf("model %s trials %lld\n", GDX_MODEL_NAME, trials);
     std::printf("regimes %lld %lld %lld %lld %lld %lld\n", reg[0], reg[1], reg[2], reg[3], reg[4], reg[5]);
     std::printf("branches %lld %lld %lld\n", branch[0], branch[1], branch[2]);
-    std::printf("mismatches %lld %lld %lld %lld %lld %lld %lld\n", bad[0], bad[1], bad[2], bad[3], bad[4], bad[5],
-                bad[6]);
-    return (bad[0] | bad[1] | bad[2] | bad[3] | bad[4] | bad[5] | bad[6]) ? 1 : 0;
+    std::printf("mismatches %lld %lld %lld %lld %lld %lld %lld %lld %lld\n", bad[0], bad[1], bad[2], bad[3], bad[4],
+                bad[5], bad[6], bad[7], bad[8]);
+    return (bad[0] | bad[1] | bad[2] | bad[3] | bad[4] | bad[5] | bad[6] | bad[7] | bad[8]) ? 1 : 0;
 }

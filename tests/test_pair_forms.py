@@ -2,8 +2,9 @@
 divisors, turbulence regimes 4-5 merged, one shared log1p, catastrophic moments per owner) equals the
 direct per-pair forms bit for bit: a host build of the same device header, run over random owners and
 partners that visit every turbulence regime and outcome branch (gamedev_amd/csrc/pair_forms_check.cpp).
-The chain's held partner factors (taken at the bath start, kept across the owner's resize) give the same
-terms bit for bit."""
+The chain's held partner factors (taken at the bath start, kept across the owner's resize), its deferred
+catastrophic moments (filled in only for the pairs in that branch) and its outcome sampled through the
+owner's held reciprocal give the same terms and sizes bit for bit."""
 import subprocess
 
 import pytest
@@ -18,7 +19,7 @@ def test_hoisted_pair_forms_bit_exact(model, trials):
     out = subprocess.run([str(exe), str(trials), "7"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = dict(l.split(" ", 1) for l in out.stdout.strip().splitlines() if " " in l)
-    assert lines["mismatches"].split() == ["0"]*7
+    assert lines["mismatches"].split() == ["0"]*9
     if model == "fiducial_3d":
         # every turbulence regime and every outcome branch was exercised
         assert all(int(v) > 1000 for v in lines["regimes"].split())
