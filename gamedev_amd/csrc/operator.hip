@@ -1639,9 +1639,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
         pool_u = col_uniform_at(pidx, seed, base + uint64_t(kl));
         pool_nl = -log(pool_u);
     };
-    // segment-uniform calls only: the source lane is in the caller's segment
-    auto draw_u = [&]() { const double v = __shfl(pool_u, segbase + pool_used, WAVE); ++pool_used; return v; };
-    auto draw_nl = [&]() { const double v = __shfl(pool_nl, segbase + pool_used, WAVE); ++pool_used; return v; };
     // counters are flushed when a segment moves to an owner of another swarm, and at the end
     auto flush = [&]() {
         if (kl == 0 && ctr_of)
@@ -1758,6 +1755,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
         if (!active) continue;
         // an event takes at most three draws
         if (pool_used > SEG - 3) refill(pool_base + uint64_t(pool_used));
+        // the event's next draws (a wait, the partner's target, a fragmentation sample), fetched from
+        // their lanes of the segment while the pair terms are evaluated and taken below in stream order
+        const double nx_nl = __shfl(pool_nl, segbase + pool_used, WAVE);
+        const double nx_u0 = __shfl(pool_u, segbase + pool_used, WAVE);
+        const double nx_u1 = __shfl(pool_u, segbase + pool_used + 1, WAVE);
+        const double nx_u2 = __shfl(pool_u, segbase + pool_used + 2, WAVE);
         // owners outside a learned swarm's audited subsample keep no compensators: rates alone
         pair_term p = moments ? eval_pair_terms<cat_moments_deferred, HOLD_PARTNER>(t, cat_moments_deferred{}, q,
                                                                                     lambda_0, pt)
@@ -1802,7 +1805,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
         else if (R == 0.0) { time = end; fin = true; }
         else
         {
-            double wait = now ? 0.0 : draw_nl() / R;
+            double wait = 0.0;
+            if (!now)
+            {
+                wait = nx_nl / R;
+                ++pool_used;
+            }
             double remaining = end - time;
             if (!now && (!isfinite(wait) || !(wait > 0.0))) { code = 3; fin = true; }
             else if (!now && wait >= remaining)
@@ -1832,7 +1840,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                     // walk stops at the first positive slot whose partial exceeds the target. Only the
                     // additions are serial; the per-slot tests are one ballot instead of a dependent
                     // compare and select at every slot.
-                    double target = draw_u()*R;
+                    double target = (now ? nx_u0 : nx_u1)*R;
+                    ++pool_used;
                     walk[wave][seg][slot_k] = p.rate;
                     __builtin_amdgcn_wave_barrier();
                     const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
@@ -1862,7 +1871,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                         double vrel = __shfl(p.vrel, segbase + bitrev_slot(slot), WAVE);
                         double mass_before = numr_i*size_i*size_i*size_i;
                         bool high_speed = vrel >= V_FRAG;
-                        double sample = high_speed ? draw_u() : 0.0;
+                        double sample = 0.0;
+                        if (high_speed)
+                        {
+                            sample = now ? nx_u1 : nx_u2;
+                            ++pool_used;
+                        }
                         double size_new = sample_outcome_owner(t, size_j, high_speed, sample);
                         if (kl == 0 && G->evw)
                         {
