@@ -859,14 +859,18 @@ __global__ __launch_bounds__(256) void k_slot_maps(const int *order, const unsig
     }
 }
 
+// the state in slot order, published as it is (the operator's publish writes only slots changed since)
 __global__ __launch_bounds__(256) void k_gather_state(bound_state u, const int *slot_pidx, int n_active,
-                                                     const uint64_t *rng_p, double2 *st, uint64_t *rng, gdx_env *env,
-                                                     double *hgs)
+                                                     const uint64_t *rng_p, double2 *st, double2 *res, float2 *res32,
+                                                     uint64_t *rng, gdx_env *env, double *hgs)
 {
     long long s = (long long)blockIdx.x*blockDim.x + threadIdx.x;
     if (s >= n_active) return;
     int p = slot_pidx[s];
-    st[s] = make_double2(ld(u.size, p), ld(u.numr, p));
+    const double2 v = make_double2(ld(u.size, p), ld(u.numr, p));
+    st[s] = v;
+    res[s] = v;
+    res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
     rng[s] = rng_p[p];
     if constexpr (COAG_KERNEL == 3)
     {
@@ -1235,7 +1239,7 @@ void load_state(context &c)
     if (c.n_active > 0)
     {
         k_gather_state<<<blocks_for(c.n_active, 256), 256, 0, st>>>(c.user, c.slot_pidx.p, c.n_active, c.rng_p.p, c.st.p,
-                                                                    c.rng.p, c.env.p, c.hgs.p);
+                                                                    c.res.p, c.res32.p, c.rng.p, c.env.p, c.hgs.p);
         hip_check(hipGetLastError(), "launch gather state");
     }
     // a new state: no kept bound is valid, no publish is pending

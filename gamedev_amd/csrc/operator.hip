@@ -195,8 +195,12 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
     if (threadIdx.x == 0) count = 0;
     __syncthreads();
     unsigned long long lo = ~0ull, hi = 0ull;
-    // every round's record, flag and replaced record in one round of loads, before any store (a flat
-    // pointer could alias the LDS list, which would keep each round's loads behind the last store)
+    // Only slots whose state changed since their last publish are written: a clean slot's published
+    // records already equal its state (the state load writes both, and every state change marks the
+    // slot dirty). Every round's record and flag in one round of loads, then the records replaced by the
+    // dirty slots (clean slots read the block's first slot instead, one cache line), all before any
+    // store (a flat pointer could alias the LDS list, which would keep each round's loads behind the
+    // last store).
     constexpr int ROUNDS = PUBLISH_OWNERS / BLOCK;
     double2 vs[ROUNDS], rs[ROUNDS];
     unsigned char ds[ROUNDS];
@@ -207,7 +211,12 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
         const int sv = s < s1 ? s : s0;
         vs[k] = st[sv];
         ds[k] = dirty[sv];
-        rs[k] = res[sv];
+    }
+#pragma unroll
+    for (int k = 0; k < ROUNDS; ++k)
+    {
+        const int s = s0 + k*BLOCK + int(threadIdx.x);
+        rs[k] = res[(s < s1 && ds[k]) ? s : s0];
     }
 #pragma unroll
     for (int k = 0; k < ROUNDS; ++k)
@@ -216,9 +225,6 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
         if (s >= s1) break;
         const double2 v = vs[k];
         const bool changed_here = ds[k];
-        const double was_x = changed_here ? rs[k].x : v.x, was_y = changed_here ? rs[k].y : v.y;
-        res[s] = v;
-        res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
         // positive doubles order like their bit patterns (col_size_scan.cu:20-24)
         if (v.x > 0 && isfinite(v.x)) { lo = min(lo, dbits(v.x)); hi = max(hi, dbits(v.x)); }
         // a changed record: the slot's own bound goes stale and every owner whose row holds it takes the
@@ -226,6 +232,9 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
         // they replaced)
         if (changed_here)
         {
+            const double was_x = rs[k].x, was_y = rs[k].y;
+            res[s] = v;
+            res32[s] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
             dirty[s] = 0;
             if (track)
             {
