@@ -136,9 +136,11 @@ struct gdx_changed
     double size, numr;
 };
 
-// changed-slot lists of one publish: the blocks on each XCD append to their own list (counter atomics
-// spread over as many addresses), each list holding up to its blocks' owners
-constexpr int CHANGED_LISTS = 8;
+// changed-slot lists of one publish: block b appends to list b % CHANGED_LISTS, so that the lists of
+// the blocks on one XCD (blocks are dealt to the XCDs in turn) are that XCD's own and each list counter
+// takes the returning atomics of a few hundred blocks, not thousands; each list holds up to its blocks'
+// owners
+constexpr int CHANGED_LISTS = 32;
 
 // due slots per block of the majorant kernel (host block starts and device share it)
 constexpr int MAJ_SLOTS = 4096;

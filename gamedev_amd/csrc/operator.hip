@@ -253,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_publish(gdx_wave w, gdx_changed *chan
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     if (lane == 0) { red[0][wave] = lo; red[1][wave] = hi; }
     __syncthreads();
-    // the blocks on one XCD share a changed-slot list
+    // the block's changed-slot list, shared with blocks of the same XCD
     const int li = int(blockIdx.x) % CHANGED_LISTS;
     if (threadIdx.x == 0)
     {
@@ -993,7 +993,7 @@ __device__ __forceinline__ void audit_wave(audit_lds &a, bool take, int bs, int 
 // and enter the audit sums; the rest are appended to their group's queue region [beg, beg + cnt).
 constexpr int SCREEN_ROUNDS = 4;
 constexpr int SCREEN_OWNERS = BLOCK*SCREEN_ROUNDS;
-static_assert(CHANGED_LISTS == XCDS, "one changed-slot list per XCD");
+static_assert(CHANGED_LISTS % XCDS == 0, "every changed-slot list belongs to the blocks of one XCD");
 static_assert(SCREEN_OWNERS == PUBLISH_OWNERS, "the screen and the publish share the 1024-owner block starts");
 
 template <int ROUNDS>
@@ -2087,10 +2087,10 @@ void launch_reset_scratch(hipStream_t st, const gdx_gref *table, int ng)
 void launch_publish(hipStream_t st, const gdx_wave &w, int blocks, run_buffers &rb)
 {
     if (!blocks) return;
-    // each list holds the owners of the blocks on its XCD: grid / CHANGED_LISTS blocks
+    // each list holds the owners of its blocks: at most ceil(grid / CHANGED_LISTS) blocks
     const int grid = xcd_grid(blocks);
-    rb.changed_cap = grid / CHANGED_LISTS*PUBLISH_OWNERS;
-    rb.changed.alloc(size_t(grid)*PUBLISH_OWNERS);
+    rb.changed_cap = (grid + CHANGED_LISTS - 1) / CHANGED_LISTS*PUBLISH_OWNERS;
+    rb.changed.alloc(size_t(CHANGED_LISTS)*size_t(rb.changed_cap));
     k_publish<<<grid, BLOCK, 0, st>>>(w, rb.changed.p, rb.changed_cap, rb.nchanged.p);
     hip_check(hipGetLastError(), "launch publish");
 }
