@@ -492,7 +492,7 @@ __device__ __forceinline__ rate_sums group_rate_sums(const gdx_gref *GP, int s, 
 {
     constexpr int P = N_K / G;                 // positions per lane
     constexpr int CP = P / 4;                  // chunks per lane
-    constexpr int LP = (P == 4) ? 2 : (P == 8) ? 3 : (P == 16) ? 4 : 5;
+    constexpr int LP = LOG2K - ((G == 1) ? 0 : (G == 2) ? 1 : (G == 4) ? 2 : 3);   // log2(P)
     static_assert(G == 1 || G == 2 || G == 4 || G == 8, "lane groups of 1, 2, 4 or 8");
     static_assert(P >= 4 && (1 << LP) == P, "at least one chunk of four positions per lane");
     const int *nbr = GP->nbr;
