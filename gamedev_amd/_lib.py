@@ -120,6 +120,7 @@ _SIGS = {
     "gdx_col_set_bound_reuse": [P, I],
     "gdx_col_set_fixed_bath": [P, D],
     "gdx_col_set_narrow_waves": [P, I],
+    "gdx_col_set_exact_walk": [P, I],
     "gdx_col_majorant_max_owners": [P],
     "gdx_col_host_schedule_adaptive": [I, D, P, P, I, P, P, P, P, P, P],
     "gdx_col_set_profile": [P, I],

@@ -231,6 +231,12 @@ class CollisionOperator:
         """Test access: every bath of later calls lasts min(duration, horizon); 0 restores the controller."""
         self._call(self.lib.gdx_col_set_fixed_bath, float(duration))
 
+    def set_exact_walk(self, on: bool) -> None:
+        """Test access: the chain picks every partner from the walk's sequential partial sums (default
+        off: tree-summed partials decide wherever provably on the same side of the target). Results do
+        not depend on it."""
+        self._call(self.lib.gdx_col_set_exact_walk, int(bool(on)))
+
     def set_narrow_waves(self, owners: int) -> None:
         """Refresh waves of at most `owners` due owners take the narrow launch shapes (four lanes per
         plain owner in the rate pass, one owner per lane in the screen); 0 always takes the wide ones.

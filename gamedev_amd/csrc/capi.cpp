@@ -113,6 +113,7 @@ gdx_gref make_gref(context &c, int g, run_buffers &rb, int run, int tbase)
     r.lam = c.lam.p;
     r.lam_add = c.lam_add.p;
     r.dirty = c.dirty.p;
+    r.exact_walk = c.exact_walk;
     const bool track = c.model && c.rev_valid;
     r.stale = track ? c.stale.p : nullptr;
     r.rev_beg = track ? c.rev_beg.p : nullptr;
@@ -1545,6 +1546,12 @@ int gdx_col_set_fixed_bath(gdx_col_handle h, double duration)
         require(std::isfinite(duration) && duration >= 0.0, GDX_COL_ERR_ARG, "fixed bath duration must be >= 0");
         h->c.fixed_bath = duration;
     });
+}
+
+int gdx_col_set_exact_walk(gdx_col_handle h, int on)
+{
+    if (!h) return GDX_COL_ERR_ARG;
+    return guard(&h->c, [&] { h->c.exact_walk = on ? 1 : 0; });
 }
 
 int gdx_col_set_narrow_waves(gdx_col_handle h, int owners)

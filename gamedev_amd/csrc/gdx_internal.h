@@ -322,6 +322,7 @@ struct context
     int thin_stride = THIN_STRIDE;   // learned: exact subsample stride (THIN_STRIDE_LARGE for large swarms)
     int majorant_min_owners = GDX_COL_MAJORANT_MIN_OWNERS;   // smaller swarms keep the plain method
     double fixed_bath = 0.0;   // > 0: every bath lasts this long (test access; the controller still audits)
+    int exact_walk = 0;        // 1: the chain's partner walk from sequential partials only (test access)
     bool diag = false;         // GDX_COL_DIAGNOSTICS
     dbuf<gdx_event_work> evw;  // per-slot event statistics of the current call
     dbuf<gdx_event_work> ev_sum;

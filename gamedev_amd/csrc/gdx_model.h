@@ -122,6 +122,7 @@ struct gdx_gref
     const unsigned *rev_beg, *rev_end;        // per slot: its range in rev
     const int *rev;                           // the owners whose partner rows hold each slot
     int bound_reuse;                          // keep the bounds of owners whose inputs are unchanged
+    int exact_walk;                           // chain partner walk from sequential partials only (test access)
     float2 *res32;                            // per slot: published {size, number} in FP32 (feature gathers)
     double2 *qstart;                          // per queue position: {start time, 1 = event at the start}
     int *cqueue;                              // thinning candidates of the bath, in the group's slot range

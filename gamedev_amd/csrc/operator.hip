@@ -141,6 +141,19 @@ __device__ __forceinline__ double seg_fold(double v)
         return lane_value(v, 63);
     }
 }
+// inclusive prefix sums over a segment's lanes in lane order (lane t: lanes 0..t of its segment), as a
+// tree: shifts by 1, 2, 4 and 8 within 16-lane rows (zeros shifted in), then row broadcasts 15 and 31
+// carry whole rows; every lane of the segment must be active
+__device__ __forceinline__ double seg_scan(double v)
+{
+    v += dpp_move<0x111, 0xf, true>(v);            // row_shr:1
+    v += dpp_move<0x112, 0xf, true>(v);            // row_shr:2
+    v += dpp_move<0x114, 0xf, true>(v);            // row_shr:4
+    v += dpp_move<0x118, 0xf, true>(v);            // row_shr:8
+    if constexpr (SEG >= 32) v += dpp_move<0x142, 0xa, false>(v);   // rows 1 and 3 take lanes 15 and 47
+    if constexpr (SEG == 64) v += dpp_move<0x143, 0xc, false>(v);   // rows 2 and 3 take lane 31
+    return v;
+}
 // segment maximum of a non-negative integer (0 fills disabled sources)
 __device__ __forceinline__ int seg_max_u(int v)
 {
@@ -1623,6 +1636,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
     // accepted thinning candidate of a learned swarm: its first event happens at its start time
     bool first_event = false;
     bool heavy_owner = false;
+    bool walk_exact = false;                  // the owner's swarm takes the sequential partner walk only
     bool moments = true;
     bool active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
@@ -1751,6 +1765,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                     if (qs.y == 2.0) events = 1;
                 }
                 heavy_owner = hv;
+                walk_exact = G->exact_walk != 0;
                 active = true;
             }
             hn += th*hstep;
@@ -1841,34 +1856,45 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                     j1 += J1*wait;
                     j2 += J2*wait;
                     jmax = fmax(jmax, JM);
-                    // partner by the cumulative walk in slot order, last positive slot on roundoff. Lane
-                    // kl of the segment adds the rates of slots 0..kl in slot order and zeros for the
-                    // later slots: the walk's own additions, since every rate here is finite and >= 0
-                    // (an invalid one ended the chain above) and adding a zero to the non-negative
-                    // cumulative changes no bit. Its partial is the walk's cumulative at slot kl; the
-                    // walk stops at the first positive slot whose partial exceeds the target. Only the
-                    // additions are serial; the per-slot tests are one ballot instead of a dependent
-                    // compare and select at every slot.
+                    // partner by the cumulative walk in slot order, last positive slot on roundoff: the
+                    // first positive slot whose partial sum exceeds the target. Lane kl takes slot kl.
                     double target = (now ? nx_u0 : nx_u1)*R;
                     ++pool_used;
                     walk[wave][seg][slot_k] = p.rate;
                     __builtin_amdgcn_wave_barrier();
-                    const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
                     const double own_rate = walk[wave][seg][kl];
-                    // the lane's slot, opaque here: each slot's lane test stays one compare inside the
-                    // walk instead of a mask kept across the event loop
-                    int upto = kl;
-                    asm volatile("" : "+v"(upto));
-                    double cumulative = 0.0;
-#pragma unroll
-                    for (int k = 0; k < N_K / 2; ++k)
+                    // The partials as a tree (seg_scan). Every rate here is finite and >= 0 (an invalid
+                    // one ended the chain above), and any order of adding at most 64 non-negative terms
+                    // lies within 64 u/(1 - 64 u) < 2^-46 relative of the exact sum, so the tree partial
+                    // and the walk's sequential one differ by less than 2^-45 of the partial: a tree
+                    // partial farther than 2^-44 (plus an absolute 2^-1000 for subnormal rounding) from
+                    // the target decides its slot's test exactly as the sequential partial does.
+                    const double a = seg_scan(own_rate);
+                    const bool above = a*(1.0 - 0x1p-44) - 0x1p-1000 > target;
+                    const bool below = a*(1.0 + 0x1p-44) + 0x1p-1000 <= target;
+                    unsigned long long hit = __ballot(own_rate > 0.0 && above) & seg_mask;
+                    if (walk_exact || (__ballot(own_rate > 0.0 && !above && !below) & seg_mask))
                     {
-                        const double2 rk = wr[k];
-                        cumulative += 2*k <= upto ? rk.x : 0.0;
-                        cumulative += 2*k + 1 <= upto ? rk.y : 0.0;
+                        // a positive slot's partial within the margin (or the swarm asks for it): the
+                        // walk's own sequential partials. Lane kl adds the rates of slots 0..kl in slot
+                        // order and zeros for the later slots (adding a zero to the non-negative
+                        // cumulative changes no bit).
+                        const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
+                        // the lane's slot, opaque here: each slot's lane test stays one compare inside
+                        // the sum instead of a mask kept across the event loop
+                        int upto = kl;
+                        asm volatile("" : "+v"(upto));
+                        double cumulative = 0.0;
+#pragma unroll
+                        for (int k = 0; k < N_K / 2; ++k)
+                        {
+                            const double2 rk = wr[k];
+                            cumulative += 2*k <= upto ? rk.x : 0.0;
+                            cumulative += 2*k + 1 <= upto ? rk.y : 0.0;
+                        }
+                        hit = __ballot(own_rate > 0.0 && cumulative > target) & seg_mask;
                     }
                     __builtin_amdgcn_wave_barrier();
-                    const unsigned long long hit = __ballot(own_rate > 0.0 && cumulative > target) & seg_mask;
                     int slot = hit ? int(__builtin_ctzll(hit)) - segbase : -1;
                     if (slot < 0) slot = LAST;
                     if (slot < 0) { code = 5; fin = true; }

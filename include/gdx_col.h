@@ -180,6 +180,11 @@ int gdx_col_set_fixed_bath(gdx_col_handle h, double duration);
  * wave spread over the whole GPU; larger waves take one lane per owner and four owners per lane
  * (default GDX_COL_NARROW_OWNERS; 0 always takes the wide shapes, < 0 is rejected). In a batch or an
  * ensemble the first handle's value applies. Results do not depend on it. */
+/* test access, no reference counterpart: on makes the chain pick every partner from the walk's
+ * sequential partial sums; off (default) decides each slot from tree-summed partials wherever they are
+ * provably on the same side of the target and falls back to the sequential ones otherwise. Results do
+ * not depend on it. */
+int gdx_col_set_exact_walk(gdx_col_handle h, int on);
 #define GDX_COL_NARROW_OWNERS 65536
 int gdx_col_set_narrow_waves(gdx_col_handle h, int owners);
 int gdx_col_synchronize(gdx_col_handle h);
