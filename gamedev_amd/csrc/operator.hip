@@ -1815,16 +1815,16 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
             J2 = seg_fold<false>(p.j2);
             JM = seg_fold<true>(p.jmax);
         }
-        // last slot with a positive rate (-1: none), -2 when any rate of the segment is invalid
-        int LAST = seg_max_u(p.rate > 0.0 ? slot_k + 1 : 0) - 1;
-        if (__ballot(bad_rate) & seg_mask) LAST = -2;
+        // an invalid rate anywhere in the segment ends the chain (the last positive slot, the walk's
+        // fallback on roundoff, is found only where the walk needs it)
+        const bool any_bad = (__ballot(bad_rate) & seg_mask) != 0ull;
         n_pairs += (kl == 0) ? N_K : 0;
         bool fin = false;
         int code = 0;
         // accepted thinning candidate: the event happens at the start time, without a wait draw
         const bool now = first_event;
         first_event = false;
-        if (LAST == -2) { code = 1; fin = true; }
+        if (any_bad) { code = 1; fin = true; }
         else if (!isfinite(R) || R < 0.0) { code = 2; fin = true; }
         else if (R == 0.0) { time = end; fin = true; }
         else
@@ -1896,7 +1896,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                     }
                     __builtin_amdgcn_wave_barrier();
                     int slot = hit ? int(__builtin_ctzll(hit)) - segbase : -1;
-                    if (slot < 0) slot = LAST;
+                    if (slot < 0) slot = seg_max_u(p.rate > 0.0 ? slot_k + 1 : 0) - 1;
                     if (slot < 0) { code = 5; fin = true; }
                     else
                     {
