@@ -1637,6 +1637,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
     bool first_event = false;
     bool heavy_owner = false;
     bool walk_exact = false;                  // the owner's swarm takes the sequential partner walk only
+    gdx_event_work *evw = nullptr;            // the owner's swarm's event statistics (null: none kept)
     bool moments = true;
     bool active = false;
     int s = 0, pos = 0, gi = 0, events = 0;
@@ -1766,6 +1767,9 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                 }
                 heavy_owner = hv;
                 walk_exact = G->exact_walk != 0;
+                // read once per owner: a load from the group table at every event would hold the
+                // event's outcome until it returned
+                evw = G->evw;
                 active = true;
             }
             hn += th*hstep;
@@ -1913,12 +1917,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                             ++pool_used;
                         }
                         double size_new = sample_outcome_owner(t, size_j, high_speed, sample);
-                        if (kl == 0 && G->evw)
+                        if (kl == 0 && evw)
                         {
                             double log_mass;
                             int cat = outcome_category(size_i, size_j, high_speed, sample, size_new, log_mass);
-                            G->evw[s].count[cat] += 1ull;
-                            G->evw[s].log_mass[cat] += log_mass;
+                            evw[s].count[cat] += 1ull;
+                            evw[s].log_mass[cat] += log_mass;
                         }
                         numr_i = mass_before / (size_new*size_new*size_new);
                         time = event_time;
