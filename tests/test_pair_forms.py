@@ -10,7 +10,7 @@ import subprocess
 import pytest
 
 
-@pytest.mark.parametrize("model,trials", [("fiducial_3d", 1_000_000), ("plain_2d", 100_000)])
+@pytest.mark.parametrize("model,trials", [("fiducial_3d", 1_000_000), ("plain_2d", 100_000), ("fiducial_k64", 200_000)])
 def test_hoisted_pair_forms_bit_exact(model, trials):
     from gamedev_amd import build
     exe = build.check_path(model)
