@@ -1833,6 +1833,46 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
         else if (R == 0.0) { time = end; fin = true; }
         else
         {
+            // partner by the cumulative walk in slot order, last positive slot on roundoff: the first
+            // positive slot whose partial sum exceeds the target. Lane kl takes slot kl. Decided before
+            // the wait is known (it needs only the rates, R and the target's draw), so that its lane
+            // sums overlap the wait's division; it is used only at an event.
+            const double target = (now ? nx_u0 : nx_u1)*R;
+            walk[wave][seg][slot_k] = p.rate;
+            __builtin_amdgcn_wave_barrier();
+            const double own_rate = walk[wave][seg][kl];
+            // The partials as a tree (seg_scan). Every rate here is finite and >= 0 (an invalid
+            // one ended the chain above), and any order of adding at most 64 non-negative terms
+            // lies within 64 u/(1 - 64 u) < 2^-46 relative of the exact sum, so the tree partial
+            // and the walk's sequential one differ by less than 2^-45 of the partial: a tree
+            // partial farther than 2^-44 (plus an absolute 2^-1000 for subnormal rounding) from
+            // the target decides its slot's test exactly as the sequential partial does.
+            const double a = seg_scan(own_rate);
+            const bool above = a*(1.0 - 0x1p-44) - 0x1p-1000 > target;
+            const bool below = a*(1.0 + 0x1p-44) + 0x1p-1000 <= target;
+            unsigned long long hit = __ballot(own_rate > 0.0 && above) & seg_mask;
+            if (walk_exact || (__ballot(own_rate > 0.0 && !above && !below) & seg_mask))
+            {
+                // a positive slot's partial within the margin (or the swarm asks for it): the
+                // walk's own sequential partials. Lane kl adds the rates of slots 0..kl in slot
+                // order and zeros for the later slots (adding a zero to the non-negative
+                // cumulative changes no bit).
+                const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
+                // the lane's slot, opaque here: each slot's lane test stays one compare inside
+                // the sum instead of a mask kept across the event loop
+                int upto = kl;
+                asm volatile("" : "+v"(upto));
+                double cumulative = 0.0;
+#pragma unroll
+                for (int k = 0; k < N_K / 2; ++k)
+                {
+                    const double2 rk = wr[k];
+                    cumulative += 2*k <= upto ? rk.x : 0.0;
+                    cumulative += 2*k + 1 <= upto ? rk.y : 0.0;
+                }
+                hit = __ballot(own_rate > 0.0 && cumulative > target) & seg_mask;
+            }
+            __builtin_amdgcn_wave_barrier();
             double wait = 0.0;
             if (!now)
             {
@@ -1860,45 +1900,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain(gdx_wave w)
                     j1 += J1*wait;
                     j2 += J2*wait;
                     jmax = fmax(jmax, JM);
-                    // partner by the cumulative walk in slot order, last positive slot on roundoff: the
-                    // first positive slot whose partial sum exceeds the target. Lane kl takes slot kl.
-                    double target = (now ? nx_u0 : nx_u1)*R;
+                    // the target's draw (the partner walk above was decided before the wait)
                     ++pool_used;
-                    walk[wave][seg][slot_k] = p.rate;
-                    __builtin_amdgcn_wave_barrier();
-                    const double own_rate = walk[wave][seg][kl];
-                    // The partials as a tree (seg_scan). Every rate here is finite and >= 0 (an invalid
-                    // one ended the chain above), and any order of adding at most 64 non-negative terms
-                    // lies within 64 u/(1 - 64 u) < 2^-46 relative of the exact sum, so the tree partial
-                    // and the walk's sequential one differ by less than 2^-45 of the partial: a tree
-                    // partial farther than 2^-44 (plus an absolute 2^-1000 for subnormal rounding) from
-                    // the target decides its slot's test exactly as the sequential partial does.
-                    const double a = seg_scan(own_rate);
-                    const bool above = a*(1.0 - 0x1p-44) - 0x1p-1000 > target;
-                    const bool below = a*(1.0 + 0x1p-44) + 0x1p-1000 <= target;
-                    unsigned long long hit = __ballot(own_rate > 0.0 && above) & seg_mask;
-                    if (walk_exact || (__ballot(own_rate > 0.0 && !above && !below) & seg_mask))
-                    {
-                        // a positive slot's partial within the margin (or the swarm asks for it): the
-                        // walk's own sequential partials. Lane kl adds the rates of slots 0..kl in slot
-                        // order and zeros for the later slots (adding a zero to the non-negative
-                        // cumulative changes no bit).
-                        const double2 *wr = reinterpret_cast<const double2 *>(&walk[wave][seg][0]);
-                        // the lane's slot, opaque here: each slot's lane test stays one compare inside
-                        // the sum instead of a mask kept across the event loop
-                        int upto = kl;
-                        asm volatile("" : "+v"(upto));
-                        double cumulative = 0.0;
-#pragma unroll
-                        for (int k = 0; k < N_K / 2; ++k)
-                        {
-                            const double2 rk = wr[k];
-                            cumulative += 2*k <= upto ? rk.x : 0.0;
-                            cumulative += 2*k + 1 <= upto ? rk.y : 0.0;
-                        }
-                        hit = __ballot(own_rate > 0.0 && cumulative > target) & seg_mask;
-                    }
-                    __builtin_amdgcn_wave_barrier();
                     int slot = hit ? int(__builtin_ctzll(hit)) - segbase : -1;
                     if (slot < 0) slot = seg_max_u(p.rate > 0.0 ? slot_k + 1 : 0) - 1;
                     if (slot < 0) { code = 5; fin = true; }
